@@ -1,0 +1,152 @@
+/*
+ * lywell: MI355X (gfx950) implementation of the MLFMAxwell hot path.
+ *
+ * C ABI called by the Fortran modules modMLFMA / modIterativo / modRCS
+ * (lywell_amd/fortran/) through ISO_C_BINDING, and by any other host code.
+ * Plain pointers and sizes only.  Conventions follow the reference's Fortran
+ * arrays so a Fortran caller passes its arrays unchanged:
+ *   - geometry arrays are column-major: (3,n) means xyz contiguous per item;
+ *   - connectivity holds 1-based indices;
+ *   - complex(8) vectors are interleaved (re, im) doubles, complex(4) vectors
+ *     interleaved floats.
+ * Every entry point returns a status code (LW_OK = 0) unless stated otherwise.
+ * The reference has no error returns; the codes replace its STOP statements
+ * and silent exits.  One handle per mesh; a handle is not reentrant.
+ */
+#ifndef LYWELL_H
+#define LYWELL_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LW_OK 0
+#define LW_ERR_ARG (-1)      /* invalid argument or size */
+#define LW_ERR_HIP (-2)      /* a HIP runtime call failed */
+#define LW_ERR_OCTREE (-3)   /* nivel_max <= 3: the reference STOPs (modMLFMA.f90:195-198) */
+#define LW_ERR_NOMEM (-4)    /* device allocation failed */
+#define LW_ERR_STATE (-5)    /* call order violated (e.g. lusol before ilu0) */
+#define LW_ERR_PIVOT (-6)    /* ilu0 zero pivot; the row is returned in *ierr */
+#define LW_ERR_NODEV (-7)    /* no HIP device visible */
+
+/* Flags for lw_mlfma_init. */
+#define LW_NEAR_FP64 1 /* store near field and ILU0 factor in complex(8)
+                          (the reference's dp_near = dp_prec = 8 variant);
+                          default keeps complex(4) storage as shipped */
+
+typedef struct lw_handle lw_handle;
+
+/* Last error message for this thread (static storage). */
+const char *lw_last_error(void);
+/* Number of HIP devices visible (0 when none). */
+int lw_device_count(void);
+
+/* ------------------------------------------------------------------ mesh
+ * Caller-side geometry helpers (the reference does this in host code:
+ * modReadStl.f90:1378-1442, modRWG.f90:8-174, 270-328).  No GPU use. */
+
+/* Icosphere sizes for a subdivision level: nt = 20*4^s, np = nt/2 + 2. */
+int lw_mesh_icosphere_size(int subdiv, int *np, int *nt);
+/* Icosphere scaled to semi-axes (ax, ay, az); t_p(3,nt) 1-based, outward
+ * unit normals t_normal(3,nt) (may be NULL). */
+int lw_mesh_icosphere(int subdiv, double ax, double ay, double az, double *p_coord,
+                      int *t_p, double *t_normal);
+/* Radial perturbation by real spherical harmonics: r *= 1 + sum c_i Y_{l_i m_i}. */
+int lw_mesh_perturb_sh(int np, double *p_coord, int ncoef, const int *deg, const int *ord,
+                       const double *coef);
+/* parametros_rwg (modRWG.f90:133-174): outputs sized for 3*nt edges.
+ * Returns the number of RWG edges (>0) or a negative status. */
+int lw_rwg_build(int np, const double *p_coord, int nt, const int *t_p, int *e_p, int *e_t,
+                 int *e_po, double *e_long, double *e_centro);
+/* hallar_geometria_rwg (modRWG.f90:270-328). */
+int lw_tri_geometry(int np, const double *p_coord, int nt, const int *t_p, double *t_area,
+                    double *t_baric, double *t_baric_sub);
+
+/* ------------------------------------------------------------------ modMLFMA */
+
+/* inicializar_MLFMA (modMLFMA.f90:113-275).  Arguments mirror the
+ * reference's dummy arguments plus lambda and conf%criterio_ladomax, which
+ * the reference reads from module globals (modGlobalParam.f90:20-43).
+ * Builds the octree and operators and uploads them to the current device. */
+int lw_mlfma_init(double lambda, double ladomax, int flags, int num_e, const double *e_centro,
+                  const int *e_t, const int *e_po, const double *e_long, int num_t,
+                  const double *t_area, const double *t_baric, const double *t_baric_sub,
+                  int num_p, const double *p_coord, const int *t_p, const int *e_p,
+                  const double *t_normal, lw_handle **out);
+/* destruir_MLFMA (modMLFMA.f90:85-110). */
+int lw_mlfma_destroy(lw_handle *h);
+
+/* Octree shape: levels 1..nlevels as in the reference. */
+int lw_mlfma_nlevels(lw_handle *h);
+int lw_mlfma_level_info(lw_handle *h, int level, int *nboxes, int *nmult, int *nth, int *nph,
+                        int *ntransf, long long *npairs);
+
+/* MatxVec_MLFMA (modMLFMA.f90:1901-1946): y = Z x, host buffers
+ * (complex(8)[num_e] each, interleaved). */
+int lw_matvec(lw_handle *h, const double *x, double *y);
+/* Same product on device buffers on the handle's stream; no host sync. */
+int lw_matvec_dev(lw_handle *h, const void *x_dev, void *y_dev);
+/* Far-field part only / near-field part only (per-stage checks). */
+int lw_matvec_parts(lw_handle *h, const double *x, double *y_far, double *y_near);
+
+/* n_esparcida_MLFMA and the CSR arrays Zesparcida_MLFMA / ia_MLFMA /
+ * ja_MLFMA (modMLFMA.f90:30-32), copied to host.  z is complex(4)
+ * interleaved (float[2*nnz]) unless the handle was built with LW_NEAR_FP64,
+ * in which case use lw_near_csr64. */
+long long lw_near_nnz(lw_handle *h);
+int lw_near_csr(lw_handle *h, int *ia, int *ja, float *z);
+int lw_near_csr64(lw_handle *h, int *ia, int *ja, double *z);
+
+/* ------------------------------------------------------------------ modIterativo */
+
+/* crear_precond ILU0 branch (main.f90:884-903) -> ilu0C (modIterativo.f90:1805)
+ * on the handle's near CSR; the MSR factor stays on the device.
+ * *ierr = 0 or the zero-pivot row (then LW_ERR_PIVOT is returned). */
+int lw_ilu0(lw_handle *h, int *ierr);
+/* Host copy of the MSR factor: alu(nnz+1) complex(4) (complex(8) with
+ * LW_NEAR_FP64: pass alu64), jlu(nnz+1), ju(num_e), 1-based. */
+int lw_ilu0_get(lw_handle *h, float *alu, double *alu64, int *jlu, int *ju);
+/* Install an MSR factor held by the caller (the arrays ilu0C fills, same
+ * layout as lw_ilu0_get) as the handle's preconditioner: for callers that
+ * build or edit the factor themselves.  alu is complex(4) unless the handle
+ * stores complex(8) (then pass alu64). */
+int lw_ilu0_set(lw_handle *h, const float *alu, const double *alu64, const int *jlu, const int *ju);
+/* lusol0C (modIterativo.f90:504-560) with the handle's factor, host buffers. */
+int lw_lusol0(lw_handle *h, const double *y, double *x);
+
+/* CGSold (modIterativo.f90:331-493) with MatxVec_MLFMA as the operator.
+ * use_prec: apply the ILU0 factor (lw_ilu0 must have succeeded).
+ * maxit: iteration cap (the reference has none).  Outputs: x, iteration
+ * count, recurrence relative residual ||r||/||b||, converged flag. */
+int lw_cgs(lw_handle *h, const double *b, const double *guess, double errmax, int use_prec,
+           int maxit, double *x, int *iters, double *relres, int *converged);
+
+/* ------------------------------------------------------------------ excitation, modRCS */
+
+/* generar_onda (main.f90:1488-1525): b for unit direction dir(3), complex
+ * amplitude ctte (2 doubles) and complex polarisation pol (6 doubles). */
+int lw_plane_wave(lw_handle *h, const double *dir, const double *ctte, const double *pol,
+                  double *b);
+/* inicializar_RCS (modRCS.f90:23-57): current vector and wave parameters. */
+int lw_rcs_init(lw_handle *h, const double *current, const double *pol, const double *ctte);
+/* sigmaRCS (modRCS.f90:91-105) at n angle pairs (radians) -> sigma in m^2. */
+int lw_rcs_sigma(lw_handle *h, int n, const double *theta, const double *phi, double *sigma);
+
+/* ------------------------------------------------------------------ measurement */
+
+/* Time `reps` device-resident matvecs (after one warm-up) with HIP events on
+ * the handle's stream.  stage_ms (may be NULL) receives the mean time of each
+ * stage: [0] aggregation, [1] interpolation up, [2] translation,
+ * [3] anterpolation down, [4] receive + near field.  Returns mean ms in *ms. */
+int lw_bench_matvec(lw_handle *h, int reps, double *ms, double *stage_ms);
+/* Time `reps` ILU0 triangular solves on device-resident vectors. */
+int lw_bench_lusol(lw_handle *h, int reps, double *ms);
+/* Algorithmic HBM bytes of one matvec in the reference data layout
+ * (pattern arrays twice, campo arrays 6x, transfer tables once, near CSR,
+ * x and y), and of the dominant kernels. */
+int lw_matvec_bytes(lw_handle *h, double *total, double *aggregation, double *receive_near);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

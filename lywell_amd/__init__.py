@@ -1,0 +1,275 @@
+"""Python bindings of the lywell C ABI (include/lywell.h) for tests and bench.
+
+The product boundary is the C ABI, called from Fortran through the modules in
+lywell_amd/fortran; this module binds the same entry points with ctypes so the
+test suite and bench.py drive exactly the code a Fortran caller drives.  It
+loads the in-tree liblywell.so and raises if it has not been built: there is
+no CPU fallback.
+"""
+import ctypes as _C
+import os as _os
+
+import numpy as _np
+
+_HERE = _os.path.dirname(_os.path.abspath(__file__))
+LIB_PATH = _os.path.join(_HERE, "liblywell.so")
+
+LW_OK = 0
+LW_ERR_ARG = -1
+LW_ERR_HIP = -2
+LW_ERR_OCTREE = -3
+LW_ERR_NOMEM = -4
+LW_ERR_STATE = -5
+LW_ERR_PIVOT = -6
+LW_ERR_NODEV = -7
+LW_NEAR_FP64 = 1
+
+_lib = None
+
+
+class LywellError(RuntimeError):
+    def __init__(self, code, msg):
+        super().__init__(f"lywell error {code}: {msg}")
+        self.code = code
+
+
+def _P(a):
+    return a.ctypes.data_as(_C.c_void_p) if a is not None else None
+
+
+def lib():
+    """The loaded liblywell.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not _os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: run `python -m lywell_amd.build` first")
+        L = _C.CDLL(LIB_PATH)
+        L.lw_last_error.restype = _C.c_char_p
+        L.lw_near_nnz.restype = _C.c_longlong
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != LW_OK:
+        raise LywellError(rc, lib().lw_last_error().decode())
+    return rc
+
+
+def device_count():
+    return lib().lw_device_count()
+
+
+class Mesh:
+    """Closed triangulated surface with the reference's RWG arrays.
+
+    Arrays follow the Fortran layout of the reference (column-major (3,n)
+    stored xyz-contiguous, 1-based connectivity)."""
+
+    def __init__(self, p_coord, t_p, t_normal=None):
+        L = lib()
+        self.p_coord = _np.ascontiguousarray(p_coord, dtype=_np.float64).reshape(-1)
+        self.t_p = _np.ascontiguousarray(t_p, dtype=_np.int32).reshape(-1)
+        self.np = self.p_coord.size // 3
+        self.nt = self.t_p.size // 3
+        if t_normal is None:
+            P = self.p_coord.reshape(-1, 3)
+            T = self.t_p.reshape(-1, 3) - 1
+            n = _np.cross(P[T[:, 1]] - P[T[:, 0]], P[T[:, 2]] - P[T[:, 0]])
+            t_normal = n / _np.linalg.norm(n, axis=1, keepdims=True)
+        self.t_normal = _np.ascontiguousarray(t_normal, dtype=_np.float64).reshape(-1)
+        cap = 3 * self.nt
+        e_p = _np.zeros(2 * cap, _np.int32)
+        e_t = _np.zeros(2 * cap, _np.int32)
+        e_po = _np.zeros(2 * cap, _np.int32)
+        e_long = _np.zeros(cap)
+        e_centro = _np.zeros(3 * cap)
+        ne = L.lw_rwg_build(self.np, _P(self.p_coord), self.nt, _P(self.t_p), _P(e_p), _P(e_t), _P(e_po),
+                            _P(e_long), _P(e_centro))
+        if ne <= 0:
+            _check(ne if ne < 0 else LW_ERR_ARG)
+        self.ne = ne
+        self.e_p = e_p[: 2 * ne].copy()
+        self.e_t = e_t[: 2 * ne].copy()
+        self.e_po = e_po[: 2 * ne].copy()
+        self.e_long = e_long[:ne].copy()
+        self.e_centro = e_centro[: 3 * ne].copy()
+        self.t_area = _np.zeros(self.nt)
+        self.t_baric = _np.zeros(3 * self.nt)
+        self.t_baric_sub = _np.zeros(27 * self.nt)
+        _check(L.lw_tri_geometry(self.np, _P(self.p_coord), self.nt, _P(self.t_p), _P(self.t_area),
+                                 _P(self.t_baric), _P(self.t_baric_sub)))
+
+    @classmethod
+    def icosphere(cls, subdiv, ax, ay=None, az=None):
+        """Icosphere of 20*4**subdiv triangles scaled to semi-axes (ax, ay, az)."""
+        ay = ax if ay is None else ay
+        az = ax if az is None else az
+        L = lib()
+        np_, nt = _C.c_int(), _C.c_int()
+        _check(L.lw_mesh_icosphere_size(int(subdiv), _C.byref(np_), _C.byref(nt)))
+        p = _np.zeros(3 * np_.value)
+        t = _np.zeros(3 * nt.value, _np.int32)
+        n = _np.zeros(3 * nt.value)
+        _check(L.lw_mesh_icosphere(int(subdiv), _C.c_double(ax), _C.c_double(ay), _C.c_double(az), _P(p), _P(t),
+                                   _P(n)))
+        return cls(p, t, n)
+
+    @classmethod
+    def perturbed_sphere(cls, subdiv, radius, deg, ord_, coef):
+        """Icosphere with a radial real-spherical-harmonic perturbation."""
+        m = cls.icosphere(subdiv, 1.0)
+        p = m.p_coord.copy()
+        deg = _np.ascontiguousarray(deg, _np.int32)
+        ord_ = _np.ascontiguousarray(ord_, _np.int32)
+        coef = _np.ascontiguousarray(coef, _np.float64)
+        _check(lib().lw_mesh_perturb_sh(m.np, _P(p), len(coef), _P(deg), _P(ord_), _P(coef)))
+        return cls(p * radius, m.t_p, None)
+
+    def init_args(self):
+        """Arrays in the order of inicializar_MLFMA's dummy arguments."""
+        return (self.ne, self.e_centro, self.e_t, self.e_po, self.e_long, self.nt, self.t_area, self.t_baric,
+                self.t_baric_sub, self.np, self.p_coord, self.t_p, self.e_p, self.t_normal)
+
+
+def det_vector(n):
+    """The reference's deterministic test vector v(i) = i/1.3823 + j 57.382/i (main.f90:619-621)."""
+    i = _np.arange(1, n + 1, dtype=_np.float64)
+    return (i / 1.3823 + 1j * 57.382 / i).astype(_np.complex128)
+
+
+class MLFMA:
+    """One MLFMA operator on the GPU (inicializar_MLFMA ... destruir_MLFMA)."""
+
+    def __init__(self, mesh, lam=1.0, ladomax=0.5, fp64_near=False):
+        L = lib()
+        self.mesh = mesh
+        h = _C.c_void_p()
+        a = mesh.init_args()
+        rc = L.lw_mlfma_init(_C.c_double(lam), _C.c_double(ladomax), LW_NEAR_FP64 if fp64_near else 0, a[0],
+                             _P(a[1]), _P(a[2]), _P(a[3]), _P(a[4]), a[5], _P(a[6]), _P(a[7]), _P(a[8]), a[9],
+                             _P(a[10]), _P(a[11]), _P(a[12]), _P(a[13]), _C.byref(h))
+        _check(rc)
+        self.h = h
+        self.ne = mesh.ne
+        self.fp64_near = fp64_near
+
+    def close(self):
+        if self.h:
+            lib().lw_mlfma_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def nlevels(self):
+        return lib().lw_mlfma_nlevels(self.h)
+
+    def level_info(self, level):
+        v = [_C.c_int() for _ in range(5)]
+        npairs = _C.c_longlong()
+        _check(lib().lw_mlfma_level_info(self.h, level, *[_C.byref(x) for x in v], _C.byref(npairs)))
+        nb, nm, nth, nph, nt = (x.value for x in v)
+        return dict(nboxes=nb, nmult=nm, nth=nth, nph=nph, nk=nth * nph, ntransf=nt, npairs=npairs.value)
+
+    def _vec(self, x):
+        x = _np.ascontiguousarray(x, dtype=_np.complex128)
+        if x.shape != (self.ne,):
+            raise ValueError("vector of length num_e expected")
+        return x
+
+    def matvec(self, x):
+        x = self._vec(x)
+        y = _np.zeros(self.ne, _np.complex128)
+        _check(lib().lw_matvec(self.h, _P(x), _P(y)))
+        return y
+
+    def matvec_parts(self, x):
+        x = self._vec(x)
+        yf = _np.zeros(self.ne, _np.complex128)
+        yn = _np.zeros(self.ne, _np.complex128)
+        _check(lib().lw_matvec_parts(self.h, _P(x), _P(yf), _P(yn)))
+        return yf, yn
+
+    @property
+    def nnz(self):
+        return lib().lw_near_nnz(self.h)
+
+    def near_csr(self):
+        nnz = self.nnz
+        ia = _np.zeros(self.ne + 1, _np.int32)
+        ja = _np.zeros(nnz, _np.int32)
+        z = _np.zeros(2 * nnz, _np.float64)
+        _check(lib().lw_near_csr64(self.h, _P(ia), _P(ja), _P(z)))
+        return ia, ja, z.view(_np.complex128)
+
+    def ilu0(self):
+        ierr = _C.c_int()
+        rc = lib().lw_ilu0(self.h, _C.byref(ierr))
+        if rc == LW_ERR_PIVOT:
+            return ierr.value
+        _check(rc)
+        return 0
+
+    def ilu0_factor(self):
+        nnz = self.nnz
+        alu = _np.zeros(2 * (nnz + 1))
+        jlu = _np.zeros(nnz + 1, _np.int32)
+        ju = _np.zeros(self.ne, _np.int32)
+        _check(lib().lw_ilu0_get(self.h, None, _P(alu), _P(jlu), _P(ju)))
+        return alu.view(_np.complex128), jlu, ju
+
+    def lusol0(self, y):
+        y = self._vec(y)
+        x = _np.zeros(self.ne, _np.complex128)
+        _check(lib().lw_lusol0(self.h, _P(y), _P(x)))
+        return x
+
+    def cgs(self, b, guess=None, errmax=1e-4, use_prec=True, maxit=10000):
+        b = self._vec(b)
+        g = None if guess is None else self._vec(guess)
+        x = _np.zeros(self.ne, _np.complex128)
+        it, conv = _C.c_int(), _C.c_int()
+        rr = _C.c_double()
+        _check(lib().lw_cgs(self.h, _P(b), _P(g), _C.c_double(errmax), int(bool(use_prec)), int(maxit), _P(x),
+                            _C.byref(it), _C.byref(rr), _C.byref(conv)))
+        return x, it.value, rr.value, bool(conv.value)
+
+    def plane_wave(self, direction, pol, ctte=1.0):
+        d = _np.ascontiguousarray(direction, _np.float64)
+        c = _np.array([complex(ctte).real, complex(ctte).imag])
+        p = _np.ascontiguousarray(pol, _np.complex128).view(_np.float64)
+        b = _np.zeros(self.ne, _np.complex128)
+        _check(lib().lw_plane_wave(self.h, _P(d), _P(c), _P(p), _P(b)))
+        return b
+
+    def rcs(self, current, pol, theta, phi, ctte=1.0):
+        cur = self._vec(current)
+        c = _np.array([complex(ctte).real, complex(ctte).imag])
+        p = _np.ascontiguousarray(pol, _np.complex128).view(_np.float64)
+        _check(lib().lw_rcs_init(self.h, _P(cur), _P(p), _P(c)))
+        th = _np.ascontiguousarray(_np.broadcast_to(theta, _np.broadcast(theta, phi).shape), _np.float64).ravel()
+        ph = _np.ascontiguousarray(_np.broadcast_to(phi, _np.broadcast(theta, phi).shape), _np.float64).ravel()
+        s = _np.zeros(th.size)
+        _check(lib().lw_rcs_sigma(self.h, th.size, _P(th), _P(ph), _P(s)))
+        return s
+
+    def bench_matvec(self, reps=10):
+        ms = _C.c_double()
+        st = _np.zeros(5)
+        _check(lib().lw_bench_matvec(self.h, int(reps), _C.byref(ms), _P(st)))
+        return ms.value, st
+
+    def bench_lusol(self, reps=3):
+        ms = _C.c_double()
+        _check(lib().lw_bench_lusol(self.h, int(reps), _C.byref(ms)))
+        return ms.value
+
+    def matvec_bytes(self):
+        t, a, r = _C.c_double(), _C.c_double(), _C.c_double()
+        _check(lib().lw_matvec_bytes(self.h, _C.byref(t), _C.byref(a), _C.byref(r)))
+        return dict(total=t.value, aggregation=a.value, receive_near=r.value)

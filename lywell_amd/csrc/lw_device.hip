@@ -1,0 +1,1231 @@
+// MLFMA matvec, ILU0 solves, CGS, plane wave and RCS on gfx950, and the C ABI
+// declared in include/lywell.h.
+//
+// Data layout in HBM (complex(8) = double2, complex(4) = float2):
+//   campo arrays per level:  F[(box*2 + pol)*nk + sample]      (reference (K,2,B))
+//   radiation patterns:      rad[(slot*2 + pol)*nk_L + sample], slots grouped
+//                            by leaf box so a box's patterns are contiguous
+//   transfer functions:      T[t*nk + sample]
+//   near field:              CSR in the reference's row order, float2 values
+//   ILU0 factor:             MSR (alu, jlu, ju) exactly as ilu0C leaves it
+// Vectors stay in the reference's (natural RWG) order.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "lw_internal.h"
+
+namespace lw {
+
+static thread_local std::string g_err;
+void set_error(const std::string& m) { g_err = m; }
+
+namespace {
+
+constexpr double kMu = Physics::mu, kEps = Physics::epsi, kPi = Physics::pi;
+constexpr int kWave = 64;
+
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ double2 cmulc(double2 a, double2 b) {  // conj(a) * b
+  return make_double2(a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x);
+}
+__device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ double2 to_d2(float2 v) { return make_double2((double)v.x, (double)v.y); }
+__device__ __forceinline__ double2 to_d2(double2 v) { return v; }
+
+__device__ __forceinline__ double2 wave_sum(double2 v) {
+#pragma unroll
+  for (int o = kWave / 2; o > 0; o >>= 1) {
+    v.x += __shfl_xor(v.x, o, kWave);
+    v.y += __shfl_xor(v.y, o, kWave);
+  }
+  return v;
+}
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+  for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+  return v;
+}
+
+// ---------------------------------------------------------------- matvec
+
+// Leaf aggregation (modMLFMA.f90:1805-1812): campoC_L(:,:,box) =
+// sum_f funcRAD(:,:,f) x(f).  One workgroup per leaf box.
+__global__ void __launch_bounds__(256) k_aggregate(int nk2, const int* __restrict__ box_slot,
+                                                   const int* __restrict__ slot_basis,
+                                                   const double2* __restrict__ rad,
+                                                   const double2* __restrict__ x, double2* __restrict__ C) {
+  const int b = blockIdx.x;
+  const int s0 = box_slot[b], s1 = box_slot[b + 1];
+  for (int e = threadIdx.x; e < nk2; e += blockDim.x) {
+    double2 acc = make_double2(0.0, 0.0);
+    for (int s = s0; s < s1; ++s) acc = cadd(acc, cmul(rad[(size_t)s * nk2 + e], x[slot_basis[s]]));
+    C[(size_t)b * nk2 + e] = acc;
+  }
+}
+
+// Upward pass (modMLFMA.f90:1813-1835): for each parent box, interpolate each
+// child's field to the parent's samples (Interpolar, :1343-1365), shift it
+// to the parent centre and sum over children in box order.  Grid: (parent,
+// chunk of 256 parent samples); the child field is staged in LDS.
+__global__ void __launch_bounds__(256) k_interp_up(int nkc, int nkf, const int* __restrict__ child_ptr,
+                                                   const double* __restrict__ pcentre,
+                                                   const double* __restrict__ ccentre,
+                                                   const double* __restrict__ kapc,
+                                                   const double* __restrict__ wint, const int* __restrict__ iint,
+                                                   const double2* __restrict__ Cf, double2* __restrict__ Cc,
+                                                   double k0) {
+  extern __shared__ double2 lds[];  // 2 * nkf
+  const int P = blockIdx.x;
+  const int i = blockIdx.y * blockDim.x + threadIdx.x;
+  double2 acc0 = make_double2(0.0, 0.0), acc1 = make_double2(0.0, 0.0);
+  double kx = 0, ky = 0, kz = 0;
+  double w[9];
+  int id[9];
+  if (i < nkc) {
+    kx = kapc[3 * i];
+    ky = kapc[3 * i + 1];
+    kz = kapc[3 * i + 2];
+#pragma unroll
+    for (int j = 0; j < 9; ++j) {
+      w[j] = wint[9 * (size_t)i + j];
+      id[j] = iint[9 * (size_t)i + j];
+    }
+  }
+  const double* pc = pcentre + 3 * P;
+  for (int c = child_ptr[P]; c < child_ptr[P + 1]; ++c) {
+    __syncthreads();
+    const double2* src = Cf + (size_t)c * 2 * nkf;
+    for (int e = threadIdx.x; e < 2 * nkf; e += blockDim.x) lds[e] = src[e];
+    __syncthreads();
+    if (i < nkc) {
+      double2 a0 = make_double2(0.0, 0.0), a1 = make_double2(0.0, 0.0);
+#pragma unroll
+      for (int j = 0; j < 9; ++j) {
+        const double2 v0 = lds[id[j]], v1 = lds[nkf + id[j]];
+        a0.x += w[j] * v0.x;
+        a0.y += w[j] * v0.y;
+        a1.x += w[j] * v1.x;
+        a1.y += w[j] * v1.y;
+      }
+      const double* cc = ccentre + 3 * c;
+      const double r0 = cc[0] - pc[0], r1 = cc[1] - pc[1], r2 = cc[2] - pc[2];
+      double sn, cs;
+      sincos(k0 * ((kx * r0 + ky * r1) + kz * r2), &sn, &cs);
+      const double2 ph = make_double2(cs, sn);
+      acc0 = cadd(acc0, cmul(a0, ph));
+      acc1 = cadd(acc1, cmul(a1, ph));
+    }
+  }
+  if (i < nkc) {
+    Cc[((size_t)P * 2 + 0) * nkc + i] = acc0;
+    Cc[((size_t)P * 2 + 1) * nkc + i] = acc1;
+  }
+}
+
+// Translation (modMLFMA.f90:1837-1853): campoB = pesos * sum_far campoC(far) *
+// T(transfer index).  Grid: (box, chunk of 256 samples).
+__global__ void __launch_bounds__(256) k_translate(int nk, const int* __restrict__ far_ptr,
+                                                   const int* __restrict__ far_box, const int* __restrict__ far_tr,
+                                                   const double2* __restrict__ T, const double* __restrict__ pes,
+                                                   const double2* __restrict__ C, double2* __restrict__ B) {
+  const int b = blockIdx.x;
+  const int k = blockIdx.y * blockDim.x + threadIdx.x;
+  if (k >= nk) return;
+  double2 a0 = make_double2(0.0, 0.0), a1 = make_double2(0.0, 0.0);
+  for (int m = far_ptr[b]; m < far_ptr[b + 1]; ++m) {
+    const int fb = far_box[m];
+    const double2 t = T[(size_t)far_tr[m] * nk + k];
+    a0 = cadd(a0, cmul(C[((size_t)fb * 2 + 0) * nk + k], t));
+    a1 = cadd(a1, cmul(C[((size_t)fb * 2 + 1) * nk + k], t));
+  }
+  const double p = pes[k];
+  B[((size_t)b * 2 + 0) * nk + k] = make_double2(a0.x * p, a0.y * p);
+  B[((size_t)b * 2 + 1) * nk + k] = make_double2(a1.x * p, a1.y * p);
+}
+
+// Downward pass (modMLFMA.f90:1857-1897): shift the parent's field to the
+// child centre, anterpolate (the transpose of Interpolar, :1370-1390, in
+// gather form) and add the child's own translated field.  One workgroup per
+// child box; the shifted parent field is staged in LDS.
+__global__ void __launch_bounds__(256) k_interp_down(int nkc, int nkf, const int* __restrict__ parent,
+                                                     const double* __restrict__ pcentre,
+                                                     const double* __restrict__ ccentre,
+                                                     const double* __restrict__ kapc,
+                                                     const int* __restrict__ wt_ptr, const int* __restrict__ wt_row,
+                                                     const double* __restrict__ wt_w,
+                                                     const double2* __restrict__ Ap, const double2* __restrict__ Bf,
+                                                     double2* __restrict__ Af, double k0) {
+  extern __shared__ double2 lds[];  // 2 * nkc
+  const int c = blockIdx.x;
+  const int P = parent[c];
+  const double* pc = pcentre + 3 * P;
+  const double* cc = ccentre + 3 * c;
+  const double r0 = cc[0] - pc[0], r1 = cc[1] - pc[1], r2 = cc[2] - pc[2];
+  const double2* src = Ap + (size_t)P * 2 * nkc;
+  for (int i = threadIdx.x; i < nkc; i += blockDim.x) {
+    double sn, cs;
+    sincos(-(k0 * ((kapc[3 * i] * r0 + kapc[3 * i + 1] * r1) + kapc[3 * i + 2] * r2)), &sn, &cs);
+    const double2 ph = make_double2(cs, sn);
+    lds[i] = cmul(src[i], ph);
+    lds[nkc + i] = cmul(src[nkc + i], ph);
+  }
+  __syncthreads();
+  for (int s = threadIdx.x; s < nkf; s += blockDim.x) {
+    double2 a0 = make_double2(0.0, 0.0), a1 = make_double2(0.0, 0.0);
+    for (int e = wt_ptr[s]; e < wt_ptr[s + 1]; ++e) {
+      const double w = wt_w[e];
+      const int r = wt_row[e];
+      a0.x += w * lds[r].x;
+      a0.y += w * lds[r].y;
+      a1.x += w * lds[nkc + r].x;
+      a1.y += w * lds[nkc + r].y;
+    }
+    const size_t o0 = ((size_t)c * 2 + 0) * nkf + s, o1 = ((size_t)c * 2 + 1) * nkf + s;
+    Af[o0] = cadd(Bf[o0], a0);
+    Af[o1] = cadd(Bf[o1], a1);
+  }
+}
+
+// Receive (modMLFMA.f90:1915-1933, funcREC = conjg(funcRAD) under EFIE) and
+// near-field CSR product (:1935-1942), one wavefront per row.
+template <typename V>
+__global__ void __launch_bounds__(256) k_receive_near(int ne, int nk2, const int* __restrict__ basis_slot,
+                                                      const int* __restrict__ basis_box,
+                                                      const double2* __restrict__ rad,
+                                                      const double2* __restrict__ AL, const int* __restrict__ ia,
+                                                      const int* __restrict__ ja, const V* __restrict__ z,
+                                                      const double2* __restrict__ x, double2 cc,
+                                                      double2* __restrict__ y, double2* __restrict__ y_near) {
+  const int m = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+  const int lane = threadIdx.x % kWave;
+  if (m >= ne) return;
+  const double2* r = rad + (size_t)basis_slot[m] * nk2;
+  const double2* a = AL + (size_t)basis_box[m] * nk2;
+  double2 f = make_double2(0.0, 0.0);
+  for (int e = lane; e < nk2; e += kWave) f = cadd(f, cmulc(r[e], a[e]));
+  f = wave_sum(f);
+  double2 nsum = make_double2(0.0, 0.0);
+  for (int k = ia[m] + lane; k < ia[m + 1]; k += kWave) nsum = cadd(nsum, cmul(x[ja[k]], to_d2(z[k])));
+  nsum = wave_sum(nsum);
+  if (lane == 0) {
+    const double2 far = cmul(f, cc);
+    if (y_near) {
+      y[m] = far;
+      y_near[m] = nsum;
+    } else {
+      y[m] = cadd(far, nsum);
+    }
+  }
+}
+
+// ---------------------------------------------------------------- ILU0 solves
+
+// lusol0C (modIterativo.f90:504-560): forward solve with the unit-diagonal L
+// then backward solve with U and the inverted diagonal, rows taken level by
+// level (the level sets of the reference's natural order).  A single
+// workgroup runs every level; the barrier between levels orders the global
+// writes of x for the next level's reads (all waves share one CU's L1).
+template <typename V>
+__global__ void __launch_bounds__(1024) k_lusol(const double2* __restrict__ y, double2* __restrict__ x,
+                                                const V* __restrict__ alu, const int* __restrict__ jlu,
+                                                const int* __restrict__ ju, const int* __restrict__ lptr,
+                                                const int* __restrict__ lrows, int nlev_l,
+                                                const int* __restrict__ uptr, const int* __restrict__ urows,
+                                                int nlev_u) {
+  const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave, nw = blockDim.x / kWave;
+  // alu(k), jlu(k), ju(i) are 1-based as in the reference: arrays carry a pad
+  for (int l = 0; l < nlev_l; ++l) {
+    for (int q = lptr[l] + wave; q < lptr[l + 1]; q += nw) {
+      const int i = lrows[q] + 1;
+      double2 s = make_double2(0.0, 0.0);
+      for (int k = jlu[i] + lane; k <= ju[i] - 1; k += kWave) s = cadd(s, cmul(to_d2(alu[k]), x[jlu[k] - 1]));
+      s = wave_sum(s);
+      if (lane == 0) {
+        const double2 yi = y[i - 1];
+        x[i - 1] = make_double2(yi.x - s.x, yi.y - s.y);
+      }
+    }
+    __syncthreads();
+  }
+  for (int l = 0; l < nlev_u; ++l) {
+    for (int q = uptr[l] + wave; q < uptr[l + 1]; q += nw) {
+      const int i = urows[q] + 1;
+      double2 s = make_double2(0.0, 0.0);
+      for (int k = ju[i] + lane; k <= jlu[i + 1] - 1; k += kWave) s = cadd(s, cmul(to_d2(alu[k]), x[jlu[k] - 1]));
+      s = wave_sum(s);
+      if (lane == 0) {
+        const double2 xi = x[i - 1];
+        x[i - 1] = cmul(to_d2(alu[i]), make_double2(xi.x - s.x, xi.y - s.y));
+      }
+    }
+    __syncthreads();
+  }
+}
+
+// ---------------------------------------------------------------- CGS pieces
+
+constexpr int kRedBlocks = 512;
+
+// partial[b] = sum over the block's stripe of (a*b) (conj_a: conj(a)*b);
+// two products in one pass when a2/b2 are given.
+__global__ void __launch_bounds__(256) k_dot2(int n, const double2* __restrict__ a, const double2* __restrict__ b,
+                                              int conj_a, const double2* __restrict__ a2,
+                                              const double2* __restrict__ b2, int conj_a2,
+                                              double* __restrict__ part) {
+  __shared__ double2 sh[2][256 / kWave];
+  double2 s1 = make_double2(0.0, 0.0), s2 = make_double2(0.0, 0.0);
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    s1 = cadd(s1, conj_a ? cmulc(a[i], b[i]) : cmul(a[i], b[i]));
+    if (a2) s2 = cadd(s2, conj_a2 ? cmulc(a2[i], b2[i]) : cmul(a2[i], b2[i]));
+  }
+  s1 = wave_sum(s1);
+  s2 = wave_sum(s2);
+  const int wv = threadIdx.x / kWave, ln = threadIdx.x % kWave;
+  if (ln == 0) {
+    sh[0][wv] = s1;
+    sh[1][wv] = s2;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double2 t1 = make_double2(0.0, 0.0), t2 = make_double2(0.0, 0.0);
+    for (int w = 0; w < (int)(blockDim.x / kWave); ++w) {
+      t1 = cadd(t1, sh[0][w]);
+      t2 = cadd(t2, sh[1][w]);
+    }
+    part[4 * blockIdx.x + 0] = t1.x;
+    part[4 * blockIdx.x + 1] = t1.y;
+    part[4 * blockIdx.x + 2] = t2.x;
+    part[4 * blockIdx.x + 3] = t2.y;
+  }
+}
+
+// Fixed-order final sum of the partials: out[0] = first product, out[1] = second.
+__global__ void __launch_bounds__(kWave) k_dot_finish(int nblk, const double* __restrict__ part,
+                                                      double2* __restrict__ out) {
+  double2 s1 = make_double2(0.0, 0.0), s2 = make_double2(0.0, 0.0);
+  for (int b = threadIdx.x; b < nblk; b += kWave) {
+    s1 = cadd(s1, make_double2(part[4 * b], part[4 * b + 1]));
+    s2 = cadd(s2, make_double2(part[4 * b + 2], part[4 * b + 3]));
+  }
+  s1 = wave_sum(s1);
+  s2 = wave_sum(s2);
+  if (threadIdx.x == 0) {
+    out[0] = s1;
+    out[1] = s2;
+  }
+}
+
+__device__ __forceinline__ double2 cdiv(double2 a, double2 b) {
+  // Smith's algorithm
+  if (fabs(b.x) >= fabs(b.y)) {
+    const double r = b.y / b.x, d = b.x + b.y * r;
+    return make_double2((a.x + a.y * r) / d, (a.y - a.x * r) / d);
+  }
+  const double r = b.x / b.y, d = b.x * r + b.y;
+  return make_double2((a.x * r + a.y) / d, (a.y * r - a.x) / d);
+}
+
+// it == 1: u = r, p = u; else beta = rho1/rho2, u = r + beta q,
+// p = u + beta (q + beta p)   (modIterativo.f90:396-403)
+__global__ void k_cgs_up(int n, int first, const double2* __restrict__ rho1p, const double2* __restrict__ rho2p,
+                         const double2* __restrict__ r, const double2* __restrict__ q, double2* __restrict__ u,
+                         double2* __restrict__ p) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (first) {
+    u[i] = r[i];
+    p[i] = r[i];
+    return;
+  }
+  const double2 beta = cdiv(*rho1p, *rho2p);
+  const double2 ui = cadd(r[i], cmul(beta, q[i]));
+  u[i] = ui;
+  p[i] = cadd(ui, cmul(beta, cadd(q[i], cmul(beta, p[i]))));
+}
+
+// alfa = rho1 / (R_ . V_); q = u - alfa V_; t = u + q   (:419-429)
+__global__ void k_cgs_q(int n, const double2* __restrict__ rho1p, const double2* __restrict__ denp,
+                        double2* __restrict__ alfa_out, const double2* __restrict__ u,
+                        const double2* __restrict__ V, double2* __restrict__ q, double2* __restrict__ t) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const double2 alfa = cdiv(*rho1p, *denp);
+  if (i == 0) *alfa_out = alfa;
+  if (i >= n) return;
+  const double2 qi = make_double2(u[i].x - (alfa.x * V[i].x - alfa.y * V[i].y),
+                                  u[i].y - (alfa.x * V[i].y + alfa.y * V[i].x));
+  q[i] = qi;
+  t[i] = cadd(u[i], qi);
+}
+
+// x += alfa U_; r -= alfa Q_   (:434-437)
+__global__ void k_cgs_xr(int n, const double2* __restrict__ alfap, const double2* __restrict__ U,
+                         const double2* __restrict__ Q, double2* __restrict__ x, double2* __restrict__ r) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const double2 a = *alfap;
+  x[i] = cadd(x[i], cmul(a, U[i]));
+  const double2 aq = cmul(a, Q[i]);
+  r[i] = make_double2(r[i].x - aq.x, r[i].y - aq.y);
+}
+
+__global__ void k_sub(int n, const double2* __restrict__ a, const double2* __restrict__ b, double2* __restrict__ o) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) o[i] = make_double2(a[i].x - b[i].x, a[i].y - b[i].y);
+}
+
+// ---------------------------------------------------------------- excitation and RCS
+
+// generar_onda (main.f90:1488-1525), EFIE term: b(i) = l_i/2 [E(c+).(c+ - v+) +
+// E(c-).(v- - c-)], E(r) = pol ctte exp(-j k d.r)
+__global__ void k_plane_wave(int ne, const double* __restrict__ tb, const double* __restrict__ pc,
+                             const int* __restrict__ et, const int* __restrict__ epo,
+                             const double* __restrict__ el, double ux, double uy, double uz, double k0,
+                             double2 p0, double2 p1, double2 p2, double2 ctte, double2* __restrict__ b) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ne) return;
+  const double2 pc0 = cmul(p0, ctte), pc1 = cmul(p1, ctte), pc2 = cmul(p2, ctte);
+  double2 acc = make_double2(0.0, 0.0);
+  for (int side = 0; side < 2; ++side) {
+    const double* c = tb + 3 * (et[2 * i + side] - 1);
+    const double* v = pc + 3 * (epo[2 * i + side] - 1);
+    double sn, cs;
+    sincos(-(k0 * ((ux * c[0] + uy * c[1]) + uz * c[2])), &sn, &cs);
+    const double2 ph = make_double2(cs, sn);
+    const double sg = side == 0 ? 1.0 : -1.0;
+    const double d0 = sg * (c[0] - v[0]), d1 = sg * (c[1] - v[1]), d2 = sg * (c[2] - v[2]);
+    const double2 e0 = cmul(pc0, ph), e1 = cmul(pc1, ph), e2 = cmul(pc2, ph);
+    acc.x += (e0.x * d0 + e1.x * d1) + e2.x * d2;
+    acc.y += (e0.y * d0 + e1.y * d1) + e2.y * d2;
+  }
+  const double f = 0.5 * el[i];
+  b[i] = make_double2(acc.x * f, acc.y * f);
+}
+
+// calc_E (modRCS.f90:60-89): E(r_obs) = (j w mu / 4 pi) sum_i g(r_obs, rm_i)
+// ((m_i.rrm) rrm/|rrm|^2 - m_i); one workgroup per observation point.
+__global__ void __launch_bounds__(256) k_rcs(int ne, const double* __restrict__ tb, const int* __restrict__ et,
+                                             const double* __restrict__ el, const double2* __restrict__ cur,
+                                             const double* __restrict__ robs, double kfs,
+                                             double2* __restrict__ Eout) {
+  __shared__ double2 sh[3][256 / kWave];
+  const double* ro = robs + 3 * blockIdx.x;
+  double2 E0 = make_double2(0, 0), E1 = make_double2(0, 0), E2 = make_double2(0, 0);
+  for (int i = threadIdx.x; i < ne; i += blockDim.x) {
+    const double* t1 = tb + 3 * (et[2 * i] - 1);
+    const double* t2 = tb + 3 * (et[2 * i + 1] - 1);
+    const double2 Il = make_double2(cur[i].x * el[i], cur[i].y * el[i]);
+    double rr[3], dm[3];
+    for (int d = 0; d < 3; ++d) {
+      const double rm = 0.5 * (t1[d] + t2[d]);
+      dm[d] = t2[d] - t1[d];
+      rr[d] = ro[d] - rm;
+    }
+    const double rr2 = (rr[0] * rr[0] + rr[1] * rr[1]) + rr[2] * rr[2];
+    const double dr = (dm[0] * rr[0] + dm[1] * rr[1]) + dm[2] * rr[2];
+    const double rn = sqrt(rr2);
+    double sn, cs;
+    sincos(-(kfs * rn), &sn, &cs);
+    const double2 g = make_double2(cs / rn, sn / rn);
+    const double2 gI = cmul(g, Il);
+    // (m.rrm) rrm / |rrm|^2 - m with m = dm * I * l
+    const double c0 = dr * rr[0] / rr2 - dm[0], c1 = dr * rr[1] / rr2 - dm[1], c2 = dr * rr[2] / rr2 - dm[2];
+    E0.x += gI.x * c0; E0.y += gI.y * c0;
+    E1.x += gI.x * c1; E1.y += gI.y * c1;
+    E2.x += gI.x * c2; E2.y += gI.y * c2;
+  }
+  E0 = wave_sum(E0);
+  E1 = wave_sum(E1);
+  E2 = wave_sum(E2);
+  const int wv = threadIdx.x / kWave, ln = threadIdx.x % kWave;
+  if (ln == 0) {
+    sh[0][wv] = E0;
+    sh[1][wv] = E1;
+    sh[2][wv] = E2;
+  }
+  __syncthreads();
+  if (threadIdx.x < 3) {
+    double2 s = make_double2(0, 0);
+    for (int w = 0; w < (int)(blockDim.x / kWave); ++w) s = cadd(s, sh[threadIdx.x][w]);
+    Eout[3 * blockIdx.x + threadIdx.x] = s;
+  }
+}
+
+// ---------------------------------------------------------------- helpers
+
+int blocks_for(long long n, int b) { return (int)((n + b - 1) / b); }
+
+// one full matvec on device vectors x -> y (y_near: optional split output)
+int run_matvec(Handle* h, const double2* x, double2* y, double2* y_near, hipEvent_t* ev = nullptr) {
+  const int L = h->L;
+  hipStream_t st = h->stream;
+  DevLevel& lf = h->lv[L];
+  if (ev) LW_HIP(hipEventRecord(ev[0], st));
+  hipLaunchKernelGGL(k_aggregate, dim3(lf.nb), dim3(256), 0, st, 2 * lf.nk, h->d_box_slot, h->d_slot_basis,
+                     h->d_rad, x, lf.C);
+  if (ev) LW_HIP(hipEventRecord(ev[1], st));
+  const double k0 = (2.0 * kPi) / h->plan.phys.lambda;
+  for (int n = L; n >= 4; --n) {
+    DevLevel& c = h->lv[n];
+    DevLevel& p = h->lv[n - 1];
+    dim3 grid(p.nb, blocks_for(p.nk, 256));
+    hipLaunchKernelGGL(k_interp_up, grid, dim3(256), 2 * c.nk * sizeof(double2), st, p.nk, c.nk, p.child_ptr,
+                       p.centre, c.centre, p.kap, p.wint, p.iint, c.C, p.C, k0);
+  }
+  if (ev) LW_HIP(hipEventRecord(ev[2], st));
+  for (int n = 3; n <= L; ++n) {
+    DevLevel& d = h->lv[n];
+    dim3 grid(d.nb, blocks_for(d.nk, 256));
+    hipLaunchKernelGGL(k_translate, grid, dim3(256), 0, st, d.nk, d.far_ptr, d.far_box, d.far_tr, d.T, d.pes,
+                       d.C, d.B);
+  }
+  if (ev) LW_HIP(hipEventRecord(ev[3], st));
+  for (int n = 4; n <= L; ++n) {
+    DevLevel& c = h->lv[n];
+    DevLevel& p = h->lv[n - 1];
+    const double2* src = (n == 4) ? p.B : p.A;
+    hipLaunchKernelGGL(k_interp_down, dim3(c.nb), dim3(256), 2 * p.nk * sizeof(double2), st, p.nk, c.nk,
+                       c.parent, p.centre, c.centre, p.kap, p.wt_ptr, p.wt_row, p.wt_w, src, c.B, c.A, k0);
+  }
+  if (ev) LW_HIP(hipEventRecord(ev[4], st));
+  const double w = h->plan.phys.w;
+  const double2 cc = make_double2(0.0, w * kMu / (4.0 * kPi));
+  const int rows_per_block = 4;
+  dim3 grid(blocks_for(h->ne, rows_per_block)), block(kWave * rows_per_block);
+  if (h->fp64_near)
+    hipLaunchKernelGGL(k_receive_near<double2>, grid, block, 0, st, h->ne, 2 * lf.nk, h->d_basis_slot,
+                       h->d_basis_box, h->d_rad, lf.A, h->d_ia, h->d_ja, (const double2*)h->d_z, x, cc, y, y_near);
+  else
+    hipLaunchKernelGGL(k_receive_near<float2>, grid, block, 0, st, h->ne, 2 * lf.nk, h->d_basis_slot,
+                       h->d_basis_box, h->d_rad, lf.A, h->d_ia, h->d_ja, (const float2*)h->d_z, x, cc, y, y_near);
+  if (ev) LW_HIP(hipEventRecord(ev[5], st));
+  LW_HIP(hipGetLastError());
+  return LW_OK;
+}
+
+int run_lusol(Handle* h, const double2* y, double2* x) {
+  if (!h->have_ilu) {
+    set_error("lusol0: no ILU0 factor (call lw_ilu0 first)");
+    return LW_ERR_STATE;
+  }
+  if (h->fp64_near)
+    hipLaunchKernelGGL(k_lusol<double2>, dim3(1), dim3(1024), 0, h->stream, y, x, (const double2*)h->d_alu,
+                       h->d_jlu, h->d_ju, h->d_lptr, h->d_lrows, h->nlev_l, h->d_uptr, h->d_urows, h->nlev_u);
+  else
+    hipLaunchKernelGGL(k_lusol<float2>, dim3(1), dim3(1024), 0, h->stream, y, x, (const float2*)h->d_alu,
+                       h->d_jlu, h->d_ju, h->d_lptr, h->d_lrows, h->nlev_l, h->d_uptr, h->d_urows, h->nlev_u);
+  LW_HIP(hipGetLastError());
+  return LW_OK;
+}
+
+// dot products: out[0] = sum(a*b) (conj a if ca), out[1] = sum(a2*b2)
+int run_dot2(Handle* h, const double2* a, const double2* b, int ca, const double2* a2, const double2* b2, int ca2,
+             double2* out) {
+  hipLaunchKernelGGL(k_dot2, dim3(kRedBlocks), dim3(256), 0, h->stream, h->ne, a, b, ca, a2, b2, ca2, h->d_part);
+  hipLaunchKernelGGL(k_dot_finish, dim3(1), dim3(kWave), 0, h->stream, kRedBlocks, h->d_part, out);
+  LW_HIP(hipGetLastError());
+  return LW_OK;
+}
+
+int check_handle(lw_handle* hh) {
+  if (!hh) {
+    set_error("null handle");
+    return LW_ERR_ARG;
+  }
+  return LW_OK;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------- init
+
+int upload_factor(Handle* h);
+
+static int init_device_state(Handle* h, const Geometry& g) {
+  Plan& P = h->plan;
+  const int L = P.L;
+  h->L = L;
+  h->ne = P.ne;
+  h->nnz = P.nnz;
+  LW_HIP(hipGetDevice(&h->device));
+  LW_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+  int rc;
+#define TRY(x)          \
+  do {                  \
+    rc = (x);           \
+    if (rc) return rc;  \
+  } while (0)
+  // geometry
+  auto vec_d = [](const double* p, size_t n) { return std::vector<double>(p, p + n); };
+  auto vec_i = [](const int* p, size_t n) { return std::vector<int>(p, p + n); };
+  TRY(upload(h, &h->d_p, vec_d(g.p_coord, 3 * (size_t)g.np)));
+  TRY(upload(h, &h->d_tb, vec_d(g.t_baric, 3 * (size_t)g.nt)));
+  TRY(upload(h, &h->d_tbs, vec_d(g.t_baric_sub, 27 * (size_t)g.nt)));
+  TRY(upload(h, &h->d_el, vec_d(g.e_long, (size_t)g.ne)));
+  TRY(upload(h, &h->d_ec, vec_d(g.e_centro, 3 * (size_t)g.ne)));
+  TRY(upload(h, &h->d_et, vec_i(g.e_t, 2 * (size_t)g.ne)));
+  TRY(upload(h, &h->d_epo, vec_i(g.e_po, 2 * (size_t)g.ne)));
+  // levels
+  h->lv.assign(L + 1, DevLevel());
+  double* d_binom2 = nullptr;
+  TRY(upload(h, &d_binom2, P.binom2));
+  for (int n = 3; n <= L; ++n) {
+    LevelPlan& lp = P.lv[n];
+    DevLevel& d = h->lv[n];
+    d.nk = lp.nk;
+    d.nb = lp.nb;
+    d.nT = lp.nT;
+    d.npairs = (long long)lp.far_box.size();
+    TRY(upload(h, &d.kap, lp.kap));
+    TRY(upload(h, &d.pes, lp.pes));
+    TRY(upload(h, &d.centre, lp.centre));
+    std::vector<int> par(lp.parent.begin(), lp.parent.end());
+    TRY(upload(h, &d.parent, par));
+    if (n < L) TRY(upload(h, &d.child_ptr, lp.child_ptr));
+    TRY(upload(h, &d.far_ptr, lp.far_ptr));
+    TRY(upload(h, &d.far_box, lp.far_box));
+    TRY(upload(h, &d.far_tr, lp.far_tr));
+    TRY(dmalloc(h, &d.T, (size_t)lp.nk * std::max(lp.nT, 1)));
+    const size_t fsz = (size_t)lp.nk * 2 * lp.nb;
+    TRY(dmalloc(h, &d.C, fsz));
+    TRY(dmalloc(h, &d.B, fsz));
+    TRY(dmalloc(h, &d.A, fsz));
+    if (n < L) {
+      TRY(upload(h, &d.wint, lp.wint));
+      TRY(upload(h, &d.iint, lp.iint));
+      TRY(upload(h, &d.wt_ptr, lp.wt_ptr));
+      TRY(upload(h, &d.wt_row, lp.wt_row));
+      TRY(upload(h, &d.wt_w, lp.wt_w));
+    }
+    double* d_tu = nullptr;
+    double* d_th = nullptr;
+    int* d_tL = nullptr;
+    TRY(upload(h, &d_tu, lp.trans_u));
+    TRY(upload(h, &d_th, lp.trans_h));
+    TRY(upload(h, &d_tL, lp.trans_L));
+    TRY(launch_setup_transfer(h, n, d_binom2, d_tu, d_tL, d_th, lp.hstride));
+  }
+  // level 3's upward output and level 2 have no parents: nothing else to upload
+  // leaf ordering and patterns
+  TRY(upload(h, &h->d_slot_basis, P.slot_basis));
+  TRY(upload(h, &h->d_basis_slot, P.basis_slot));
+  TRY(upload(h, &h->d_basis_box, P.basis_box));
+  TRY(upload(h, &h->d_box_slot, P.box_slot));
+  TRY(dmalloc(h, &h->d_rad, (size_t)P.lv[L].nk * 2 * P.ne));
+  TRY(launch_setup_rad(h));
+  // near field
+  TRY(upload(h, &h->d_ia, P.ia));
+  TRY(upload(h, &h->d_ja, P.ja));
+  if (h->fp64_near) {
+    double2* z = nullptr;
+    TRY(dmalloc(h, &z, (size_t)P.nnz));
+    h->d_z = z;
+  } else {
+    float2* z = nullptr;
+    TRY(dmalloc(h, &z, (size_t)P.nnz));
+    h->d_z = z;
+  }
+  TRY(launch_setup_near(h));
+  // work vectors and reduction buffers
+  TRY(dmalloc(h, &h->d_x, (size_t)P.ne));
+  TRY(dmalloc(h, &h->d_y, (size_t)P.ne));
+  for (auto& w : h->d_w) TRY(dmalloc(h, &w, (size_t)P.ne));
+  TRY(dmalloc(h, &h->d_part, 4 * (size_t)kRedBlocks));
+  h->scal_cap = 64;
+  TRY(dmalloc(h, &h->d_scal, (size_t)h->scal_cap));
+  LW_HIP(hipStreamSynchronize(h->stream));
+  // large LDS requests for the downward kernel at the top level
+  int maxk = 0;
+  for (int n = 3; n <= L; ++n) maxk = std::max(maxk, P.lv[n].nk);
+  if (2 * maxk * (int)sizeof(double2) > 160 * 1024) {
+    set_error("lw_mlfma_init: sample count exceeds LDS capacity of the pass kernels");
+    return LW_ERR_ARG;
+  }
+  LW_HIP(hipFuncSetAttribute((const void*)k_interp_down, hipFuncAttributeMaxDynamicSharedMemorySize,
+                             2 * maxk * (int)sizeof(double2)));
+  LW_HIP(hipFuncSetAttribute((const void*)k_interp_up, hipFuncAttributeMaxDynamicSharedMemorySize,
+                             2 * maxk * (int)sizeof(double2)));
+#undef TRY
+  return LW_OK;
+}
+
+}  // namespace lw
+
+using lw::Handle;
+
+struct lw_handle {
+  Handle h;
+};
+
+extern "C" {
+
+const char* lw_last_error(void) { return lw::g_err.c_str(); }
+
+int lw_device_count(void) {
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+  return n;
+}
+
+static void free_handle(lw_handle* hh) {
+  if (!hh) return;
+  Handle* h = &hh->h;
+  if (h->stream) (void)hipStreamSynchronize(h->stream);
+  for (void* p : h->allocs) (void)hipFree(p);
+  if (h->stream) (void)hipStreamDestroy(h->stream);
+  delete hh;
+}
+
+int lw_mlfma_init(double lambda, double ladomax, int flags, int num_e, const double* e_centro, const int* e_t,
+                  const int* e_po, const double* e_long, int num_t, const double* t_area, const double* t_baric,
+                  const double* t_baric_sub, int num_p, const double* p_coord, const int* t_p, const int* e_p,
+                  const double* t_normal, lw_handle** out) {
+  if (!out || !e_centro || !e_t || !e_po || !e_long || !t_baric || !t_baric_sub || !p_coord || num_e < 2 ||
+      num_t < 2 || num_p < 3) {
+    lw::set_error("lw_mlfma_init: missing geometry array or empty mesh");
+    return LW_ERR_ARG;
+  }
+  *out = nullptr;
+  if (lw_device_count() < 1) {
+    lw::set_error("lw_mlfma_init: no HIP device visible");
+    return LW_ERR_NODEV;
+  }
+  lw::Geometry g;
+  g.ne = num_e;
+  g.nt = num_t;
+  g.np = num_p;
+  g.e_centro = e_centro;
+  g.e_long = e_long;
+  g.t_area = t_area;
+  g.t_baric = t_baric;
+  g.t_baric_sub = t_baric_sub;
+  g.p_coord = p_coord;
+  g.t_normal = t_normal;
+  g.e_t = e_t;
+  g.e_po = e_po;
+  g.t_p = t_p;
+  g.e_p = e_p;
+  for (int i = 0; i < 2 * num_e; ++i)
+    if (e_t[i] < 1 || e_t[i] > num_t || e_po[i] < 1 || e_po[i] > num_p) {
+      lw::set_error("lw_mlfma_init: connectivity index out of range");
+      return LW_ERR_ARG;
+    }
+  lw_handle* hh = new lw_handle();
+  Handle* h = &hh->h;
+  h->flags = flags;
+  h->fp64_near = (flags & LW_NEAR_FP64) != 0;
+  std::string msg;
+  int rc = lw::build_plan(g, lambda, ladomax, h->plan, msg);
+  if (rc) {
+    lw::set_error(msg);
+    delete hh;
+    return rc;
+  }
+  rc = lw::init_device_state(h, g);
+  if (rc) {
+    free_handle(hh);
+    return rc;
+  }
+  *out = hh;
+  return LW_OK;
+}
+
+int lw_mlfma_destroy(lw_handle* hh) {
+  free_handle(hh);
+  return LW_OK;
+}
+
+int lw_mlfma_nlevels(lw_handle* hh) { return hh ? hh->h.L : LW_ERR_ARG; }
+
+int lw_mlfma_level_info(lw_handle* hh, int level, int* nboxes, int* nmult, int* nth, int* nph, int* ntransf,
+                        long long* npairs) {
+  if (lw::check_handle(hh)) return LW_ERR_ARG;
+  Handle* h = &hh->h;
+  if (level < 1 || level > h->L) return LW_ERR_ARG;
+  const lw::LevelPlan& lp = h->plan.lv[level];
+  if (nboxes) *nboxes = lp.nb;
+  if (nmult) *nmult = lp.nmult;
+  if (nth) *nth = lp.nth;
+  if (nph) *nph = lp.nph;
+  if (ntransf) *ntransf = lp.nT;
+  if (npairs) *npairs = (long long)lp.far_box.size();
+  return LW_OK;
+}
+
+int lw_matvec_dev(lw_handle* hh, const void* x_dev, void* y_dev) {
+  if (lw::check_handle(hh) || !x_dev || !y_dev) return LW_ERR_ARG;
+  return lw::run_matvec(&hh->h, (const double2*)x_dev, (double2*)y_dev, nullptr);
+}
+
+int lw_matvec(lw_handle* hh, const double* x, double* y) {
+  if (lw::check_handle(hh) || !x || !y) return LW_ERR_ARG;
+  Handle* h = &hh->h;
+  const size_t bytes = sizeof(double2) * h->ne;
+  LW_HIP(hipMemcpyAsync(h->d_x, x, bytes, hipMemcpyHostToDevice, h->stream));
+  int rc = lw::run_matvec(h, h->d_x, h->d_y, nullptr);
+  if (rc) return rc;
+  LW_HIP(hipMemcpyAsync(y, h->d_y, bytes, hipMemcpyDeviceToHost, h->stream));
+  LW_HIP(hipStreamSynchronize(h->stream));
+  return LW_OK;
+}
+
+int lw_matvec_parts(lw_handle* hh, const double* x, double* y_far, double* y_near) {
+  if (lw::check_handle(hh) || !x || !y_far || !y_near) return LW_ERR_ARG;
+  Handle* h = &hh->h;
+  const size_t bytes = sizeof(double2) * h->ne;
+  LW_HIP(hipMemcpyAsync(h->d_x, x, bytes, hipMemcpyHostToDevice, h->stream));
+  int rc = lw::run_matvec(h, h->d_x, h->d_y, h->d_w[0]);
+  if (rc) return rc;
+  LW_HIP(hipMemcpyAsync(y_far, h->d_y, bytes, hipMemcpyDeviceToHost, h->stream));
+  LW_HIP(hipMemcpyAsync(y_near, h->d_w[0], bytes, hipMemcpyDeviceToHost, h->stream));
+  LW_HIP(hipStreamSynchronize(h->stream));
+  return LW_OK;
+}
+
+long long lw_near_nnz(lw_handle* hh) { return hh ? hh->h.nnz : LW_ERR_ARG; }
+
+int lw_near_csr(lw_handle* hh, int* ia, int* ja, float* z) {
+  if (lw::check_handle(hh)) return LW_ERR_ARG;
+  Handle* h = &hh->h;
+  if (h->fp64_near && z) {
+    lw::set_error("lw_near_csr: handle stores complex(8) values; use lw_near_csr64");
+    return LW_ERR_ARG;
+  }
+  if (ia)
+    for (int i = 0; i <= h->ne; ++i) ia[i] = h->plan.ia[i] + 1;
+  if (ja)
+    for (long long k = 0; k < h->nnz; ++k) ja[k] = h->plan.ja[k] + 1;
+  if (z) {
+    LW_HIP(hipMemcpyAsync(z, h->d_z, sizeof(float2) * h->nnz, hipMemcpyDeviceToHost, h->stream));
+    LW_HIP(hipStreamSynchronize(h->stream));
+  }
+  return LW_OK;
+}
+
+int lw_near_csr64(lw_handle* hh, int* ia, int* ja, double* z) {
+  if (lw::check_handle(hh)) return LW_ERR_ARG;
+  Handle* h = &hh->h;
+  if (ia)
+    for (int i = 0; i <= h->ne; ++i) ia[i] = h->plan.ia[i] + 1;
+  if (ja)
+    for (long long k = 0; k < h->nnz; ++k) ja[k] = h->plan.ja[k] + 1;
+  if (z) {
+    if (h->fp64_near) {
+      LW_HIP(hipMemcpyAsync(z, h->d_z, sizeof(double2) * h->nnz, hipMemcpyDeviceToHost, h->stream));
+      LW_HIP(hipStreamSynchronize(h->stream));
+    } else {
+      std::vector<float> tmp(2 * (size_t)h->nnz);
+      LW_HIP(hipMemcpyAsync(tmp.data(), h->d_z, sizeof(float2) * h->nnz, hipMemcpyDeviceToHost, h->stream));
+      LW_HIP(hipStreamSynchronize(h->stream));
+      for (size_t k = 0; k < tmp.size(); ++k) z[k] = tmp[k];
+    }
+  }
+  return LW_OK;
+}
+
+int lw_ilu0(lw_handle* hh, int* ierr) {
+  if (lw::check_handle(hh)) return LW_ERR_ARG;
+  Handle* h = &hh->h;
+  const int n = h->ne;
+  int piv;
+  if (h->fp64_near) {
+    std::vector<double> a(2 * (size_t)h->nnz);
+    LW_HIP(hipMemcpyAsync(a.data(), h->d_z, sizeof(double2) * h->nnz, hipMemcpyDeviceToHost, h->stream));
+    LW_HIP(hipStreamSynchronize(h->stream));
+    piv = lw::ilu0_msr<double>(n, h->plan.ia.data(), h->plan.ja.data(), a.data(), h->h_alu64, h->h_jlu, h->h_ju);
+  } else {
+    std::vector<float> a(2 * (size_t)h->nnz);
+    LW_HIP(hipMemcpyAsync(a.data(), h->d_z, sizeof(float2) * h->nnz, hipMemcpyDeviceToHost, h->stream));
+    LW_HIP(hipStreamSynchronize(h->stream));
+    piv = lw::ilu0_msr<float>(n, h->plan.ia.data(), h->plan.ja.data(), a.data(), h->h_alu32, h->h_jlu, h->h_ju);
+  }
+  if (ierr) *ierr = piv;
+  if (piv) {
+    h->have_ilu = false;
+    lw::set_error("ilu0: zero pivot at row " + std::to_string(piv));
+    return LW_ERR_PIVOT;
+  }
+  return lw::upload_factor(h);
+}
+
+}  // extern "C"
+
+namespace lw {
+// Device copies of the host MSR factor in h->h_alu*/h_jlu/h_ju (1-based with a
+// pad element so alu(k) is d_alu[k]) and the level sets of both sweeps.
+int upload_factor(Handle* h) {
+  const int n = h->ne;
+  const size_t na = (size_t)h->nnz + 2;
+  if (!h->d_alu) {
+    if (h->fp64_near) {
+      double2* p = nullptr;
+      if (int rc = lw::dmalloc(h, &p, na)) return rc;
+      h->d_alu = p;
+    } else {
+      float2* p = nullptr;
+      if (int rc = lw::dmalloc(h, &p, na)) return rc;
+      h->d_alu = p;
+    }
+    if (int rc = lw::dmalloc(h, &h->d_jlu, na)) return rc;
+    std::vector<int> ju1((size_t)n + 1, 0);
+    if (int rc = lw::dmalloc(h, &h->d_ju, ju1.size())) return rc;
+  }
+  if (h->fp64_near)
+    LW_HIP(hipMemcpyAsync(h->d_alu, h->h_alu64.data(), sizeof(double2) * na, hipMemcpyHostToDevice, h->stream));
+  else
+    LW_HIP(hipMemcpyAsync(h->d_alu, h->h_alu32.data(), sizeof(float2) * na, hipMemcpyHostToDevice, h->stream));
+  LW_HIP(hipMemcpyAsync(h->d_jlu, h->h_jlu.data(), sizeof(int) * na, hipMemcpyHostToDevice, h->stream));
+  std::vector<int> ju1((size_t)n + 1, 0);
+  for (int i = 0; i < n; ++i) ju1[i + 1] = h->h_ju[i];
+  LW_HIP(hipMemcpyAsync(h->d_ju, ju1.data(), sizeof(int) * ju1.size(), hipMemcpyHostToDevice, h->stream));
+  std::vector<int> lptr, lrows, uptr, urows;
+  lw::tri_levels(n, h->h_jlu, h->h_ju, false, lptr, lrows);
+  lw::tri_levels(n, h->h_jlu, h->h_ju, true, uptr, urows);
+  h->nlev_l = (int)lptr.size() - 1;
+  h->nlev_u = (int)uptr.size() - 1;
+  if (int rc = lw::upload(h, &h->d_lptr, lptr)) return rc;
+  if (int rc = lw::upload(h, &h->d_lrows, lrows)) return rc;
+  if (int rc = lw::upload(h, &h->d_uptr, uptr)) return rc;
+  if (int rc = lw::upload(h, &h->d_urows, urows)) return rc;
+  LW_HIP(hipStreamSynchronize(h->stream));
+  h->have_ilu = true;
+  return LW_OK;
+}
+}  // namespace lw
+
+extern "C" {
+
+int lw_ilu0_set(lw_handle* hh, const float* alu, const double* alu64, const int* jlu, const int* ju) {
+  if (lw::check_handle(hh) || !jlu || !ju) return LW_ERR_ARG;
+  Handle* h = &hh->h;
+  const long long nnz = h->nnz;
+  const int n = h->ne;
+  if (jlu[0] != n + 2 || jlu[n] - 1 > nnz + 1 || jlu[n] < jlu[0]) {
+    lw::set_error("lw_ilu0_set: jlu does not describe an MSR factor of this operator");
+    return LW_ERR_ARG;
+  }
+  const size_t na = (size_t)nnz + 1;
+  h->h_jlu.assign(na + 1, 0);
+  std::memcpy(h->h_jlu.data() + 1, jlu, sizeof(int) * na);
+  h->h_ju.assign(ju, ju + n);
+  for (int i = 0; i < n; ++i)
+    if (ju[i] < jlu[i] || ju[i] > jlu[i + 1]) {
+      lw::set_error("lw_ilu0_set: ju out of range");
+      return LW_ERR_ARG;
+    }
+  for (size_t k = (size_t)n + 2; k <= (size_t)jlu[n] - 1; ++k)
+    if (jlu[k - 1] < 1 || jlu[k - 1] > n) {
+      lw::set_error("lw_ilu0_set: column index out of range");
+      return LW_ERR_ARG;
+    }
+  if (h->fp64_near) {
+    if (!alu64) return LW_ERR_ARG;
+    h->h_alu64.assign(2 * (na + 1), 0.0);
+    std::memcpy(h->h_alu64.data() + 2, alu64, sizeof(double) * 2 * na);
+  } else {
+    if (!alu) return LW_ERR_ARG;
+    h->h_alu32.assign(2 * (na + 1), 0.0f);
+    std::memcpy(h->h_alu32.data() + 2, alu, sizeof(float) * 2 * na);
+  }
+  return lw::upload_factor(h);
+}
+
+int lw_ilu0_get(lw_handle* hh, float* alu, double* alu64, int* jlu, int* ju) {
+  if (lw::check_handle(hh)) return LW_ERR_ARG;
+  Handle* h = &hh->h;
+  if (!h->have_ilu) {
+    lw::set_error("lw_ilu0_get: no factor");
+    return LW_ERR_STATE;
+  }
+  const size_t na = (size_t)h->nnz + 1;
+  if (alu) {
+    if (h->fp64_near) return LW_ERR_ARG;
+    std::memcpy(alu, h->h_alu32.data() + 2, sizeof(float) * 2 * na);
+  }
+  if (alu64) {
+    if (h->fp64_near)
+      std::memcpy(alu64, h->h_alu64.data() + 2, sizeof(double) * 2 * na);
+    else
+      for (size_t k = 0; k < 2 * na; ++k) alu64[k] = h->h_alu32[2 + k];
+  }
+  if (jlu) std::memcpy(jlu, h->h_jlu.data() + 1, sizeof(int) * na);
+  if (ju) std::memcpy(ju, h->h_ju.data(), sizeof(int) * h->ne);
+  return LW_OK;
+}
+
+int lw_lusol0(lw_handle* hh, const double* y, double* x) {
+  if (lw::check_handle(hh) || !y || !x) return LW_ERR_ARG;
+  Handle* h = &hh->h;
+  const size_t bytes = sizeof(double2) * h->ne;
+  LW_HIP(hipMemcpyAsync(h->d_x, y, bytes, hipMemcpyHostToDevice, h->stream));
+  int rc = lw::run_lusol(h, h->d_x, h->d_y);
+  if (rc) return rc;
+  LW_HIP(hipMemcpyAsync(x, h->d_y, bytes, hipMemcpyDeviceToHost, h->stream));
+  LW_HIP(hipStreamSynchronize(h->stream));
+  return LW_OK;
+}
+
+int lw_cgs(lw_handle* hh, const double* b, const double* guess, double errmax, int use_prec, int maxit, double* x,
+           int* iters, double* relres, int* converged) {
+  if (lw::check_handle(hh) || !b || !x || maxit < 1) return LW_ERR_ARG;
+  Handle* h = &hh->h;
+  if (use_prec && !h->have_ilu) {
+    lw::set_error("lw_cgs: preconditioner requested but no ILU0 factor");
+    return LW_ERR_STATE;
+  }
+  const int n = h->ne;
+  const size_t bytes = sizeof(double2) * n;
+  hipStream_t st = h->stream;
+  double2 *d_b = h->d_w[0], *r = h->d_w[1], *R_ = h->d_w[2], *u = h->d_w[3], *p = h->d_w[4], *P_ = h->d_w[5],
+          *V = h->d_w[6], *q = h->d_w[7], *U = h->d_w[8], *Q = h->d_w[9], *t = h->d_w[10], *xv = h->d_x;
+  // device scalars: [0] rho1 [1] rho2 [2] den [3] alfa [4] |r|^2 or |b|^2 [5] next rho
+  double2* s = h->d_scal;
+  LW_HIP(hipMemcpyAsync(d_b, b, bytes, hipMemcpyHostToDevice, st));
+  if (guess)
+    LW_HIP(hipMemcpyAsync(xv, guess, bytes, hipMemcpyHostToDevice, st));
+  else
+    LW_HIP(hipMemsetAsync(xv, 0, bytes, st));
+  int rc = lw::run_matvec(h, xv, t, nullptr);
+  if (rc) return rc;
+  const int vb = lw::blocks_for(n, 256);
+  hipLaunchKernelGGL(lw::k_sub, dim3(vb), dim3(256), 0, st, n, d_b, t, r);
+  LW_HIP(hipMemcpyAsync(R_, r, bytes, hipMemcpyDeviceToDevice, st));
+  // |b|^2 (conjugated) and rho1 = sum(r*R_) (unconjugated)
+  if ((rc = lw::run_dot2(h, d_b, d_b, 1, r, R_, 0, s + 4))) return rc;
+  double2 hs[2];
+  LW_HIP(hipMemcpyAsync(hs, s + 4, sizeof(hs), hipMemcpyDeviceToHost, st));
+  LW_HIP(hipStreamSynchronize(st));
+  const double nb = std::sqrt(hs[0].x);
+  double2 rho1 = hs[1];
+  LW_HIP(hipMemcpyAsync(s + 0, s + 5, sizeof(double2), hipMemcpyDeviceToDevice, st));
+  int it = 0;
+  bool conv = false;
+  double rr = 0.0;
+  while (it < maxit) {
+    ++it;
+    if (rho1.x == 0.0 && rho1.y == 0.0) break;  // breakdown exit (modIterativo.f90:392-395)
+    hipLaunchKernelGGL(lw::k_cgs_up, dim3(vb), dim3(256), 0, st, n, it == 1 ? 1 : 0, s + 0, s + 1, r, q, u, p);
+    if (use_prec) {
+      if ((rc = lw::run_lusol(h, p, P_))) return rc;
+    } else {
+      LW_HIP(hipMemcpyAsync(P_, p, bytes, hipMemcpyDeviceToDevice, st));
+    }
+    if ((rc = lw::run_matvec(h, P_, V, nullptr))) return rc;
+    if ((rc = lw::run_dot2(h, R_, V, 0, nullptr, nullptr, 0, s + 2))) return rc;
+    hipLaunchKernelGGL(lw::k_cgs_q, dim3(vb), dim3(256), 0, st, n, s + 0, s + 2, s + 3, u, V, q, t);
+    if (use_prec) {
+      if ((rc = lw::run_lusol(h, t, U))) return rc;
+    } else {
+      LW_HIP(hipMemcpyAsync(U, t, bytes, hipMemcpyDeviceToDevice, st));
+    }
+    if ((rc = lw::run_matvec(h, U, Q, nullptr))) return rc;
+    hipLaunchKernelGGL(lw::k_cgs_xr, dim3(vb), dim3(256), 0, st, n, s + 3, U, Q, xv, r);
+    // |r|^2 and the next rho = sum(r*R_)
+    if ((rc = lw::run_dot2(h, r, r, 1, r, R_, 0, s + 4))) return rc;
+    LW_HIP(hipMemcpyAsync(hs, s + 4, sizeof(hs), hipMemcpyDeviceToHost, st));
+    LW_HIP(hipStreamSynchronize(st));
+    const double nr = std::sqrt(hs[0].x);
+    rr = nr / nb;
+    if (nr < errmax * nb) {
+      conv = true;
+      break;
+    }
+    // rho2 = rho1; rho1 = next (modIterativo.f90:486)
+    LW_HIP(hipMemcpyAsync(s + 1, s + 0, sizeof(double2), hipMemcpyDeviceToDevice, st));
+    LW_HIP(hipMemcpyAsync(s + 0, s + 5, sizeof(double2), hipMemcpyDeviceToDevice, st));
+    rho1 = hs[1];
+  }
+  LW_HIP(hipMemcpyAsync(x, xv, bytes, hipMemcpyDeviceToHost, st));
+  LW_HIP(hipStreamSynchronize(st));
+  if (iters) *iters = it;
+  if (relres) *relres = rr;
+  if (converged) *converged = conv ? 1 : 0;
+  return LW_OK;
+}
+
+int lw_plane_wave(lw_handle* hh, const double* dir, const double* ctte, const double* pol, double* b) {
+  if (lw::check_handle(hh) || !dir || !ctte || !pol || !b) return LW_ERR_ARG;
+  Handle* h = &hh->h;
+  const double nd = std::sqrt((dir[0] * dir[0] + dir[1] * dir[1]) + dir[2] * dir[2]);
+  if (!(nd > 0)) return LW_ERR_ARG;
+  const double k0 = h->plan.phys.w * std::sqrt(lw::kMu * lw::kEps);
+  hipLaunchKernelGGL(lw::k_plane_wave, dim3(lw::blocks_for(h->ne, 256)), dim3(256), 0, h->stream, h->ne, h->d_tb,
+                     h->d_p, h->d_et, h->d_epo, h->d_el, dir[0] / nd, dir[1] / nd, dir[2] / nd, k0,
+                     make_double2(pol[0], pol[1]), make_double2(pol[2], pol[3]), make_double2(pol[4], pol[5]),
+                     make_double2(ctte[0], ctte[1]), h->d_y);
+  LW_HIP(hipGetLastError());
+  LW_HIP(hipMemcpyAsync(b, h->d_y, sizeof(double2) * h->ne, hipMemcpyDeviceToHost, h->stream));
+  LW_HIP(hipStreamSynchronize(h->stream));
+  return LW_OK;
+}
+
+int lw_rcs_init(lw_handle* hh, const double* current, const double* pol, const double* ctte) {
+  if (lw::check_handle(hh) || !current || !pol || !ctte) return LW_ERR_ARG;
+  Handle* h = &hh->h;
+  if (!h->d_cur)
+    if (int rc = lw::dmalloc(h, &h->d_cur, (size_t)h->ne)) return rc;
+  LW_HIP(hipMemcpyAsync(h->d_cur, current, sizeof(double2) * h->ne, hipMemcpyHostToDevice, h->stream));
+  for (int c = 0; c < 3; ++c) h->pol[c] = make_double2(pol[2 * c], pol[2 * c + 1]);
+  h->ctte = make_double2(ctte[0], ctte[1]);
+  h->rcs_ready = true;
+  return LW_OK;
+}
+
+int lw_rcs_sigma(lw_handle* hh, int n, const double* theta, const double* phi, double* sigma) {
+  if (lw::check_handle(hh) || n < 0 || (n > 0 && (!theta || !phi || !sigma))) return LW_ERR_ARG;
+  Handle* h = &hh->h;
+  if (!h->rcs_ready) {
+    lw::set_error("lw_rcs_sigma: call lw_rcs_init first");
+    return LW_ERR_STATE;
+  }
+  if (n == 0) return LW_OK;
+  // calc_RCS (modRCS.f90:109-153): far-zone distance from the bounding box of
+  // the edge centres
+  const lw::Geometry* gp = nullptr;
+  (void)gp;
+  const lw::Plan& P = h->plan;
+  std::vector<double> ec(3 * (size_t)h->ne);
+  LW_HIP(hipMemcpyAsync(ec.data(), h->d_ec, sizeof(double) * ec.size(), hipMemcpyDeviceToHost, h->stream));
+  LW_HIP(hipStreamSynchronize(h->stream));
+  int lim[6] = {0, 0, 0, 0, 0, 0};
+  for (int i = 1; i < h->ne; ++i) {
+    const double* c = &ec[3 * (size_t)i];
+    if (c[0] > ec[3 * lim[0]]) lim[0] = i;
+    if (c[0] < ec[3 * lim[1]]) lim[1] = i;
+    if (c[1] > ec[3 * lim[2] + 1]) lim[2] = i;
+    if (c[1] < ec[3 * lim[3] + 1]) lim[3] = i;
+    if (c[2] > ec[3 * lim[4] + 2]) lim[4] = i;
+    if (c[2] < ec[3 * lim[5] + 2]) lim[5] = i;
+  }
+  const double mx = ec[3 * lim[0]], mnx = ec[3 * lim[1]], my = ec[3 * lim[2] + 1], mny = ec[3 * lim[3] + 1],
+               mz = ec[3 * lim[4] + 2], mnz = ec[3 * lim[5] + 2];
+  const double w = P.phys.w;
+  const double lam = 2 * lw::kPi / (w * std::sqrt(lw::kMu * lw::kEps));
+  const double D = std::sqrt(((mx - mnx) * (mx - mnx) + (my - mny) * (my - mny)) + (mz - mnz) * (mz - mnz));
+  double rzl = 2 * (D * D) / lam;
+  if (50 * D > rzl) rzl = 50 * D;
+  if (20 * lam > rzl) rzl = 20 * lam;
+  rzl = rzl * 10;
+  const double roo[3] = {0.5 * (mx + mnx), 0.5 * (my + mny), 0.5 * (mz + mnz)};
+  std::vector<double> ro(3 * (size_t)n);
+  for (int a = 0; a < n; ++a) {
+    ro[3 * a + 0] = roo[0] + (rzl * std::sin(theta[a]) * std::cos(phi[a]));
+    ro[3 * a + 1] = roo[1] + (rzl * std::sin(theta[a]) * std::sin(phi[a]));
+    ro[3 * a + 2] = roo[2] + (rzl * std::cos(theta[a]));
+  }
+  double* d_ro = nullptr;
+  double2* d_E = nullptr;
+  LW_HIP(hipMalloc(&d_ro, sizeof(double) * ro.size()));
+  LW_HIP(hipMalloc(&d_E, sizeof(double2) * 3 * n));
+  LW_HIP(hipMemcpyAsync(d_ro, ro.data(), sizeof(double) * ro.size(), hipMemcpyHostToDevice, h->stream));
+  hipLaunchKernelGGL(lw::k_rcs, dim3(n), dim3(256), 0, h->stream, h->ne, h->d_tb, h->d_et, h->d_el, h->d_cur, d_ro,
+                     w * std::sqrt(lw::kMu * lw::kEps), d_E);
+  std::vector<double2> E(3 * (size_t)n);
+  LW_HIP(hipMemcpyAsync(E.data(), d_E, sizeof(double2) * E.size(), hipMemcpyDeviceToHost, h->stream));
+  LW_HIP(hipStreamSynchronize(h->stream));
+  (void)hipFree(d_ro);
+  (void)hipFree(d_E);
+  // E *= j w mu / 4pi; sigma = 4 pi r^2 |E|^2 / |pol ctte|^2 (modRCS.f90:91-105)
+  const double cm = w * lw::kMu / (4 * lw::kPi);
+  double pn = 0.0;
+  for (int c = 0; c < 3; ++c) {
+    const double re = h->pol[c].x * h->ctte.x - h->pol[c].y * h->ctte.y;
+    const double im = h->pol[c].x * h->ctte.y + h->pol[c].y * h->ctte.x;
+    pn += re * re + im * im;
+  }
+  for (int a = 0; a < n; ++a) {
+    double e2 = 0.0;
+    for (int c = 0; c < 3; ++c) {
+      const double2 v = E[3 * (size_t)a + c];
+      e2 += (v.x * v.x + v.y * v.y) * (cm * cm);
+    }
+    sigma[a] = e2 * 4 * lw::kPi * (rzl * rzl) / pn;
+  }
+  return LW_OK;
+}
+
+int lw_bench_matvec(lw_handle* hh, int reps, double* ms, double* stage_ms) {
+  if (lw::check_handle(hh) || reps < 1 || !ms) return LW_ERR_ARG;
+  Handle* h = &hh->h;
+  hipEvent_t ev[6];
+  for (auto& e : ev) LW_HIP(hipEventCreate(&e));
+  // deterministic input
+  std::vector<double2> xin(h->ne);
+  for (int i = 0; i < h->ne; ++i) xin[i] = make_double2((i + 1) / 1.3823, 57.382 / (i + 1));
+  LW_HIP(hipMemcpyAsync(h->d_x, xin.data(), sizeof(double2) * h->ne, hipMemcpyHostToDevice, h->stream));
+  int rc = lw::run_matvec(h, h->d_x, h->d_y, nullptr);
+  if (rc) return rc;
+  LW_HIP(hipStreamSynchronize(h->stream));
+  double acc[5] = {0, 0, 0, 0, 0}, tot = 0.0;
+  for (int r = 0; r < reps; ++r) {
+    if ((rc = lw::run_matvec(h, h->d_x, h->d_y, nullptr, ev))) return rc;
+    LW_HIP(hipEventSynchronize(ev[5]));
+    for (int s = 0; s < 5; ++s) {
+      float t = 0.f;
+      LW_HIP(hipEventElapsedTime(&t, ev[s], ev[s + 1]));
+      acc[s] += t;
+    }
+    float t = 0.f;
+    LW_HIP(hipEventElapsedTime(&t, ev[0], ev[5]));
+    tot += t;
+  }
+  for (auto& e : ev) (void)hipEventDestroy(e);
+  *ms = tot / reps;
+  if (stage_ms)
+    for (int s = 0; s < 5; ++s) stage_ms[s] = acc[s] / reps;
+  return LW_OK;
+}
+
+int lw_bench_lusol(lw_handle* hh, int reps, double* ms) {
+  if (lw::check_handle(hh) || reps < 1 || !ms) return LW_ERR_ARG;
+  Handle* h = &hh->h;
+  hipEvent_t e0, e1;
+  LW_HIP(hipEventCreate(&e0));
+  LW_HIP(hipEventCreate(&e1));
+  int rc = lw::run_lusol(h, h->d_y, h->d_x);
+  if (rc) return rc;
+  LW_HIP(hipEventRecord(e0, h->stream));
+  for (int r = 0; r < reps; ++r)
+    if ((rc = lw::run_lusol(h, h->d_y, h->d_x))) return rc;
+  LW_HIP(hipEventRecord(e1, h->stream));
+  LW_HIP(hipEventSynchronize(e1));
+  float t = 0.f;
+  LW_HIP(hipEventElapsedTime(&t, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  *ms = t / reps;
+  return LW_OK;
+}
+
+int lw_matvec_bytes(lw_handle* hh, double* total, double* aggregation, double* receive_near) {
+  if (lw::check_handle(hh)) return LW_ERR_ARG;
+  Handle* h = &hh->h;
+  const lw::Plan& P = h->plan;
+  const double c16 = 16.0;
+  const double pattern = (double)P.lv[P.L].nk * 2 * P.ne * c16;  // one pass over funcRAD
+  double campo = 0.0, trans = 0.0;
+  for (int n = 3; n <= P.L; ++n) {
+    campo += (double)P.lv[n].nk * 2 * P.lv[n].nb * c16;
+    trans += (double)P.lv[n].nk * P.lv[n].nT * c16;
+  }
+  const double zb = h->fp64_near ? 16.0 : 8.0;
+  const double near = (zb + 4.0) * (double)P.nnz + 4.0 * (P.ne + 1);
+  const double xy = 2.0 * P.ne * c16;
+  if (total) *total = 2 * pattern + 6 * campo + trans + near + xy;
+  // aggregation: reads the patterns and x once, writes campoC at the leaf
+  if (aggregation) *aggregation = pattern + P.ne * c16 + (double)P.lv[P.L].nk * 2 * P.lv[P.L].nb * c16;
+  // receive + near: patterns, leaf campoA, near CSR, x gathered once, y written
+  if (receive_near)
+    *receive_near = pattern + (double)P.lv[P.L].nk * 2 * P.lv[P.L].nb * c16 + near + 2.0 * P.ne * c16;
+  return LW_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,118 @@
+// Device-side state of one MLFMA operator (one mesh) and shared helpers.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "lw_plan.h"
+#include "lywell.h"
+
+namespace lw {
+
+void set_error(const std::string& m);
+
+#define LW_HIP(call)                                                                 \
+  do {                                                                               \
+    hipError_t _e = (call);                                                          \
+    if (_e != hipSuccess) {                                                          \
+      ::lw::set_error(std::string(#call) + ": " + hipGetErrorString(_e));            \
+      return _e == hipErrorOutOfMemory ? LW_ERR_NOMEM : LW_ERR_HIP;                  \
+    }                                                                                \
+  } while (0)
+
+// Device arrays of one level (levels >= 3).
+struct DevLevel {
+  int nk = 0, nb = 0, nT = 0;
+  double* kap = nullptr;      // (3, nk)
+  double* pes = nullptr;      // (nk)
+  double* centre = nullptr;   // (3, nb)
+  int* parent = nullptr;      // (nb) into level-1
+  int* child_ptr = nullptr;   // (nb+1) into level+1 (non-leaf)
+  int* far_ptr = nullptr;     // (nb+1)
+  int* far_box = nullptr;     // (npairs)
+  int* far_tr = nullptr;      // (npairs)
+  double2* T = nullptr;       // (nk, nT), transfer function of vector t at T[t*nk + k]
+  double2* C = nullptr;       // campoC (nk, 2, nb): C[(b*2+p)*nk + k]
+  double2* B = nullptr;       // campoB
+  double2* A = nullptr;       // campoA
+  // interpolation from level+1 to this level (3 <= n < L)
+  double* wint = nullptr;     // (9, nk)
+  int* iint = nullptr;        // (9, nk) fine sample, 0-based
+  int* wt_ptr = nullptr;      // (nk_fine + 1)
+  int* wt_row = nullptr;      // coarse sample of each transposed entry
+  double* wt_w = nullptr;     // weight of each transposed entry
+  long long npairs = 0;
+};
+
+struct Handle {
+  Plan plan;
+  int flags = 0;
+  bool fp64_near = false;
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::vector<DevLevel> lv;  // [1..L]
+  int L = 0, ne = 0;
+  long long nnz = 0;
+  // geometry on the device
+  double *d_p = nullptr, *d_tb = nullptr, *d_tbs = nullptr, *d_el = nullptr, *d_ec = nullptr;
+  int *d_et = nullptr, *d_epo = nullptr;
+  // leaf patterns and ordering
+  double2* d_rad = nullptr;  // (nk_L, 2, ne) in slot order
+  int *d_slot_basis = nullptr, *d_basis_slot = nullptr, *d_basis_box = nullptr, *d_box_slot = nullptr;
+  // near field CSR (natural order)
+  int *d_ia = nullptr, *d_ja = nullptr;
+  void* d_z = nullptr;  // float2 or double2 values
+  // ILU0 factor (MSR, 1-based indices kept as in the reference)
+  bool have_ilu = false;
+  void* d_alu = nullptr;  // float2 / double2, alu(k) at index k
+  int *d_jlu = nullptr, *d_ju = nullptr;
+  int *d_lptr = nullptr, *d_lrows = nullptr, *d_uptr = nullptr, *d_urows = nullptr;
+  int nlev_l = 0, nlev_u = 0;
+  std::vector<float> h_alu32;
+  std::vector<double> h_alu64;
+  std::vector<int> h_jlu, h_ju;
+  // work vectors (complex(8)[ne])
+  double2 *d_x = nullptr, *d_y = nullptr, *d_w[12] = {};
+  double* d_part = nullptr;   // reduction partials
+  double2* d_scal = nullptr;  // CGS scalars
+  int scal_cap = 0;
+  // RCS state
+  double2* d_cur = nullptr;
+  double2 pol[3] = {};
+  double2 ctte = {};
+  bool rcs_ready = false;
+  std::vector<void*> allocs;
+};
+
+template <typename T>
+int dmalloc(Handle* h, T** p, size_t n) {
+  void* q = nullptr;
+  if (n == 0) n = 1;
+  hipError_t e = hipMalloc(&q, n * sizeof(T));
+  if (e != hipSuccess) {
+    set_error(std::string("hipMalloc: ") + hipGetErrorString(e));
+    return LW_ERR_NOMEM;
+  }
+  h->allocs.push_back(q);
+  *p = reinterpret_cast<T*>(q);
+  return LW_OK;
+}
+
+template <typename T>
+int upload(Handle* h, T** p, const std::vector<T>& v) {
+  int rc = dmalloc(h, p, v.size());
+  if (rc) return rc;
+  // synchronous copy: callers may pass temporaries
+  if (!v.empty()) LW_HIP(hipMemcpy(*p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+  return LW_OK;
+}
+
+// setup kernels (lw_setup_kernels.hip)
+int launch_setup_rad(Handle* h);
+int launch_setup_transfer(Handle* h, int level, const double* d_binom2, const double* d_tu,
+                          const int* d_tL, const double* d_th, int hstride);
+int launch_setup_near(Handle* h);
+
+}  // namespace lw
