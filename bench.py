@@ -1,0 +1,207 @@
+"""Headline benchmark: RHS solved per second at 1e-4 relative residual.
+
+Workload (the single-GPU configuration the metric is quoted on): PEC sphere of
+radius 4 lambda (lambda = 1 m), icosphere subdivision 6 = 81,920 triangles =
+122,880 RWG unknowns, ladomax 0.25 -> 6-level MLFMA, EFIE, plane wave
+travelling along -z, x-polarised; ILU0 on the near field and CGS to relative
+residual 1e-4 (CGSold semantics, FP64 vectors, complex(4)-stored near field
+and ILU0 factor as the reference ships them).
+
+A step is one right-hand side solved from a zero initial guess.  Inputs are
+resident in HBM before the timed region (operators are built and the
+right-hand sides uploaded first).  Timed: K solves between device syncs.
+
+Also reported:
+  roofline: the dominant kernel of a solve, its algorithmic bytes per launch
+            over its mean duration (HIP events on the operator's stream);
+  cpu_baseline: the C oracle (the reference's algorithm restated, 1 core) on
+            a bounded sample of the same workload, scaled to RHS/s.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--subdiv S]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1)
+    ap.add_argument("--warmup", type=int, default=0)
+    ap.add_argument("--subdiv", type=int, default=6)
+    ap.add_argument("--radius", type=float, default=4.0)
+    ap.add_argument("--ladomax", type=float, default=0.25)
+    ap.add_argument("--errmax", type=float, default=1e-4)
+    ap.add_argument("--maxit", type=int, default=20000)
+    ap.add_argument("--no-prec", action="store_true", help="CGS without ILU0")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
+    ap.add_argument("--cpu-iters", type=int, default=3)
+    args = ap.parse_args()
+
+    if args.gpus != 1:
+        print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
+                          "n_gpus": args.gpus}))
+        sys.exit(2)
+
+    import lywell_amd as lw
+    if lw.device_count() < 1:
+        raise SystemExit("bench: no HIP device visible")
+
+    t0 = time.time()
+    mesh = lw.Mesh.icosphere(args.subdiv, args.radius)
+    log(f"mesh: {mesh.ne} RWG unknowns ({time.time() - t0:.1f}s)")
+    t0 = time.time()
+    op = lw.MLFMA(mesh, lam=1.0, ladomax=args.ladomax)
+    setup_s = time.time() - t0
+    levels = [op.level_info(l) for l in range(1, op.nlevels + 1)]
+    log(f"operator: {op.nlevels} levels, near nnz {op.nnz}, setup {setup_s:.1f}s")
+    use_prec = not args.no_prec
+    ilu_s = 0.0
+    if use_prec:
+        t0 = time.time()
+        ierr = op.ilu0()
+        ilu_s = time.time() - t0
+        log(f"ilu0: ierr {ierr}, {ilu_s:.1f}s")
+        if ierr != 0:
+            use_prec = False
+
+    # right-hand sides: x-polarised waves along -z, then rotated in azimuth
+    def rhs(i):
+        ph = np.pi * i / max(1, args.steps + args.warmup)
+        d = np.array([0.0, 0.0, -1.0])
+        pol = np.array([np.cos(ph), np.sin(ph), 0.0], np.complex128)
+        return op.plane_wave(d, pol)
+
+    # per-kernel timing (HIP events on the operator's stream)
+    mv_ms, stages = op.bench_matvec(reps=10)
+    bytes_ = op.matvec_bytes()
+    lus_ms = op.bench_lusol(reps=3) if use_prec else 0.0
+    log(f"matvec {mv_ms:.3f} ms (stages {np.round(stages, 3).tolist()}), lusol {lus_ms:.3f} ms")
+
+    bs = [rhs(i) for i in range(args.warmup + args.steps)]
+    for i in range(args.warmup):
+        op.cgs(bs[i], errmax=args.errmax, use_prec=use_prec, maxit=args.maxit)
+    iters, conv_all, rr_all = [], [], []
+    t0 = time.time()
+    for i in range(args.steps):
+        x, it, rr, conv = op.cgs(bs[args.warmup + i], errmax=args.errmax, use_prec=use_prec, maxit=args.maxit)
+        iters.append(it)
+        conv_all.append(conv)
+        rr_all.append(rr)
+        log(f"solve {i}: {it} iterations, relres {rr:.3e}, converged {conv}")
+    wall = time.time() - t0
+    rhs_per_s = args.steps / wall
+    # true residual of the last solution with the GPU operator
+    true_res = float(np.linalg.norm(op.matvec(x) - bs[-1]) / np.linalg.norm(bs[-1]))
+
+    # dominant kernel of a solve: per CGS iteration 2 matvecs + 2 lusol
+    it_mean = float(np.mean(iters))
+    per_iter_ms = wall * 1e3 / max(1.0, float(np.sum(iters)))
+    stage_names = ["aggregation", "interp_up", "translation", "interp_down", "receive_near"]
+    cand = {name: 2 * t for name, t in zip(stage_names, stages)}
+    if use_prec:
+        cand["lusol"] = 2 * lus_ms
+    dom = max(cand, key=cand.get)
+    nnz = op.nnz
+    if dom == "lusol":
+        # forward + backward sweep over the MSR factor: complex(4) values and
+        # int32 columns once, the row pointers, y read, x written and gathered
+        alg = 12.0 * nnz + 8.0 * (mesh.ne + 1) + 16.0 * mesh.ne * 3
+        dom_ms = lus_ms
+    elif dom == "receive_near":
+        alg = bytes_["receive_near"]
+        dom_ms = stages[4]
+    elif dom == "aggregation":
+        alg = bytes_["aggregation"]
+        dom_ms = stages[0]
+    else:
+        alg = bytes_["total"]
+        dom_ms = mv_ms
+    achieved = alg / (dom_ms * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+                "algorithmic_bytes": alg, "mean_ms": round(dom_ms, 4)}
+    matvec_gbs = bytes_["total"] / (mv_ms * 1e-3) / 1e9
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(mesh, args, use_prec, it_mean)
+
+    out = {
+        "metric": "RHS solved/s at 1e-4 residual (123k RWG)",
+        "value": round(rhs_per_s, 6),
+        "unit": "RHS/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(wall * 1e3 / args.steps, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "c128 (near field and ILU0 stored c64)",
+        "data": "synthetic icosphere mesh and plane-wave excitation",
+        "config": {
+            "workload": f"PEC sphere R={args.radius} lambda, icosphere s={args.subdiv}, {mesh.ne} RWG, "
+                        f"{op.nlevels}-level MLFMA (ladomax {args.ladomax}), "
+                        f"{'ILU0-' if use_prec else ''}CGS to {args.errmax:g}, x-pol plane wave along -z",
+            "unknowns": mesh.ne, "levels": op.nlevels, "near_nnz": nnz,
+            "preconditioner": "ILU0" if use_prec else "none",
+        },
+        "iterations": iters,
+        "converged": all(conv_all),
+        "relres": rr_all,
+        "true_relres": true_res,
+        "ms_per_iteration": round(per_iter_ms, 3),
+        "matvec_ms": round(mv_ms, 4),
+        "matvec_stage_ms": dict(zip(stage_names, [round(float(s), 4) for s in stages])),
+        "matvec_GBps_algorithmic": round(matvec_gbs, 1),
+        "lusol_ms": round(lus_ms, 4),
+        "setup_s": round(setup_s, 2),
+        "ilu0_s": round(ilu_s, 2),
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out))
+
+
+def cpu_baseline(mesh, args, use_prec, gpu_iters):
+    """The C oracle on the host: a few CGS iterations at the full size, scaled
+    by the GPU's iteration count to an RHS/s figure."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import _oracle
+    ncores = len(os.sched_getaffinity(0))
+    os.environ.setdefault("OMP_NUM_THREADS", str(min(16, ncores)))
+    t0 = time.time()
+    ora = _oracle.Oracle(mesh, lam=1.0, ladomax=args.ladomax)  # setup may use OpenMP threads
+    if use_prec:
+        ora.ilu0()
+    log(f"cpu oracle setup {time.time() - t0:.1f}s")
+    b = ora.plane_wave([0, 0, -1.0], [1, 0, 0])
+    n = max(1, args.cpu_iters)
+    t0 = time.time()
+    ora.cgs(b, errmax=1e-30, use_prec=use_prec, maxit=n)  # matvec, lusol and vector work: one core
+    dt = time.time() - t0
+    per_it = dt / n
+    log(f"cpu oracle: {n} CGS iterations in {dt:.1f}s")
+    return {"value": round(1.0 / (per_it * gpu_iters), 8), "unit": "RHS/s", "cores": 1, "kind": "port",
+            "sample": f"{n} {'ILU0-' if use_prec else ''}CGS iterations of the C oracle at {mesh.ne} RWG "
+                      f"(1 core, {per_it:.2f} s/iteration) scaled by the GPU's {gpu_iters:.0f} iterations to 1e-4",
+            "seconds_per_iteration": round(per_it, 3)}
+
+
+if __name__ == "__main__":
+    main()

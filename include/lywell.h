@@ -54,6 +54,17 @@ int lw_mesh_icosphere(int subdiv, double ax, double ay, double az, double *p_coo
 /* Radial perturbation by real spherical harmonics: r *= 1 + sum c_i Y_{l_i m_i}. */
 int lw_mesh_perturb_sh(int np, double *p_coord, int ncoef, const int *deg, const int *ord,
                        const double *coef);
+/* ASCII STL (obtener_p_t, modReadStl.f90:1378-1533): sizes after merging
+ * duplicate nodes, then the merged nodes p_coord(3,np), triangles t_p(3,nt)
+ * (1-based) and facet normals t_normal(3,nt) in the order the reference's
+ * reader and limpiarnodos produce them. */
+int lw_stl_size(const char *path, int *np, int *nt);
+int lw_stl_read(const char *path, double *p_coord, int *t_p, double *t_normal);
+/* Writes an ASCII STL whose coordinates the reference's parser (s_to_r8)
+ * reads back exactly (fixed point, 12 decimals, |x| < 1000).  t_normal may be
+ * NULL (normals from the vertex order). */
+int lw_stl_write(const char *path, int np, const double *p_coord, int nt, const int *t_p,
+                 const double *t_normal);
 /* parametros_rwg (modRWG.f90:133-174): outputs sized for 3*nt edges.
  * Returns the number of RWG edges (>0) or a negative status. */
 int lw_rwg_build(int np, const double *p_coord, int nt, const int *t_p, int *e_p, int *e_t,

@@ -126,6 +126,23 @@ class Mesh:
         _check(lib().lw_mesh_perturb_sh(m.np, _P(p), len(coef), _P(deg), _P(ord_), _P(coef)))
         return cls(p * radius, m.t_p, None)
 
+    @classmethod
+    def from_stl(cls, path):
+        """ASCII STL read as the reference reads it (obtener_p_t)."""
+        L = lib()
+        np_, nt = _C.c_int(), _C.c_int()
+        b = _os.fsencode(path)
+        _check(L.lw_stl_size(b, _C.byref(np_), _C.byref(nt)))
+        p = _np.zeros(3 * np_.value)
+        t = _np.zeros(3 * nt.value, _np.int32)
+        n = _np.zeros(3 * nt.value)
+        _check(L.lw_stl_read(b, _P(p), _P(t), _P(n)))
+        return cls(p, t, n)
+
+    def write_stl(self, path):
+        _check(lib().lw_stl_write(_os.fsencode(path), self.np, _P(self.p_coord), self.nt, _P(self.t_p),
+                                  _P(self.t_normal)))
+
     def init_args(self):
         """Arrays in the order of inicializar_MLFMA's dummy arguments."""
         return (self.ne, self.e_centro, self.e_t, self.e_po, self.e_long, self.nt, self.t_area, self.t_baric,
@@ -273,3 +290,80 @@ class MLFMA:
         t, a, r = _C.c_double(), _C.c_double(), _C.c_double()
         _check(lib().lw_matvec_bytes(self.h, _C.byref(t), _C.byref(a), _C.byref(r)))
         return dict(total=t.value, aggregation=a.value, receive_near=r.value)
+
+
+class Batch:
+    """R right-hand sides through one operator pass: FP32 far field, FP64 near
+    field, ILU0 solves and CGS (lw_batch_*)."""
+
+    def __init__(self, op, nrhs):
+        L = lib()
+        b = _C.c_void_p()
+        _check(L.lw_batch_create(op.h, int(nrhs), _C.byref(b)))
+        self.b = b
+        self.op = op
+        self.R = int(nrhs)
+        self.ne = op.ne
+
+    def close(self):
+        if self.b:
+            lib().lw_batch_destroy(self.b)
+            self.b = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _mat(self, X):
+        X = _np.ascontiguousarray(X, dtype=_np.complex128)
+        if X.shape != (self.ne, self.R):
+            raise ValueError(f"({self.ne}, {self.R}) complex array expected")
+        return X
+
+    def matvec(self, X):
+        X = self._mat(X)
+        Y = _np.zeros_like(X)
+        _check(lib().lw_batch_matvec(self.b, _P(X), _P(Y)))
+        return Y
+
+    def lusol0(self, Y):
+        Y = self._mat(Y)
+        X = _np.zeros_like(Y)
+        _check(lib().lw_batch_lusol0(self.b, _P(Y), _P(X)))
+        return X
+
+    def cgs(self, B, guess=None, errmax=1e-4, use_prec=True, maxit=10000):
+        B = self._mat(B)
+        G = None if guess is None else self._mat(guess)
+        X = _np.zeros_like(B)
+        it = _np.zeros(self.R, _np.int32)
+        rr = _np.zeros(self.R)
+        cv = _np.zeros(self.R, _np.int32)
+        _check(lib().lw_batch_cgs(self.b, _P(B), _P(G), _C.c_double(errmax), int(bool(use_prec)), int(maxit), _P(X),
+                                  _P(it), _P(rr), _P(cv)))
+        return X, it, rr, cv.astype(bool)
+
+    def plane_wave(self, dirs, pols, ctte=1.0):
+        d = _np.ascontiguousarray(dirs, _np.float64).reshape(self.R, 3)
+        p = _np.ascontiguousarray(pols, _np.complex128).reshape(self.R, 3)
+        c = _np.array([complex(ctte).real, complex(ctte).imag])
+        B = _np.zeros((self.ne, self.R), _np.complex128)
+        _check(lib().lw_batch_plane_wave(self.b, _P(d), _P(p), _P(c), _P(B)))
+        return B
+
+    def rcs(self, I, pols, theta, phi, ctte=1.0):
+        I = self._mat(I)
+        p = _np.ascontiguousarray(pols, _np.complex128).reshape(self.R, 3)
+        c = _np.array([complex(ctte).real, complex(ctte).imag])
+        th = _np.ascontiguousarray(_np.broadcast_to(theta, (self.R,)), _np.float64)
+        ph = _np.ascontiguousarray(_np.broadcast_to(phi, (self.R,)), _np.float64)
+        s = _np.zeros(self.R)
+        _check(lib().lw_batch_rcs(self.b, _P(I), _P(p), _P(c), _P(th), _P(ph), _P(s)))
+        return s
+
+    def bench(self, reps=5):
+        mv, lu = _C.c_double(), _C.c_double()
+        _check(lib().lw_bench_batch(self.b, int(reps), _C.byref(mv), _C.byref(lu)))
+        return mv.value, lu.value

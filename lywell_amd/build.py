@@ -46,6 +46,7 @@ def build_hip():
     objs = []
     units = [
         ("lw_device.hip", []),
+        ("lw_batch.hip", []),
         # bit-exact Legendre sums: no contraction into FMAs in this unit
         ("lw_setup_kernels.hip", ["-ffp-contract=off"]),
         ("lw_plan.cpp", ["-x", "hip"]),

@@ -661,10 +661,6 @@ static int init_device_state(Handle* h, const Geometry& g) {
 
 using lw::Handle;
 
-struct lw_handle {
-  Handle h;
-};
-
 extern "C" {
 
 const char* lw_last_error(void) { return lw::g_err.c_str(); }

@@ -116,3 +116,8 @@ int launch_setup_transfer(Handle* h, int level, const double* d_binom2, const do
 int launch_setup_near(Handle* h);
 
 }  // namespace lw
+
+// The opaque handle of the C ABI.
+struct lw_handle {
+  lw::Handle h;
+};

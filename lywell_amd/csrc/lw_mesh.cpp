@@ -261,3 +261,177 @@ int lw_tri_geometry(int np, const double* p_coord, int nt, const int* t_p, doubl
 }
 
 }  // extern "C"
+
+// ---------------------------------------------------------------------------
+// ASCII STL, the reference's input format (modReadStl.f90).
+//
+// Writer: coordinates in fixed-point decimal with 12 fractional digits and no
+// exponent, so that the reference's digit-accumulating parser (s_to_r8,
+// modReadStl.f90:133-266: an exact integer mantissa divided by an exact power
+// of ten) and a correctly rounded strtod read the same doubles.
+//
+// Reader: the facet vertices become 3*nf nodes in file order; duplicates are
+// then merged exactly as limpiarnodos does (modReadStl.f90:1444-1533): for
+// each node i in turn, the first later node with identical coordinates is
+// overwritten by the current last node, triangles pointing at it are moved to
+// i, triangles pointing at the last node are moved to its slot, and the node
+// count shrinks by one.  Hash maps replace the reference's O(P^2) scans.
+#include <cstdio>
+#include <map>
+#include <set>
+#include <string>
+
+namespace {
+
+struct Key3 {
+  double x, y, z;
+  bool operator<(const Key3& o) const {
+    if (x != o.x) return x < o.x;
+    if (y != o.y) return y < o.y;
+    return z < o.z;
+  }
+};
+
+struct StlData {
+  std::vector<double> p, n;  // (3, np) nodes after merging, (3, nt) facet normals
+  std::vector<int> t;        // (3, nt), 1-based
+};
+
+int parse_stl(const char* path, StlData& out) {
+  FILE* f = std::fopen(path, "r");
+  if (!f) return LW_ERR_ARG;
+  std::vector<double> node, nrm;
+  char line[512];
+  int state = 0, vert = 0;
+  while (std::fgets(line, sizeof line, f)) {
+    char w1[64] = {0}, w2[64] = {0};
+    double a = 0, b = 0, c = 0;
+    int nw = std::sscanf(line, "%63s %63s", w1, w2);
+    if (nw < 1) continue;
+    std::string s1(w1);
+    for (auto& ch : s1) ch = (char)std::tolower((unsigned char)ch);
+    if (s1 == "facet") {
+      if (std::sscanf(line, " %*s %*s %lf %lf %lf", &a, &b, &c) != 3) { a = b = c = 0.0; }
+      nrm.push_back(a);
+      nrm.push_back(b);
+      nrm.push_back(c);
+      state = 2;
+      vert = 0;
+    } else if (s1 == "vertex") {
+      if (state != 2 || std::sscanf(line, " %*s %lf %lf %lf", &a, &b, &c) != 3) {
+        std::fclose(f);
+        return LW_ERR_ARG;
+      }
+      node.push_back(a);
+      node.push_back(b);
+      node.push_back(c);
+      ++vert;
+    } else if (s1 == "endfacet") {
+      if (vert != 3) {
+        std::fclose(f);
+        return LW_ERR_ARG;
+      }
+      state = 1;
+    }
+  }
+  std::fclose(f);
+  const int nt = (int)(nrm.size() / 3);
+  if (nt < 1 || (int)node.size() != 9 * nt) return LW_ERR_ARG;
+  int np = 3 * nt;
+  std::vector<int> tri(3 * (size_t)nt);
+  for (int i = 0; i < 3 * nt; ++i) tri[i] = i + 1;
+  // node -> triangle corners referencing it; coordinates -> live positions
+  std::vector<std::vector<int>> refs((size_t)np + 1);
+  for (int c = 0; c < 3 * nt; ++c) refs[tri[c]].push_back(c);
+  std::map<Key3, std::set<int>> where;
+  auto key = [&](int id) { return Key3{node[3 * (id - 1)], node[3 * (id - 1) + 1], node[3 * (id - 1) + 2]}; };
+  for (int id = 1; id <= np; ++id) where[key(id)].insert(id);
+  auto retarget = [&](int from, int to) {
+    for (int c : refs[from]) tri[c] = to;
+    auto& dst = refs[to];
+    dst.insert(dst.end(), refs[from].begin(), refs[from].end());
+    refs[from].clear();
+  };
+  for (int i = 1; i <= np; ++i) {
+    for (;;) {
+      auto& s = where[key(i)];
+      auto it = s.upper_bound(i);
+      if (it == s.end()) break;
+      const int res = *it;
+      s.erase(it);
+      // node res := node np (then np is dropped)
+      if (res != np) {
+        where[key(np)].erase(np);
+        for (int d = 0; d < 3; ++d) node[3 * (res - 1) + d] = node[3 * (np - 1) + d];
+        where[key(res)].insert(res);
+      }
+      retarget(res, i);
+      if (res != np) retarget(np, res);
+      --np;
+    }
+  }
+  out.p.assign(node.begin(), node.begin() + 3 * (size_t)np);
+  out.t = tri;
+  out.n = nrm;
+  return LW_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int lw_stl_size(const char* path, int* np, int* nt) {
+  if (!path || !np || !nt) return LW_ERR_ARG;
+  StlData d;
+  int rc = parse_stl(path, d);
+  if (rc) return rc;
+  *np = (int)(d.p.size() / 3);
+  *nt = (int)(d.t.size() / 3);
+  return LW_OK;
+}
+
+int lw_stl_read(const char* path, double* p_coord, int* t_p, double* t_normal) {
+  if (!path || !p_coord || !t_p) return LW_ERR_ARG;
+  StlData d;
+  int rc = parse_stl(path, d);
+  if (rc) return rc;
+  std::memcpy(p_coord, d.p.data(), sizeof(double) * d.p.size());
+  std::memcpy(t_p, d.t.data(), sizeof(int) * d.t.size());
+  if (t_normal) std::memcpy(t_normal, d.n.data(), sizeof(double) * d.n.size());
+  return LW_OK;
+}
+
+int lw_stl_write(const char* path, int np, const double* p_coord, int nt, const int* t_p,
+                 const double* t_normal) {
+  if (!path || np < 3 || nt < 1 || !p_coord || !t_p) return LW_ERR_ARG;
+  for (int i = 0; i < 3 * np; ++i)
+    if (!(std::fabs(p_coord[i]) < 1.0e3)) return LW_ERR_ARG;  // keeps 16 significant digits exact
+  FILE* f = std::fopen(path, "w");
+  if (!f) return LW_ERR_ARG;
+  std::fprintf(f, "solid lywell\n");
+  for (int i = 0; i < nt; ++i) {
+    const int* q = t_p + 3 * i;
+    double n[3] = {0, 0, 0};
+    if (t_normal) {
+      for (int k = 0; k < 3; ++k) n[k] = t_normal[3 * i + k];
+    } else {
+      const double *A = p_coord + 3 * (q[0] - 1), *B = p_coord + 3 * (q[1] - 1), *C = p_coord + 3 * (q[2] - 1);
+      double e1[3] = {B[0] - A[0], B[1] - A[1], B[2] - A[2]}, e2[3] = {C[0] - A[0], C[1] - A[1], C[2] - A[2]};
+      n[0] = e1[1] * e2[2] - e1[2] * e2[1];
+      n[1] = e1[2] * e2[0] - e1[0] * e2[2];
+      n[2] = e1[0] * e2[1] - e1[1] * e2[0];
+      const double nn = std::sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+      for (int k = 0; k < 3; ++k) n[k] /= nn;
+    }
+    std::fprintf(f, "  facet normal %.12f %.12f %.12f\n    outer loop\n", n[0], n[1], n[2]);
+    for (int k = 0; k < 3; ++k) {
+      const double* P = p_coord + 3 * (q[k] - 1);
+      std::fprintf(f, "      vertex %.12f %.12f %.12f\n", P[0], P[1], P[2]);
+    }
+    std::fprintf(f, "    endloop\n  endfacet\n");
+  }
+  std::fprintf(f, "endsolid lywell\n");
+  return std::fclose(f) == 0 ? LW_OK : LW_ERR_ARG;
+}
+
+}  // extern "C"

@@ -43,4 +43,6 @@ for m in $MODS; do
 done
 "$FC" -O2 -I"$TMP" -c "$HERE/ref_driver.f90" -o "$TMP/ref_driver.o"
 "$FC" -O2 -o "$OUT/ref_driver" "$TMP/ref_driver.o" $objs
-echo "built $OUT/ref_driver"
+"$FC" -O2 -I"$TMP" -c "$HERE/ref_stl.f90" -o "$TMP/ref_stl.o"
+"$FC" -O2 -o "$OUT/ref_stl" "$TMP/ref_stl.o" $objs
+echo "built $OUT/ref_driver $OUT/ref_stl"

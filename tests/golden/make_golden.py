@@ -9,7 +9,9 @@ that holds the reference at /root/reference or $LYWELL_REFERENCE).
      after Interpolar/Anterpolar, and the sphere.stl surface itself;
    - ilu0_saad.npz: the 10x10 COO matrix {i,j,s,si}.dat and the ILU0 factor
      testSAAD wrote (main.f90:719-826, ilu0_{ao_real,ao_imag,ir,jc}).
-2. Golden vectors from the flang build of the reference (oracle/_ref,
+2. sphere.stl itself (gzip) and the arrays the reference's STL reader
+   returns for it (stl_sphere_ref.npz).
+3. Golden vectors from the flang build of the reference (oracle/_ref,
    oracle/build_ref.sh) on the 1,920-unknown sphere (radius 0.5 m, icosphere
    subdivision 3, lambda = 1 m, ladomax = 0.2 -> 4 levels):
    - ref_sphere1920.npz: the mesh fed to the reference, its RWG arrays,
@@ -150,7 +152,28 @@ def golden_from_reference_build():
         x_noprec=runs[0]["x"], sigma_noprec=runs[0]["sigma"])
 
 
+def run_ref_stl(path):
+    with tempfile.TemporaryDirectory() as td:
+        subprocess.run([os.path.join(ROOT, "oracle", "_ref", "ref_stl"), path, os.path.join(td, "o.bin")],
+                       check=True, stdout=subprocess.DEVNULL)
+        with open(os.path.join(td, "o.bin"), "rb") as f:
+            np_, nt = (int(v) for v in np.fromfile(f, np.int32, 2))
+            return np.fromfile(f, np.float64, 3 * np_), np.fromfile(f, np.int32, 3 * nt), np.fromfile(f, np.float64, 3 * nt)
+
+
+def stl_fixture():
+    """sphere.stl as shipped (gzip) and what the reference's reader returns for
+    it (obtener_p_t: node order after limpiarnodos, triangles, normals)."""
+    import gzip
+    src = os.path.join(REF, "sphere.stl")
+    with open(src, "rb") as f, gzip.open(os.path.join(HERE, "sphere.stl.gz"), "wb") as g:
+        g.write(f.read())
+    p, t, n = run_ref_stl(src)
+    np.savez_compressed(os.path.join(HERE, "stl_sphere_ref.npz"), p=p, t=t, n=n)
+
+
 if __name__ == "__main__":
     fixtures_from_reference_files()
+    stl_fixture()
     golden_from_reference_build()
     print("fixtures written to", HERE)

@@ -98,3 +98,42 @@ def test_init_without_device_fails_loudly():
     with pytest.raises(lw.LywellError) as e:
         lw.MLFMA(m, lam=1.0, ladomax=0.25)
     assert e.value.code == lw.LW_ERR_NODEV
+
+
+def _gunzip(tmp_path, name):
+    import gzip
+    src = os.path.join(ROOT, "tests", "golden", name)
+    dst = tmp_path / name[:-3]
+    with gzip.open(src, "rb") as g:
+        dst.write_bytes(g.read())
+    return str(dst)
+
+
+def test_stl_reader_matches_reference_reader(tmp_path):
+    # obtener_p_t on the reference's own sphere.stl: same merged nodes, node
+    # order, triangles and normals
+    path = _gunzip(tmp_path, "sphere.stl.gz")
+    m = lw.Mesh.from_stl(path)
+    g = np.load(os.path.join(ROOT, "tests", "golden", "stl_sphere_ref.npz"))
+    assert np.array_equal(m.p_coord, g["p"])
+    assert np.array_equal(m.t_p, g["t"])
+    assert np.array_equal(m.t_normal, g["n"])
+
+
+def test_stl_round_trip(tmp_path):
+    m = lw.Mesh.icosphere(3, 0.5)
+    path = str(tmp_path / "ico.stl")
+    m.write_stl(path)
+    r = lw.Mesh.from_stl(path)
+    assert r.np == m.np and r.nt == m.nt and r.ne == m.ne
+    assert np.abs(np.sort(r.p_coord) - np.sort(m.p_coord)).max() < 1e-12
+    ref_stl = os.path.join(ROOT, "oracle", "_ref", "ref_stl")
+    if os.path.exists(ref_stl):
+        out = str(tmp_path / "o.bin")
+        subprocess.run([ref_stl, path, out], check=True, stdout=subprocess.DEVNULL)
+        with open(out, "rb") as f:
+            npt, nt = (int(v) for v in np.fromfile(f, np.int32, 2))
+            p = np.fromfile(f, np.float64, 3 * npt)
+            t = np.fromfile(f, np.int32, 3 * nt)
+        # the reference's parser reads back exactly the doubles this reader returns
+        assert np.array_equal(p, r.p_coord) and np.array_equal(t, r.t_p)

@@ -1,0 +1,101 @@
+"""GPU checks of the multi-RHS form (FP32 far field, FP64 near field, ILU0
+solves and CGS) against the FP64 single-RHS path and the oracle."""
+import numpy as np
+import pytest
+
+import lywell_amd as lw
+from _oracle import Oracle
+
+pytestmark = pytest.mark.gpu
+
+_cache = {}
+
+
+def setup(R=16):
+    if R not in _cache:
+        print(f"[batch R={R}] building 7,680-unknown ellipsoid", flush=True)
+        mesh = lw.Mesh.icosphere(4, 1.6, 0.9, 0.7)
+        op = lw.MLFMA(mesh, lam=1.0, ladomax=0.2)
+        assert op.ilu0() == 0
+        _cache[R] = (mesh, op, lw.Batch(op, R))
+    return _cache[R]
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _need_device():
+    if lw.device_count() < 1:
+        pytest.skip("no HIP device")
+
+
+@pytest.mark.parametrize("R", [16, 64])
+def test_batch_matvec_fp32_far_field(R):
+    mesh, op, bt = setup(R)
+    rng = np.random.default_rng(10)
+    X = rng.standard_normal((mesh.ne, R)) + 1j * rng.standard_normal((mesh.ne, R))
+    Y = bt.matvec(X)
+    Y64 = np.stack([op.matvec(X[:, r]) for r in range(R)], axis=1)
+    # FP32 far field: relative error of the product at the FP32 level
+    errs = [rel(Y[:, r], Y64[:, r]) for r in range(R)]
+    assert max(errs) < 2e-5
+    # the near part is FP64: with x supported on one leaf box's neighbourhood the
+    # difference is the far field's only
+    assert np.all(np.isfinite(Y))
+
+
+def test_batch_matvec_against_oracle():
+    mesh, op, bt = setup(16)
+    ora = Oracle(mesh, lam=1.0, ladomax=0.2)
+    X = np.stack([lw.det_vector(mesh.ne) * (1 + 0.1j * r) for r in range(16)], axis=1)
+    Y = bt.matvec(X)
+    for r in (0, 7, 15):
+        assert rel(Y[:, r], ora.matvec(X[:, r])) < 2e-5
+
+
+def test_batch_lusol_matches_single():
+    mesh, op, bt = setup(16)
+    rng = np.random.default_rng(11)
+    Yv = rng.standard_normal((mesh.ne, 16)) + 1j * rng.standard_normal((mesh.ne, 16))
+    X = bt.lusol0(Yv)
+    for r in (0, 5, 15):
+        assert rel(X[:, r], op.lusol0(Yv[:, r])) < 1e-13
+
+
+def test_batch_plane_wave_and_rcs_match_single():
+    mesh, op, bt = setup(16)
+    ph = np.linspace(0, np.pi, 16, endpoint=False)
+    dirs = np.stack([-np.cos(ph), -np.sin(ph), np.zeros(16)], axis=1)
+    pols = np.tile(np.array([0, 0, 1.0], np.complex128), (16, 1))
+    B = bt.plane_wave(dirs, pols)
+    for r in (0, 9):
+        assert rel(B[:, r], op.plane_wave(dirs[r], pols[r])) < 1e-14
+    s = bt.rcs(B, pols, np.full(16, np.pi / 2), ph)
+    for r in (0, 9):
+        s1 = op.rcs(B[:, r], pols[r], np.pi / 2, ph[r])
+        assert abs(s[r] - s1[0]) <= 1e-10 * s1[0]
+
+
+@pytest.mark.parametrize("use_prec", [False, True])
+def test_batch_cgs_solves_every_column(use_prec):
+    mesh, op, bt = setup(16)
+    ph = np.linspace(0, np.pi, 16, endpoint=False)
+    dirs = np.stack([-np.cos(ph), -np.sin(ph), np.zeros(16)], axis=1)
+    pols = np.tile(np.array([0, 0, 1.0], np.complex128), (16, 1))
+    B = bt.plane_wave(dirs, pols)
+    X, it, rr, conv = bt.cgs(B, errmax=1e-4, use_prec=use_prec, maxit=3000)
+    assert conv.all() and (rr < 1e-4).all()
+    # true residual with the FP64 operator stays at the tolerance
+    for r in (0, 8, 15):
+        res = rel(op.matvec(X[:, r]), B[:, r])
+        assert res < 3e-4
+    # against independent FP64 single-RHS solves: same solution to the
+    # tolerance, RCS within 0.1 dB
+    for r in (0, 11):
+        x64, it64, rr64, c64 = op.cgs(B[:, r], errmax=1e-4, use_prec=use_prec, maxit=3000)
+        assert c64
+        s32 = op.rcs(X[:, r], pols[r], np.pi / 2, ph[r])[0]
+        s64 = op.rcs(x64, pols[r], np.pi / 2, ph[r])[0]
+        assert abs(10 * np.log10(s32 / s64)) < 0.1
