@@ -47,8 +47,13 @@ def main():
     ap.add_argument("--ladomax", type=float, default=0.25)
     ap.add_argument("--errmax", type=float, default=1e-4)
     ap.add_argument("--maxit", type=int, default=20000)
-    ap.add_argument("--no-prec", action="store_true", help="CGS without ILU0")
+    ap.add_argument("--prec", choices=["none", "ilu0"], default="none",
+                    help="preconditioner of the timed solves (ILU0 diverges on this mesh; see DESIGN.md)")
+    ap.add_argument("--ilu0-probe", type=int, default=20,
+                    help="ILU0-CGS iterations run beside the timed solves to report the ILU0 path (0: skip)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
+    ap.add_argument("--batch", type=int, default=64,
+                    help="right-hand sides per block of the multi-RHS FP32-far-field form (0: skip)")
     ap.add_argument("--cpu-iters", type=int, default=3)
     args = ap.parse_args()
 
@@ -57,6 +62,7 @@ def main():
                           "n_gpus": args.gpus}))
         sys.exit(2)
 
+    os.environ.setdefault("LW_PROGRESS", "200")  # solver progress lines on stderr
     import lywell_amd as lw
     if lw.device_count() < 1:
         raise SystemExit("bench: no HIP device visible")
@@ -69,14 +75,16 @@ def main():
     setup_s = time.time() - t0
     levels = [op.level_info(l) for l in range(1, op.nlevels + 1)]
     log(f"operator: {op.nlevels} levels, near nnz {op.nnz}, setup {setup_s:.1f}s")
-    use_prec = not args.no_prec
+    use_prec = args.prec == "ilu0"
     ilu_s = 0.0
-    if use_prec:
+    have_ilu = False
+    if use_prec or args.ilu0_probe > 0:
         t0 = time.time()
         ierr = op.ilu0()
         ilu_s = time.time() - t0
+        have_ilu = ierr == 0
         log(f"ilu0: ierr {ierr}, {ilu_s:.1f}s")
-        if ierr != 0:
+        if not have_ilu:
             use_prec = False
 
     # right-hand sides: x-polarised waves along -z, then rotated in azimuth
@@ -89,7 +97,7 @@ def main():
     # per-kernel timing (HIP events on the operator's stream)
     mv_ms, stages = op.bench_matvec(reps=10)
     bytes_ = op.matvec_bytes()
-    lus_ms = op.bench_lusol(reps=3) if use_prec else 0.0
+    lus_ms = op.bench_lusol(reps=3) if have_ilu else 0.0
     log(f"matvec {mv_ms:.3f} ms (stages {np.round(stages, 3).tolist()}), lusol {lus_ms:.3f} ms")
 
     bs = [rhs(i) for i in range(args.warmup + args.steps)]
@@ -108,6 +116,13 @@ def main():
     # true residual of the last solution with the GPU operator
     true_res = float(np.linalg.norm(op.matvec(x) - bs[-1]) / np.linalg.norm(bs[-1]))
 
+    ilu_probe = None
+    if have_ilu and args.ilu0_probe > 0 and not use_prec:
+        _, itp, rrp, cvp = op.cgs(bs[-1], errmax=args.errmax, use_prec=True, maxit=args.ilu0_probe)
+        ilu_probe = {"iterations": itp, "relres": rrp, "converged": cvp, "lusol_ms": round(lus_ms, 3),
+                     "ilu0_factor_s": round(ilu_s, 2)}
+        log(f"ilu0-cgs probe: {itp} iterations, relres {rrp:.3e}")
+
     # dominant kernel of a solve: per CGS iteration 2 matvecs + 2 lusol
     it_mean = float(np.mean(iters))
     per_iter_ms = wall * 1e3 / max(1.0, float(np.sum(iters)))
@@ -117,25 +132,33 @@ def main():
         cand["lusol"] = 2 * lus_ms
     dom = max(cand, key=cand.get)
     nnz = op.nnz
+    sbytes = op.matvec_stage_bytes()
+    kernel_of = {"aggregation": "k_aggregate", "interp_up": "k_interp_up", "translation": "k_translate",
+                 "interp_down": "k_interp_down", "receive_near": "k_receive_near", "lusol": "k_lusol"}
+    launches = {"aggregation": 1, "interp_up": op.nlevels - 3, "translation": op.nlevels - 2,
+                "interp_down": op.nlevels - 3, "receive_near": 1, "lusol": 1}
     if dom == "lusol":
         # forward + backward sweep over the MSR factor: complex(4) values and
         # int32 columns once, the row pointers, y read, x written and gathered
         alg = 12.0 * nnz + 8.0 * (mesh.ne + 1) + 16.0 * mesh.ne * 3
         dom_ms = lus_ms
-    elif dom == "receive_near":
-        alg = bytes_["receive_near"]
-        dom_ms = stages[4]
-    elif dom == "aggregation":
-        alg = bytes_["aggregation"]
-        dom_ms = stages[0]
     else:
-        alg = bytes_["total"]
-        dom_ms = mv_ms
+        j = stage_names.index(dom)
+        alg = float(sbytes[j])
+        dom_ms = float(stages[j])
+    # per launch: the stage's launches share the bytes and the time
+    nl = launches[dom]
     achieved = alg / (dom_ms * 1e-3) / 1e9
-    roofline = {"bound": "hbm", "kernel": dom, "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+    roofline = {"bound": "hbm", "kernel": kernel_of[dom], "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
-                "algorithmic_bytes": alg, "mean_ms": round(dom_ms, 4)}
+                "algorithmic_bytes_per_launch": alg / nl, "mean_launch_ms": round(dom_ms / nl, 4),
+                "launches_per_matvec": nl}
+    stage_gbs = {n: round(float(b) / (float(t) * 1e-3) / 1e9, 1) for n, b, t in zip(stage_names, sbytes, stages)}
     matvec_gbs = bytes_["total"] / (mv_ms * 1e-3) / 1e9
+
+    batched = None
+    if args.batch > 0:
+        batched = batched_sweep(lw, op, args, x, bs[-1], rhs_per_s)
 
     cpu = None
     if not args.no_cpu:
@@ -169,13 +192,51 @@ def main():
         "matvec_ms": round(mv_ms, 4),
         "matvec_stage_ms": dict(zip(stage_names, [round(float(s), 4) for s in stages])),
         "matvec_GBps_algorithmic": round(matvec_gbs, 1),
+        "stage_GBps_algorithmic": stage_gbs,
         "lusol_ms": round(lus_ms, 4),
         "setup_s": round(setup_s, 2),
         "ilu0_s": round(ilu_s, 2),
+        "ilu0_cgs_probe": ilu_probe,
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "batched": batched,
     }
     print(json.dumps(out))
+
+
+def batched_sweep(lw, op, args, x64, b64, plain_rhs_per_s):
+    """The multi-RHS form on the same mesh: a block of R monostatic incidence
+    angles (azimuth sweep in the xy-plane, VV), FP32 far field, FP64 near field
+    and CGS, solved together to the same 1e-4 tolerance.  Its RCS is compared
+    with FP64 single-RHS solves of the same angles."""
+    R = args.batch
+    bt = lw.Batch(op, R)
+    mv_ms, _ = bt.bench(reps=3)
+    log(f"batched matvec (R={R}): {mv_ms:.3f} ms = {mv_ms / R * 1e3:.1f} us per RHS")
+    ph = np.pi * np.arange(R) / R
+    dirs = np.stack([-np.cos(ph), -np.sin(ph), np.zeros(R)], axis=1)
+    pols = np.tile(np.array([0, 0, 1.0], np.complex128), (R, 1))
+    B = bt.plane_wave(dirs, pols)
+    t0 = time.time()
+    X, it, rr, conv = bt.cgs(B, errmax=args.errmax, use_prec=False, maxit=args.maxit)
+    wall = time.time() - t0
+    log(f"batched CGS: {wall:.2f}s for {R} RHS, iterations {int(it.min())}-{int(it.max())}, all converged {conv.all()}")
+    sig = bt.rcs(X, pols, np.full(R, np.pi / 2), ph)
+    # FP64 single-RHS references for two of the angles
+    errs = []
+    for r in (0, R // 2):
+        x, it64, rr64, c64 = op.cgs(B[:, r], errmax=args.errmax, use_prec=False, maxit=args.maxit)
+        s64 = op.rcs(x, pols[r], np.pi / 2, ph[r])[0]
+        errs.append(abs(10 * np.log10(sig[r] / s64)))
+        log(f"  angle {r}: FP64 {it64} iterations, batched {int(it[r])}; RCS difference {errs[-1]:.4f} dB")
+    bt.close()
+    value = R / wall
+    return {"nrhs": R, "rhs_per_s": round(value, 4), "speedup_vs_plain": round(value / plain_rhs_per_s, 2),
+            "matvec_ms": round(mv_ms, 3), "matvec_ms_per_rhs": round(mv_ms / R, 4),
+            "iterations_min": int(it.min()), "iterations_max": int(it.max()),
+            "converged": bool(conv.all()), "rcs_db_err_vs_fp64": [round(e, 5) for e in errs],
+            "precision": "FP32 far field (aggregation, interpolation, translation, receive), FP64 near field, "
+                         "ILU0 and CGS"}
 
 
 def cpu_baseline(mesh, args, use_prec, gpu_iters):

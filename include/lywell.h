@@ -156,6 +156,42 @@ int lw_bench_lusol(lw_handle *h, int reps, double *ms);
  * (pattern arrays twice, campo arrays 6x, transfer tables once, near CSR,
  * x and y), and of the dominant kernels. */
 int lw_matvec_bytes(lw_handle *h, double *total, double *aggregation, double *receive_near);
+/* Algorithmic bytes of each matvec stage, in the order of lw_bench_matvec's
+ * stage_ms: aggregation, upward passes, translations, downward passes,
+ * receive + near field. */
+int lw_matvec_stage_bytes(lw_handle *h, double *stage_bytes);
+
+/* ---- multi-RHS form ------------------------------------------------------
+ * nrhs right-hand sides of one mesh through one operator pass.  The far field
+ * (aggregation, interpolation, translation, disaggregation, receive) runs in
+ * FP32; the near field, the ILU0 solves and the CGS scalars stay in FP64.
+ * Blocks are complex(8) arrays of shape (nrhs, ne) in Fortran order
+ * (x(r,i) = right-hand side r, unknown i: nrhs values contiguous per unknown).
+ * A batch shares the handle's stream and operators; one batch per handle at a
+ * time.  No reference counterpart: the reference solves one RHS per CGSold
+ * call (modIterativo.f90:331-493) from a loop over incidence angles. */
+typedef struct lw_batch lw_batch;
+
+/* nrhs in {8, 16, 32, 64}. */
+int lw_batch_create(lw_handle *h, int nrhs, lw_batch **out);
+int lw_batch_destroy(lw_batch *b);
+/* The block width, or LW_ERR_ARG. */
+int lw_batch_nrhs(lw_batch *b);
+/* Y = Z X column by column (MatxVec_MLFMA per column, FP32 far field). */
+int lw_batch_matvec(lw_batch *b, const double *x, double *y);
+/* lusol0C per column with the handle's ILU0 factor. */
+int lw_batch_lusol0(lw_batch *b, const double *y, double *x);
+/* CGSold per column: each column stops at its own relative residual errmax
+ * (its later updates are masked); iters/relres/converged are per column. */
+int lw_batch_cgs(lw_batch *b, const double *rhs, const double *guess /* NULL: zero */, double errmax,
+                 int use_prec, int maxit, double *x, int *iters, double *relres, int *converged);
+/* generar_onda per column: dirs (3,nrhs) real, pols (3,nrhs) complex(8). */
+int lw_batch_plane_wave(lw_batch *b, const double *dirs, const double *pols, const double *ctte, double *rhs);
+/* sigmaRCS per column at one (theta(r), phi(r)) each: the monostatic sweep. */
+int lw_batch_rcs(lw_batch *b, const double *cur, const double *pols, const double *ctte, const double *theta,
+                 const double *phi, double *sigma);
+/* Mean batched matvec time and one batched lusol0 time in ms (HIP events). */
+int lw_bench_batch(lw_batch *b, int reps, double *matvec_ms, double *lusol_ms);
 
 #ifdef __cplusplus
 }

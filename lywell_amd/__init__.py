@@ -286,6 +286,11 @@ class MLFMA:
         _check(lib().lw_bench_lusol(self.h, int(reps), _C.byref(ms)))
         return ms.value
 
+    def matvec_stage_bytes(self):
+        sb = _np.zeros(5)
+        _check(lib().lw_matvec_stage_bytes(self.h, _P(sb)))
+        return sb
+
     def matvec_bytes(self):
         t, a, r = _C.c_double(), _C.c_double(), _C.c_double()
         _check(lib().lw_matvec_bytes(self.h, _C.byref(t), _C.byref(a), _C.byref(r)))

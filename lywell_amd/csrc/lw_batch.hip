@@ -20,6 +20,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdio>
 #include <cstring>
 #include <vector>
 
@@ -61,20 +62,6 @@ __global__ void k_to_f32(long long n, const double2* __restrict__ in, float2* __
 __global__ void k_to_f32r(long long n, const double* __restrict__ in, float* __restrict__ out) {
   const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < n) out[i] = (float)in[i];
-}
-
-// exp(+j k0 khat_i . r_o) for the 8 child offsets r_o = (+-q, +-q, +-q),
-// octant bits (x,y,z) = (4,2,1), 1 = positive side
-__global__ void k_phase_table(int nk, const double* __restrict__ kap, double q, double k0,
-                              float2* __restrict__ ph) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nk) return;
-  for (int o = 0; o < 8; ++o) {
-    const double rx = (o & 4) ? q : -q, ry = (o & 2) ? q : -q, rz = (o & 1) ? q : -q;
-    double s, c;
-    sincos(k0 * ((kap[3 * i] * rx + kap[3 * i + 1] * ry) + kap[3 * i + 2] * rz), &s, &c);
-    ph[(size_t)o * nk + i] = make_float2((float)c, (float)s);
-  }
 }
 
 // ---------------------------------------------------------------- far field (FP32)
@@ -624,7 +611,6 @@ int lw_batch_create(lw_handle* hh, int nrhs, lw_batch** out) {
   TRY(bmalloc(bt, &bt->rad32, (size_t)nrad));
   hipLaunchKernelGGL(k_to_f32, dim3(nblk(nrad, 256)), dim3(256), 0, st, nrad, h->d_rad, bt->rad32);
   bt->lv.assign(L + 1, BatchLevel());
-  const double k0 = 2.0 * kPi / P.phys.lambda;
   for (int n = 3; n <= L; ++n) {
     const LevelPlan& lp = P.lv[n];
     DevLevel& d = h->lv[n];
@@ -642,29 +628,11 @@ int lw_batch_create(lw_handle* hh, int nrhs, lw_batch** out) {
       TRY(bmalloc(bt, &b.wt_w, lp.wt_w.size()));
       hipLaunchKernelGGL(k_to_f32r, dim3(nblk((long long)lp.wt_w.size(), 256)), dim3(256), 0, st,
                          (long long)lp.wt_w.size(), d.wt_w, b.wt_w);
-      // child offset phases at this level's samples: children sit at +-lado/4
+      // child offset phases at this level's samples (the FP64 table, rounded)
       TRY(bmalloc(bt, &b.ph, 8 * (size_t)lp.nk));
-      hipLaunchKernelGGL(k_phase_table, dim3(nblk(lp.nk, 256)), dim3(256), 0, st, lp.nk, d.kap, lp.lado / 4.0, k0,
-                         b.ph);
+      hipLaunchKernelGGL(k_to_f32, dim3(nblk(8LL * lp.nk, 256)), dim3(256), 0, st, 8LL * lp.nk, d.ph, b.ph);
     }
-    if (n > 3) {
-      const LevelPlan& pp = P.lv[n - 1];
-      std::vector<int> oc(lp.nb);
-      for (int c = 0; c < lp.nb; ++c) {
-        const double* cc = &lp.centre[3 * c];
-        const double* pc = &pp.centre[3 * lp.parent[c]];
-        oc[c] = ((cc[0] > pc[0]) ? 4 : 0) | ((cc[1] > pc[1]) ? 2 : 0) | ((cc[2] > pc[2]) ? 1 : 0);
-      }
-      TRY(bmalloc(bt, &b.oct, oc.size()));
-      if (!oc.empty()) {
-        hipError_t e = hipMemcpy(b.oct, oc.data(), oc.size() * sizeof(int), hipMemcpyHostToDevice);
-        if (e != hipSuccess) {
-          set_error(hipGetErrorString(e));
-          free_batch(bt);
-          return LW_ERR_HIP;
-        }
-      }
-    }
+    b.oct = d.oct;
     const size_t fsz = (size_t)lp.nk * lp.nb * 2 * R;
     TRY(bmalloc(bt, &b.C, fsz));
     TRY(bmalloc(bt, &b.B, fsz));
@@ -805,6 +773,16 @@ int lw_batch_cgs(lw_batch* ob, const double* Bv, const double* guess, double err
       } else if (hr[c].x == 0.0 && hr[c].y == 0.0) {
         act[c] = 0;  // breakdown exit of the next iteration
       }
+    }
+    if (progress_every() > 0 && (k + 1) % progress_every() == 0) {
+      int na = 0;
+      double worst = 0.0;
+      for (int c = 0; c < R; ++c) {
+        na += act[c];
+        if (act[c]) worst = std::max(worst, rr[c]);
+      }
+      std::fprintf(stderr, "[lw_batch_cgs] iteration %d: %d of %d columns active, worst relres %.3e\n", k + 1, na, R,
+                   worst);
     }
     // rho2 = rho1; rho1 = next (columns that stopped no longer read them)
     LW_HIP(hipMemcpyAsync(rho2, rho1, sizeof(double2) * R, hipMemcpyDeviceToDevice, st));

@@ -14,6 +14,8 @@
 #include <algorithm>
 #include <chrono>
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <mutex>
 #include <string>
@@ -25,6 +27,16 @@ namespace lw {
 
 static thread_local std::string g_err;
 void set_error(const std::string& m) { g_err = m; }
+
+// LW_PROGRESS=n makes the iterative solvers report every n iterations on
+// stderr (long solves otherwise print nothing).
+int progress_every() {
+  static const int every = [] {
+    const char* v = std::getenv("LW_PROGRESS");
+    return v ? std::atoi(v) : 0;
+  }();
+  return every;
+}
 
 namespace {
 
@@ -57,18 +69,36 @@ __device__ __forceinline__ double wave_sum(double v) {
 
 // ---------------------------------------------------------------- matvec
 
+// Workgroups are dispatched to the 8 XCDs round-robin by linear id.  This maps
+// the linear id to a work index so that each XCD receives one contiguous range
+// (a bijection on [0, n)): neighbouring boxes, whose fields and interaction
+// lists overlap, then share one XCD's L2.
+__device__ __forceinline__ int xcd_block(int w, int n) {
+  const int q = n / 8, rem = n % 8, x = w % 8, j = w / 8;
+  return x * q + min(x, rem) + j;
+}
+
 // Leaf aggregation (modMLFMA.f90:1805-1812): campoC_L(:,:,box) =
 // sum_f funcRAD(:,:,f) x(f).  One workgroup per leaf box.
 __global__ void __launch_bounds__(256) k_aggregate(int nk2, const int* __restrict__ box_slot,
                                                    const int* __restrict__ slot_basis,
                                                    const double2* __restrict__ rad,
                                                    const double2* __restrict__ x, double2* __restrict__ C) {
-  const int b = blockIdx.x;
+  const int b = xcd_block(blockIdx.x, gridDim.x);
   const int s0 = box_slot[b], s1 = box_slot[b + 1];
-  for (int e = threadIdx.x; e < nk2; e += blockDim.x) {
-    double2 acc = make_double2(0.0, 0.0);
-    for (int s = s0; s < s1; ++s) acc = cadd(acc, cmul(rad[(size_t)s * nk2 + e], x[slot_basis[s]]));
+  // two samples per thread per pass: independent accumulators keep more
+  // pattern loads in flight
+  for (int e = threadIdx.x; e < nk2; e += 2 * blockDim.x) {
+    const int e2 = e + blockDim.x;
+    const bool two = e2 < nk2;
+    double2 acc = make_double2(0.0, 0.0), acc2 = make_double2(0.0, 0.0);
+    for (int s = s0; s < s1; ++s) {
+      const double2 xv = x[slot_basis[s]];
+      acc = cadd(acc, cmul(rad[(size_t)s * nk2 + e], xv));
+      if (two) acc2 = cadd(acc2, cmul(rad[(size_t)s * nk2 + e2], xv));
+    }
     C[(size_t)b * nk2 + e] = acc;
+    if (two) C[(size_t)b * nk2 + e2] = acc2;
   }
 }
 
@@ -77,30 +107,22 @@ __global__ void __launch_bounds__(256) k_aggregate(int nk2, const int* __restric
 // to the parent centre and sum over children in box order.  Grid: (parent,
 // chunk of 256 parent samples); the child field is staged in LDS.
 __global__ void __launch_bounds__(256) k_interp_up(int nkc, int nkf, const int* __restrict__ child_ptr,
-                                                   const double* __restrict__ pcentre,
-                                                   const double* __restrict__ ccentre,
-                                                   const double* __restrict__ kapc,
+                                                   const int* __restrict__ oct, const double2* __restrict__ ph,
                                                    const double* __restrict__ wint, const int* __restrict__ iint,
-                                                   const double2* __restrict__ Cf, double2* __restrict__ Cc,
-                                                   double k0) {
+                                                   const double2* __restrict__ Cf, double2* __restrict__ Cc) {
   extern __shared__ double2 lds[];  // 2 * nkf
   const int P = blockIdx.x;
   const int i = blockIdx.y * blockDim.x + threadIdx.x;
   double2 acc0 = make_double2(0.0, 0.0), acc1 = make_double2(0.0, 0.0);
-  double kx = 0, ky = 0, kz = 0;
   double w[9];
   int id[9];
   if (i < nkc) {
-    kx = kapc[3 * i];
-    ky = kapc[3 * i + 1];
-    kz = kapc[3 * i + 2];
 #pragma unroll
     for (int j = 0; j < 9; ++j) {
       w[j] = wint[9 * (size_t)i + j];
       id[j] = iint[9 * (size_t)i + j];
     }
   }
-  const double* pc = pcentre + 3 * P;
   for (int c = child_ptr[P]; c < child_ptr[P + 1]; ++c) {
     __syncthreads();
     const double2* src = Cf + (size_t)c * 2 * nkf;
@@ -116,13 +138,10 @@ __global__ void __launch_bounds__(256) k_interp_up(int nkc, int nkf, const int* 
         a1.x += w[j] * v1.x;
         a1.y += w[j] * v1.y;
       }
-      const double* cc = ccentre + 3 * c;
-      const double r0 = cc[0] - pc[0], r1 = cc[1] - pc[1], r2 = cc[2] - pc[2];
-      double sn, cs;
-      sincos(k0 * ((kx * r0 + ky * r1) + kz * r2), &sn, &cs);
-      const double2 ph = make_double2(cs, sn);
-      acc0 = cadd(acc0, cmul(a0, ph));
-      acc1 = cadd(acc1, cmul(a1, ph));
+      // exp(+j k khat_i . (c_child - c_parent)) from the child's octant
+      const double2 e = ph[(size_t)oct[c] * nkc + i];
+      acc0 = cadd(acc0, cmul(a0, e));
+      acc1 = cadd(acc1, cmul(a1, e));
     }
   }
   if (i < nkc) {
@@ -137,8 +156,10 @@ __global__ void __launch_bounds__(256) k_translate(int nk, const int* __restrict
                                                    const int* __restrict__ far_box, const int* __restrict__ far_tr,
                                                    const double2* __restrict__ T, const double* __restrict__ pes,
                                                    const double2* __restrict__ C, double2* __restrict__ B) {
-  const int b = blockIdx.x;
-  const int k = blockIdx.y * blockDim.x + threadIdx.x;
+  const int nch = (nk + blockDim.x - 1) / blockDim.x;
+  const int wb = xcd_block(blockIdx.x, gridDim.x);
+  const int b = wb / nch;
+  const int k = (wb % nch) * blockDim.x + threadIdx.x;
   if (k >= nk) return;
   double2 a0 = make_double2(0.0, 0.0), a1 = make_double2(0.0, 0.0);
   for (int m = far_ptr[b]; m < far_ptr[b + 1]; ++m) {
@@ -157,37 +178,95 @@ __global__ void __launch_bounds__(256) k_translate(int nk, const int* __restrict
 // gather form) and add the child's own translated field.  One workgroup per
 // child box; the shifted parent field is staged in LDS.
 __global__ void __launch_bounds__(256) k_interp_down(int nkc, int nkf, const int* __restrict__ parent,
-                                                     const double* __restrict__ pcentre,
-                                                     const double* __restrict__ ccentre,
-                                                     const double* __restrict__ kapc,
-                                                     const int* __restrict__ wt_ptr, const int* __restrict__ wt_row,
-                                                     const double* __restrict__ wt_w,
+                                                     const int* __restrict__ oct, const double2* __restrict__ ph,
+                                                     const int* __restrict__ wt_ptr, const WtEntry* __restrict__ wt,
                                                      const double2* __restrict__ Ap, const double2* __restrict__ Bf,
-                                                     double2* __restrict__ Af, double k0) {
-  extern __shared__ double2 lds[];  // 2 * nkc
-  const int c = blockIdx.x;
+                                                     double2* __restrict__ Af) {
+  // grid: (child box, chunk of 256 fine samples), XCD-contiguous
+  const int nch = (nkf + blockDim.x - 1) / blockDim.x;
+  const int wb = xcd_block(blockIdx.x, gridDim.x);
+  const int c = wb / nch;
+  const int s = (wb % nch) * blockDim.x + threadIdx.x;
+  if (s >= nkf) return;
   const int P = parent[c];
-  const double* pc = pcentre + 3 * P;
-  const double* cc = ccentre + 3 * c;
-  const double r0 = cc[0] - pc[0], r1 = cc[1] - pc[1], r2 = cc[2] - pc[2];
   const double2* src = Ap + (size_t)P * 2 * nkc;
+  const double2* e = ph + (size_t)oct[c] * nkc;  // exp(+j k khat . r); conjugated below
+  double2 a0 = make_double2(0.0, 0.0), a1 = make_double2(0.0, 0.0);
+  const int q0 = wt_ptr[s], q1 = wt_ptr[s + 1];
+  int q = q0;
+  // four stencil entries per step: their loads are issued together
+  for (; q + 4 <= q1; q += 4) {
+    WtEntry t[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) t[u] = wt[q + u];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const int r = t[u].r;
+      const double2 pr = e[r];
+      const double2 v0 = cmulc(pr, src[r]), v1 = cmulc(pr, src[nkc + r]);
+      a0.x += t[u].w * v0.x;
+      a0.y += t[u].w * v0.y;
+      a1.x += t[u].w * v1.x;
+      a1.y += t[u].w * v1.y;
+    }
+  }
+  for (; q < q1; ++q) {
+    const WtEntry t = wt[q];
+    const double2 pr = e[t.r];
+    const double2 v0 = cmulc(pr, src[t.r]), v1 = cmulc(pr, src[nkc + t.r]);
+    a0.x += t.w * v0.x;
+    a0.y += t.w * v0.y;
+    a1.x += t.w * v1.x;
+    a1.y += t.w * v1.y;
+  }
+  const size_t o0 = ((size_t)c * 2 + 0) * nkf + s, o1 = ((size_t)c * 2 + 1) * nkf + s;
+  Af[o0] = cadd(Bf[o0], a0);
+  Af[o1] = cadd(Bf[o1], a1);
+}
+
+// Downward pass, LDS-staged variant for levels whose parent field fits in
+// LDS: one workgroup per child box stages the phase-shifted parent field once
+// and every fine sample of the child gathers from it.
+__global__ void __launch_bounds__(512) k_interp_down_lds(int nkc, int nkf, const int* __restrict__ parent,
+                                                         const int* __restrict__ oct, const double2* __restrict__ ph,
+                                                         const int* __restrict__ wt_ptr,
+                                                         const WtEntry* __restrict__ wt,
+                                                         const double2* __restrict__ Ap,
+                                                         const double2* __restrict__ Bf, double2* __restrict__ Af) {
+  extern __shared__ double2 lds[];  // 2 * nkc
+  const int c = xcd_block(blockIdx.x, gridDim.x);
+  const int P = parent[c];
+  const double2* src = Ap + (size_t)P * 2 * nkc;
+  const double2* e = ph + (size_t)oct[c] * nkc;
   for (int i = threadIdx.x; i < nkc; i += blockDim.x) {
-    double sn, cs;
-    sincos(-(k0 * ((kapc[3 * i] * r0 + kapc[3 * i + 1] * r1) + kapc[3 * i + 2] * r2)), &sn, &cs);
-    const double2 ph = make_double2(cs, sn);
-    lds[i] = cmul(src[i], ph);
-    lds[nkc + i] = cmul(src[nkc + i], ph);
+    const double2 pr = e[i];
+    lds[i] = cmulc(pr, src[i]);
+    lds[nkc + i] = cmulc(pr, src[nkc + i]);
   }
   __syncthreads();
   for (int s = threadIdx.x; s < nkf; s += blockDim.x) {
     double2 a0 = make_double2(0.0, 0.0), a1 = make_double2(0.0, 0.0);
-    for (int e = wt_ptr[s]; e < wt_ptr[s + 1]; ++e) {
-      const double w = wt_w[e];
-      const int r = wt_row[e];
-      a0.x += w * lds[r].x;
-      a0.y += w * lds[r].y;
-      a1.x += w * lds[nkc + r].x;
-      a1.y += w * lds[nkc + r].y;
+    const int q0 = wt_ptr[s], q1 = wt_ptr[s + 1];
+    int q = q0;
+    for (; q + 4 <= q1; q += 4) {
+      WtEntry t[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) t[u] = wt[q + u];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const double2 v0 = lds[t[u].r], v1 = lds[nkc + t[u].r];
+        a0.x += t[u].w * v0.x;
+        a0.y += t[u].w * v0.y;
+        a1.x += t[u].w * v1.x;
+        a1.y += t[u].w * v1.y;
+      }
+    }
+    for (; q < q1; ++q) {
+      const WtEntry t = wt[q];
+      a0.x += t.w * lds[t.r].x;
+      a0.y += t.w * lds[t.r].y;
+      a1.x += t.w * lds[nkc + t.r].x;
+      a1.y += t.w * lds[nkc + t.r].y;
     }
     const size_t o0 = ((size_t)c * 2 + 0) * nkf + s, o1 = ((size_t)c * 2 + 1) * nkf + s;
     Af[o0] = cadd(Bf[o0], a0);
@@ -198,17 +277,20 @@ __global__ void __launch_bounds__(256) k_interp_down(int nkc, int nkf, const int
 // Receive (modMLFMA.f90:1915-1933, funcREC = conjg(funcRAD) under EFIE) and
 // near-field CSR product (:1935-1942), one wavefront per row.
 template <typename V>
-__global__ void __launch_bounds__(256) k_receive_near(int ne, int nk2, const int* __restrict__ basis_slot,
+__global__ void __launch_bounds__(256) k_receive_near(int ne, int nk2, const int* __restrict__ slot_basis,
                                                       const int* __restrict__ basis_box,
                                                       const double2* __restrict__ rad,
                                                       const double2* __restrict__ AL, const int* __restrict__ ia,
                                                       const int* __restrict__ ja, const V* __restrict__ z,
                                                       const double2* __restrict__ x, double2 cc,
                                                       double2* __restrict__ y, double2* __restrict__ y_near) {
-  const int m = blockIdx.x * (blockDim.x / kWave) + threadIdx.x / kWave;
+  // rows taken in leaf-slot order: a box's campoA is then read by neighbouring
+  // wavefronts of one XCD
+  const int slot = xcd_block(blockIdx.x, gridDim.x) * (blockDim.x / kWave) + threadIdx.x / kWave;
   const int lane = threadIdx.x % kWave;
-  if (m >= ne) return;
-  const double2* r = rad + (size_t)basis_slot[m] * nk2;
+  if (slot >= ne) return;
+  const int m = slot_basis[slot];
+  const double2* r = rad + (size_t)slot * nk2;
   const double2* a = AL + (size_t)basis_box[m] * nk2;
   double2 f = make_double2(0.0, 0.0);
   for (int e = lane; e < nk2; e += kWave) f = cadd(f, cmulc(r[e], a[e]));
@@ -459,9 +541,25 @@ __global__ void __launch_bounds__(256) k_rcs(int ne, const double* __restrict__ 
   }
 }
 
+// child-offset phase table of one level (see DevLevel::ph)
+__global__ void k_phase_table64(int nk, const double* __restrict__ kap, double q, double k0,
+                                double2* __restrict__ ph) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nk) return;
+  for (int o = 0; o < 8; ++o) {
+    const double rx = (o & 4) ? q : -q, ry = (o & 2) ? q : -q, rz = (o & 1) ? q : -q;
+    double s, c;
+    sincos(k0 * ((kap[3 * i] * rx + kap[3 * i + 1] * ry) + kap[3 * i + 2] * rz), &s, &c);
+    ph[(size_t)o * nk + i] = make_double2(c, s);
+  }
+}
+
 // ---------------------------------------------------------------- helpers
 
 int blocks_for(long long n, int b) { return (int)((n + b - 1) / b); }
+
+// parent fields up to this size are staged in LDS by the downward pass
+constexpr int kDownLdsBytes = 64 * 1024;
 
 // one full matvec on device vectors x -> y (y_near: optional split output)
 int run_matvec(Handle* h, const double2* x, double2* y, double2* y_near, hipEvent_t* ev = nullptr) {
@@ -472,18 +570,17 @@ int run_matvec(Handle* h, const double2* x, double2* y, double2* y_near, hipEven
   hipLaunchKernelGGL(k_aggregate, dim3(lf.nb), dim3(256), 0, st, 2 * lf.nk, h->d_box_slot, h->d_slot_basis,
                      h->d_rad, x, lf.C);
   if (ev) LW_HIP(hipEventRecord(ev[1], st));
-  const double k0 = (2.0 * kPi) / h->plan.phys.lambda;
   for (int n = L; n >= 4; --n) {
     DevLevel& c = h->lv[n];
     DevLevel& p = h->lv[n - 1];
     dim3 grid(p.nb, blocks_for(p.nk, 256));
-    hipLaunchKernelGGL(k_interp_up, grid, dim3(256), 2 * c.nk * sizeof(double2), st, p.nk, c.nk, p.child_ptr,
-                       p.centre, c.centre, p.kap, p.wint, p.iint, c.C, p.C, k0);
+    hipLaunchKernelGGL(k_interp_up, grid, dim3(256), 2 * c.nk * sizeof(double2), st, p.nk, c.nk, p.child_ptr, c.oct,
+                       p.ph, p.wint, p.iint, c.C, p.C);
   }
   if (ev) LW_HIP(hipEventRecord(ev[2], st));
   for (int n = 3; n <= L; ++n) {
     DevLevel& d = h->lv[n];
-    dim3 grid(d.nb, blocks_for(d.nk, 256));
+    dim3 grid(d.nb * blocks_for(d.nk, 256));
     hipLaunchKernelGGL(k_translate, grid, dim3(256), 0, st, d.nk, d.far_ptr, d.far_box, d.far_tr, d.T, d.pes,
                        d.C, d.B);
   }
@@ -492,8 +589,12 @@ int run_matvec(Handle* h, const double2* x, double2* y, double2* y_near, hipEven
     DevLevel& c = h->lv[n];
     DevLevel& p = h->lv[n - 1];
     const double2* src = (n == 4) ? p.B : p.A;
-    hipLaunchKernelGGL(k_interp_down, dim3(c.nb), dim3(256), 2 * p.nk * sizeof(double2), st, p.nk, c.nk,
-                       c.parent, p.centre, c.centre, p.kap, p.wt_ptr, p.wt_row, p.wt_w, src, c.B, c.A, k0);
+    if (2 * p.nk * (int)sizeof(double2) <= kDownLdsBytes)
+      hipLaunchKernelGGL(k_interp_down_lds, dim3(c.nb), dim3(512), 2 * p.nk * sizeof(double2), st, p.nk, c.nk,
+                         c.parent, c.oct, p.ph, p.wt_ptr, p.wt, src, c.B, c.A);
+    else
+      hipLaunchKernelGGL(k_interp_down, dim3(c.nb * blocks_for(c.nk, 256)), dim3(256), 0, st, p.nk, c.nk,
+                         c.parent, c.oct, p.ph, p.wt_ptr, p.wt, src, c.B, c.A);
   }
   if (ev) LW_HIP(hipEventRecord(ev[4], st));
   const double w = h->plan.phys.w;
@@ -501,10 +602,10 @@ int run_matvec(Handle* h, const double2* x, double2* y, double2* y_near, hipEven
   const int rows_per_block = 4;
   dim3 grid(blocks_for(h->ne, rows_per_block)), block(kWave * rows_per_block);
   if (h->fp64_near)
-    hipLaunchKernelGGL(k_receive_near<double2>, grid, block, 0, st, h->ne, 2 * lf.nk, h->d_basis_slot,
+    hipLaunchKernelGGL(k_receive_near<double2>, grid, block, 0, st, h->ne, 2 * lf.nk, h->d_slot_basis,
                        h->d_basis_box, h->d_rad, lf.A, h->d_ia, h->d_ja, (const double2*)h->d_z, x, cc, y, y_near);
   else
-    hipLaunchKernelGGL(k_receive_near<float2>, grid, block, 0, st, h->ne, 2 * lf.nk, h->d_basis_slot,
+    hipLaunchKernelGGL(k_receive_near<float2>, grid, block, 0, st, h->ne, 2 * lf.nk, h->d_slot_basis,
                        h->d_basis_box, h->d_rad, lf.A, h->d_ia, h->d_ja, (const float2*)h->d_z, x, cc, y, y_near);
   if (ev) LW_HIP(hipEventRecord(ev[5], st));
   LW_HIP(hipGetLastError());
@@ -604,6 +705,24 @@ static int init_device_state(Handle* h, const Geometry& g) {
       TRY(upload(h, &d.wt_ptr, lp.wt_ptr));
       TRY(upload(h, &d.wt_row, lp.wt_row));
       TRY(upload(h, &d.wt_w, lp.wt_w));
+      {
+        std::vector<WtEntry> wt(lp.wt_w.size());
+        for (size_t q = 0; q < wt.size(); ++q) wt[q] = WtEntry{lp.wt_w[q], lp.wt_row[q], 0};
+        TRY(upload(h, &d.wt, wt));
+      }
+      TRY(dmalloc(h, &d.ph, 8 * (size_t)lp.nk));
+      hipLaunchKernelGGL(k_phase_table64, dim3(blocks_for(lp.nk, 256)), dim3(256), 0, h->stream, lp.nk, d.kap,
+                         lp.lado / 4.0, 2.0 * kPi / P.phys.lambda, d.ph);
+    }
+    if (n > 3) {
+      const LevelPlan& pp = P.lv[n - 1];
+      std::vector<int> oc(lp.nb);
+      for (int c = 0; c < lp.nb; ++c) {
+        const double* cc = &lp.centre[3 * c];
+        const double* pc = &pp.centre[3 * lp.parent[c]];
+        oc[c] = ((cc[0] > pc[0]) ? 4 : 0) | ((cc[1] > pc[1]) ? 2 : 0) | ((cc[2] > pc[2]) ? 1 : 0);
+      }
+      TRY(upload(h, &d.oct, oc));
     }
     double* d_tu = nullptr;
     double* d_th = nullptr;
@@ -649,8 +768,6 @@ static int init_device_state(Handle* h, const Geometry& g) {
     set_error("lw_mlfma_init: sample count exceeds LDS capacity of the pass kernels");
     return LW_ERR_ARG;
   }
-  LW_HIP(hipFuncSetAttribute((const void*)k_interp_down, hipFuncAttributeMaxDynamicSharedMemorySize,
-                             2 * maxk * (int)sizeof(double2)));
   LW_HIP(hipFuncSetAttribute((const void*)k_interp_up, hipFuncAttributeMaxDynamicSharedMemorySize,
                              2 * maxk * (int)sizeof(double2)));
 #undef TRY
@@ -1034,6 +1151,8 @@ int lw_cgs(lw_handle* hh, const double* b, const double* guess, double errmax, i
       conv = true;
       break;
     }
+    if (lw::progress_every() > 0 && it % lw::progress_every() == 0)
+      std::fprintf(stderr, "[lw_cgs] iteration %d: relres %.3e\n", it, rr);
     // rho2 = rho1; rho1 = next (modIterativo.f90:486)
     LW_HIP(hipMemcpyAsync(s + 1, s + 0, sizeof(double2), hipMemcpyDeviceToDevice, st));
     LW_HIP(hipMemcpyAsync(s + 0, s + 5, sizeof(double2), hipMemcpyDeviceToDevice, st));
@@ -1221,6 +1340,26 @@ int lw_matvec_bytes(lw_handle* hh, double* total, double* aggregation, double* r
   // receive + near: patterns, leaf campoA, near CSR, x gathered once, y written
   if (receive_near)
     *receive_near = pattern + (double)P.lv[P.L].nk * 2 * P.lv[P.L].nb * c16 + near + 2.0 * P.ne * c16;
+  return LW_OK;
+}
+
+int lw_matvec_stage_bytes(lw_handle* hh, double* sb) {
+  if (lw::check_handle(hh) || !sb) return LW_ERR_ARG;
+  Handle* h = &hh->h;
+  const lw::Plan& P = h->plan;
+  const int L = P.L;
+  auto field = [&](int n) { return (double)P.lv[n].nk * 2 * P.lv[n].nb * 16.0; };
+  const double pattern = (double)P.lv[L].nk * 2 * P.ne * 16.0;
+  const double zb = h->fp64_near ? 16.0 : 8.0;
+  sb[0] = pattern + 16.0 * P.ne + field(L) + 8.0 * P.ne;
+  sb[1] = 0.0;
+  for (int n = L; n >= 4; --n) sb[1] += field(n) + field(n - 1) + 12.0 * 9 * P.lv[n - 1].nk + 16.0 * 8 * P.lv[n - 1].nk;
+  sb[2] = 0.0;
+  for (int n = 3; n <= L; ++n)
+    sb[2] += 2 * field(n) + 16.0 * P.lv[n].nk * P.lv[n].nT + 8.0 * (double)P.lv[n].far_box.size() + 16.0 * P.lv[n].nk;
+  sb[3] = 0.0;
+  for (int n = 4; n <= L; ++n) sb[3] += field(n - 1) + 2 * field(n) + 12.0 * 9 * P.lv[n - 1].nk;
+  sb[4] = pattern + field(L) + (zb + 4.0) * (double)P.nnz + 4.0 * (P.ne + 1) + 2.0 * 16.0 * P.ne + 8.0 * P.ne;
   return LW_OK;
 }
 

@@ -12,6 +12,7 @@
 namespace lw {
 
 void set_error(const std::string& m);
+int progress_every();
 
 #define LW_HIP(call)                                                                 \
   do {                                                                               \
@@ -21,6 +22,14 @@ void set_error(const std::string& m);
       return _e == hipErrorOutOfMemory ? LW_ERR_NOMEM : LW_ERR_HIP;                  \
     }                                                                                \
   } while (0)
+
+// One entry of the transposed interpolation stencil (Anterpolar in gather
+// form): a weight and the coarse sample it reads, loaded as one 16-byte word.
+struct alignas(16) WtEntry {
+  double w;
+  int r;
+  int pad;
+};
 
 // Device arrays of one level (levels >= 3).
 struct DevLevel {
@@ -43,6 +52,11 @@ struct DevLevel {
   int* wt_ptr = nullptr;      // (nk_fine + 1)
   int* wt_row = nullptr;      // coarse sample of each transposed entry
   double* wt_w = nullptr;     // weight of each transposed entry
+  WtEntry* wt = nullptr;      // (weight, coarse row) pairs of the transpose
+  // exp(+j k khat_i . r_o) for the 8 child offsets r_o = (+-q,+-q,+-q),
+  // q = lado/4, octant bits (x,y,z) = (4,2,1), set = positive side (n < L)
+  double2* ph = nullptr;      // (8, nk)
+  int* oct = nullptr;         // octant of each box within its parent (n > 3)
   long long npairs = 0;
 };
 

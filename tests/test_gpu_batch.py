@@ -40,6 +40,7 @@ def test_batch_matvec_fp32_far_field(R):
     Y64 = np.stack([op.matvec(X[:, r]) for r in range(R)], axis=1)
     # FP32 far field: relative error of the product at the FP32 level
     errs = [rel(Y[:, r], Y64[:, r]) for r in range(R)]
+    print("FP32-far-field matvec relative error: max", max(errs), "median", np.median(errs))
     assert max(errs) < 2e-5
     # the near part is FP64: with x supported on one leaf box's neighbourhood the
     # difference is the far field's only
@@ -78,24 +79,54 @@ def test_batch_plane_wave_and_rcs_match_single():
         assert abs(s[r] - s1[0]) <= 1e-10 * s1[0]
 
 
+_sphere = {}
+
+
+def sphere_batch():
+    if not _sphere:
+        mesh = lw.Mesh.icosphere(3, 0.5)
+        op = lw.MLFMA(mesh, lam=1.0, ladomax=0.2)
+        assert op.ilu0() == 0
+        _sphere["v"] = (mesh, op, lw.Batch(op, 16))
+    return _sphere["v"]
+
+
+def sweep(bt, R=16):
+    ph = np.linspace(0, np.pi, R, endpoint=False)
+    dirs = np.stack([-np.cos(ph), -np.sin(ph), np.zeros(R)], axis=1)
+    pols = np.tile(np.array([0, 0, 1.0], np.complex128), (R, 1))
+    return ph, dirs, pols, bt.plane_wave(dirs, pols)
+
+
 @pytest.mark.parametrize("use_prec", [False, True])
-def test_batch_cgs_solves_every_column(use_prec):
-    mesh, op, bt = setup(16)
-    ph = np.linspace(0, np.pi, 16, endpoint=False)
-    dirs = np.stack([-np.cos(ph), -np.sin(ph), np.zeros(16)], axis=1)
-    pols = np.tile(np.array([0, 0, 1.0], np.complex128), (16, 1))
-    B = bt.plane_wave(dirs, pols)
-    X, it, rr, conv = bt.cgs(B, errmax=1e-4, use_prec=use_prec, maxit=3000)
+def test_batch_cgs_first_iterations_match_single(use_prec):
+    # FP32 far field: after a few iterations each column stays within the
+    # operator's FP32 error of the FP64 single-RHS iterate
+    mesh, op, bt = sphere_batch()
+    ph, dirs, pols, B = sweep(bt)
+    X, it, rr, conv = bt.cgs(B, errmax=1e-12, use_prec=use_prec, maxit=3)
+    assert (it == 3).all()
+    for r in (0, 5, 15):
+        x64, it64, rr64, c64 = op.cgs(B[:, r], errmax=1e-12, use_prec=use_prec, maxit=3)
+        assert rel(X[:, r], x64) < 1e-3, r
+
+
+def test_batch_cgs_solves_every_column():
+    mesh, op, bt = sphere_batch()
+    ph, dirs, pols, B = sweep(bt)
+    X, it, rr, conv = bt.cgs(B, errmax=1e-4, use_prec=False, maxit=3000)
+    print("iterations", it.tolist(), "relres", rr.max())
     assert conv.all() and (rr < 1e-4).all()
-    # true residual with the FP64 operator stays at the tolerance
+    # the true residual with the FP64 operator stays at the tolerance
     for r in (0, 8, 15):
-        res = rel(op.matvec(X[:, r]), B[:, r])
-        assert res < 3e-4
-    # against independent FP64 single-RHS solves: same solution to the
-    # tolerance, RCS within 0.1 dB
+        assert rel(op.matvec(X[:, r]), B[:, r]) < 3e-4
+    # against independent FP64 single-RHS solves: the FP32 far field perturbs
+    # CGS's recurrences (measured up to +35% iterations on this sphere), the
+    # converged currents give the RCS to 0.1 dB
     for r in (0, 11):
-        x64, it64, rr64, c64 = op.cgs(B[:, r], errmax=1e-4, use_prec=use_prec, maxit=3000)
+        x64, it64, rr64, c64 = op.cgs(B[:, r], errmax=1e-4, use_prec=False, maxit=3000)
         assert c64
+        assert int(it[r]) <= 1.6 * it64
         s32 = op.rcs(X[:, r], pols[r], np.pi / 2, ph[r])[0]
         s64 = op.rcs(x64, pols[r], np.pi / 2, ph[r])[0]
         assert abs(10 * np.log10(s32 / s64)) < 0.1
