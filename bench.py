@@ -55,6 +55,8 @@ def main():
     ap.add_argument("--batch", type=int, default=64,
                     help="right-hand sides per block of the multi-RHS FP32-far-field form (0: skip)")
     ap.add_argument("--cpu-iters", type=int, default=3)
+    ap.add_argument("--matvec-only", action="store_true",
+                    help="time the single and batched matvecs only (for kernel profiles), no solves")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -99,6 +101,16 @@ def main():
     bytes_ = op.matvec_bytes()
     lus_ms = op.bench_lusol(reps=3) if have_ilu else 0.0
     log(f"matvec {mv_ms:.3f} ms (stages {np.round(stages, 3).tolist()}), lusol {lus_ms:.3f} ms")
+
+    if args.matvec_only:
+        if args.batch > 0:
+            for fp64 in (False, True):
+                bt = lw.Batch(op, args.batch, far_fp64=fp64)
+                bmv, _ = bt.bench(reps=5)
+                log(f"batched matvec (R={args.batch}, far field {'FP64' if fp64 else 'FP32'}): {bmv:.3f} ms = "
+                    f"{bmv / args.batch * 1e3:.1f} us per RHS")
+                bt.close()
+        return
 
     bs = [rhs(i) for i in range(args.warmup + args.steps)]
     for i in range(args.warmup):

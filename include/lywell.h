@@ -164,7 +164,9 @@ int lw_matvec_stage_bytes(lw_handle *h, double *stage_bytes);
 /* ---- multi-RHS form ------------------------------------------------------
  * nrhs right-hand sides of one mesh through one operator pass.  The far field
  * (aggregation, interpolation, translation, disaggregation, receive) runs in
- * FP32; the near field, the ILU0 solves and the CGS scalars stay in FP64.
+ * FP32 (complex(4) campo arrays, FP32 accumulation) or, with
+ * LW_BATCH_FAR_FP64, in FP64; the near field, the ILU0 solves and the CGS
+ * vectors and scalars are FP64 in both.
  * Blocks are complex(8) arrays of shape (nrhs, ne) in Fortran order
  * (x(r,i) = right-hand side r, unknown i: nrhs values contiguous per unknown).
  * A batch shares the handle's stream and operators; one batch per handle at a
@@ -172,8 +174,11 @@ int lw_matvec_stage_bytes(lw_handle *h, double *stage_bytes);
  * call (modIterativo.f90:331-493) from a loop over incidence angles. */
 typedef struct lw_batch lw_batch;
 
-/* nrhs in {8, 16, 32, 64}. */
-int lw_batch_create(lw_handle *h, int nrhs, lw_batch **out);
+/* Flags for lw_batch_create. */
+#define LW_BATCH_FAR_FP64 1 /* far field in complex(8) (default: complex(4)) */
+
+/* nrhs in {64, 128}. */
+int lw_batch_create(lw_handle *h, int nrhs, int flags, lw_batch **out);
 int lw_batch_destroy(lw_batch *b);
 /* The block width, or LW_ERR_ARG. */
 int lw_batch_nrhs(lw_batch *b);
@@ -192,6 +197,46 @@ int lw_batch_rcs(lw_batch *b, const double *cur, const double *pols, const doubl
                  const double *phi, double *sigma);
 /* Mean batched matvec time and one batched lusol0 time in ms (HIP events). */
 int lw_bench_batch(lw_batch *b, int reps, double *matvec_ms, double *lusol_ms);
+
+/* ---- learned initial guess for monostatic sweeps ---------------------------
+ * No reference counterpart except the warm start it generalises: the
+ * reference starts each angle of a sweep from the previous angle's current
+ * times exp(-j k (d - d_prev) . e_centro) (main.f90:1017-1022).  Here a
+ * pointwise MLP (in -> hidden x depth (tanh) -> out; FP32 master weights,
+ * bfloat16 GEMM operands, FP32 accumulation on MFMA) maps 16 features per
+ * unknown to 4 outputs o, and the guess for a new angle is
+ *   (1 + o0 + j o1) xt1 + (o2 + j o3) (xt1 - xt2),
+ * xt1 and xt2 being the two nearest solved currents with that phase
+ * correction (o = 0 is the reference's warm start).  Parameters are laid out
+ * layer by layer as W (out x in, row-major) then b. */
+typedef struct lw_mlp lw_mlp;
+
+int lw_mlp_create(int in_dim, int hidden, int depth, int out_dim, unsigned long long seed, lw_mlp **out);
+int lw_mlp_destroy(lw_mlp *m);
+long long lw_mlp_nparams(lw_mlp *m);
+int lw_mlp_get_params(lw_mlp *m, float *p);
+int lw_mlp_set_params(lw_mlp *m, const float *p);
+/* y(rows, out) = MLP(x(rows, in)), host row-major buffers. */
+int lw_mlp_forward(lw_mlp *m, const float *x, int rows, float *y);
+/* Gradients of sum(dy .* MLP(x)) with respect to every parameter (grad,
+ * nparams) and to x (dx, rows x in); either may be NULL. */
+int lw_mlp_backward(lw_mlp *m, const float *x, int rows, const float *dy, float *grad, float *dx);
+/* The guess for ncol new angles on the handle's mesh: dir(3,ncol) unit
+ * propagation directions, pol(3,ncol) real polarisations, dir1/dir2 the
+ * directions of the solved neighbours and x1/x2 their currents (complex(8),
+ * (ncol, num_e) in Fortran order as in lw_batch_*).  guess: same layout. */
+int lw_guess_predict(lw_mlp *m, lw_handle *h, int ncol, const double *dir, const double *pol,
+                     const double *dir1, const double *x1, const double *dir2, const double *x2,
+                     double *guess);
+/* Appends training rows (one per unknown and column) with the converged
+ * currents xtrue as targets; lw_guess_train then runs Adam steps on random
+ * minibatches of batch_rows rows (loss per step into loss[steps] if not
+ * NULL: mean relative squared error of the guess). */
+int lw_guess_add_samples(lw_mlp *m, lw_handle *h, int ncol, const double *dir, const double *pol,
+                         const double *dir1, const double *x1, const double *dir2, const double *x2,
+                         const double *xtrue);
+long long lw_guess_nsamples(lw_mlp *m);
+int lw_guess_train(lw_mlp *m, int steps, int batch_rows, float lr, unsigned long long seed, float *loss);
 
 #ifdef __cplusplus
 }

@@ -169,6 +169,7 @@ class MLFMA:
         _check(rc)
         self.h = h
         self.ne = mesh.ne
+        self.lam = float(lam)
         self.fp64_near = fp64_near
 
     def close(self):
@@ -298,13 +299,15 @@ class MLFMA:
 
 
 class Batch:
-    """R right-hand sides through one operator pass: FP32 far field, FP64 near
-    field, ILU0 solves and CGS (lw_batch_*)."""
+    """R right-hand sides through one operator pass (lw_batch_*): far field in
+    FP32 (default) or FP64 (far_fp64=True); near field, ILU0 solves and CGS in
+    FP64."""
 
-    def __init__(self, op, nrhs):
+    def __init__(self, op, nrhs, far_fp64=False):
         L = lib()
         b = _C.c_void_p()
-        _check(L.lw_batch_create(op.h, int(nrhs), _C.byref(b)))
+        _check(L.lw_batch_create(op.h, int(nrhs), 1 if far_fp64 else 0, _C.byref(b)))
+        self.far_fp64 = bool(far_fp64)
         self.b = b
         self.op = op
         self.R = int(nrhs)
@@ -368,7 +371,112 @@ class Batch:
         _check(lib().lw_batch_rcs(self.b, _P(I), _P(p), _P(c), _P(th), _P(ph), _P(s)))
         return s
 
-    def bench(self, reps=5):
+    def bench(self, reps=5, lusol=False):
+        """Mean batched matvec time in ms (and one batched lusol0 time when
+        lusol is set and the operator has an ILU0 factor)."""
         mv, lu = _C.c_double(), _C.c_double()
-        _check(lib().lw_bench_batch(self.b, int(reps), _C.byref(mv), _C.byref(lu)))
-        return mv.value, lu.value
+        _check(lib().lw_bench_batch(self.b, int(reps), _C.byref(mv), _C.byref(lu) if lusol else None))
+        return mv.value, (lu.value if lusol else None)
+
+
+class Mlp:
+    """Pointwise MLP of the learned initial guess (lw_mlp_*): in -> hidden x
+    depth (tanh) -> out, FP32 master weights, bfloat16 GEMM operands with FP32
+    accumulation on the GPU."""
+
+    FEATURES = 16
+    OUTPUTS = 4
+
+    def __init__(self, in_dim=16, hidden=256, depth=3, out_dim=4, seed=0):
+        L = lib()
+        L.lw_mlp_nparams.restype = _C.c_longlong
+        L.lw_guess_nsamples.restype = _C.c_longlong
+        m = _C.c_void_p()
+        _check(L.lw_mlp_create(int(in_dim), int(hidden), int(depth), int(out_dim), _C.c_ulonglong(seed),
+                               _C.byref(m)))
+        self.m = m
+        self.in_dim, self.hidden, self.depth, self.out_dim = in_dim, hidden, depth, out_dim
+        self.nparams = int(L.lw_mlp_nparams(m))
+
+    def close(self):
+        if self.m:
+            lib().lw_mlp_destroy(self.m)
+            self.m = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def shapes(self):
+        """(W shape, b shape) per layer, in the flat parameter order."""
+        dims = [self.in_dim] + [self.hidden] * self.depth + [self.out_dim]
+        return [((dims[l + 1], dims[l]), (dims[l + 1],)) for l in range(len(dims) - 1)]
+
+    def get_params(self):
+        p = _np.zeros(self.nparams, _np.float32)
+        _check(lib().lw_mlp_get_params(self.m, _P(p)))
+        return p
+
+    def set_params(self, p):
+        p = _np.ascontiguousarray(p, _np.float32)
+        if p.size != self.nparams:
+            raise ValueError("parameter vector of the wrong size")
+        _check(lib().lw_mlp_set_params(self.m, _P(p)))
+
+    def forward(self, x):
+        x = _np.ascontiguousarray(x, _np.float32)
+        if x.ndim != 2 or x.shape[1] != self.in_dim:
+            raise ValueError(f"(rows, {self.in_dim}) input expected")
+        y = _np.zeros((x.shape[0], self.out_dim), _np.float32)
+        _check(lib().lw_mlp_forward(self.m, _P(x), int(x.shape[0]), _P(y)))
+        return y
+
+    def backward(self, x, dy):
+        x = _np.ascontiguousarray(x, _np.float32)
+        dy = _np.ascontiguousarray(dy, _np.float32)
+        g = _np.zeros(self.nparams, _np.float32)
+        dx = _np.zeros_like(x)
+        _check(lib().lw_mlp_backward(self.m, _P(x), int(x.shape[0]), _P(dy), _P(g), _P(dx)))
+        return g, dx
+
+    # -- learned guess for sweeps
+    @staticmethod
+    def _cols(a, ncol, width):
+        return _np.ascontiguousarray(_np.broadcast_to(_np.asarray(a, _np.float64), (ncol, width)))
+
+    def _guess_args(self, op, dirs, pols, dir1, x1, dir2, x2):
+        x1 = _np.ascontiguousarray(x1, _np.complex128)
+        if x1.ndim == 1:
+            x1 = x1[:, None]
+        ncol = x1.shape[1]
+        x2 = _np.ascontiguousarray(_np.asarray(x2, _np.complex128).reshape(op.ne, ncol))
+        d = self._cols(dirs, ncol, 3)
+        p = self._cols(_np.real(pols), ncol, 3)
+        d1 = self._cols(dir1, ncol, 3)
+        d2 = self._cols(dir2, ncol, 3)
+        if x1.shape[0] != op.ne:
+            raise ValueError("currents must be (num_e, ncol)")
+        return ncol, d, p, d1, x1, d2, x2
+
+    def predict_guess(self, op, dirs, pols, dir1, x1, dir2, x2):
+        ncol, d, p, d1, x1, d2, x2 = self._guess_args(op, dirs, pols, dir1, x1, dir2, x2)
+        g = _np.zeros((op.ne, ncol), _np.complex128)
+        _check(lib().lw_guess_predict(self.m, op.h, ncol, _P(d), _P(p), _P(d1), _P(x1), _P(d2), _P(x2), _P(g)))
+        return g
+
+    def add_samples(self, op, dirs, pols, dir1, x1, dir2, x2, xtrue):
+        ncol, d, p, d1, x1, d2, x2 = self._guess_args(op, dirs, pols, dir1, x1, dir2, x2)
+        xt = _np.ascontiguousarray(_np.asarray(xtrue, _np.complex128).reshape(op.ne, ncol))
+        _check(lib().lw_guess_add_samples(self.m, op.h, ncol, _P(d), _P(p), _P(d1), _P(x1), _P(d2), _P(x2),
+                                          _P(xt)))
+
+    def nsamples(self):
+        return int(lib().lw_guess_nsamples(self.m))
+
+    def train(self, steps, batch_rows=16384, lr=1e-3, seed=1):
+        loss = _np.zeros(int(steps), _np.float32)
+        _check(lib().lw_guess_train(self.m, int(steps), int(batch_rows), _C.c_float(lr), _C.c_ulonglong(seed),
+                                    _P(loss)))
+        return loss

@@ -47,6 +47,7 @@ def build_hip():
     units = [
         ("lw_device.hip", []),
         ("lw_batch.hip", []),
+        ("lw_mlp.hip", []),
         # bit-exact Legendre sums: no contraction into FMAs in this unit
         ("lw_setup_kernels.hip", ["-ffp-contract=off"]),
         ("lw_plan.cpp", ["-x", "hip"]),

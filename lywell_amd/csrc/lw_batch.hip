@@ -11,7 +11,7 @@
 //   campo arrays       F[((k*nb + b)*2 + p)*R + r] (complex(4), sample-major so
 //                      one sample's slice of a level is contiguous for the
 //                      translation kernel)
-//   patterns           rad32[(s*2 + p)*nk + k]     (complex(4), leaf slot order)
+//   patterns           rad[(s*2 + p)*nk + k]       (leaf slot order)
 // The per-RHS CGS follows CGSold (modIterativo.f90:331-493) column by column:
 // a column that meets its tolerance (or breaks down) is frozen, so every
 // column's iterate is the one a single-RHS run would produce with this
@@ -32,13 +32,6 @@ namespace {
 constexpr double kMu = Physics::mu, kEps = Physics::epsi, kPi = Physics::pi;
 constexpr int kWave = 64;
 
-__device__ __forceinline__ float2 fmul(float2 a, float2 b) {
-  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
-}
-__device__ __forceinline__ float2 fmulc(float2 a, float2 b) {  // conj(a) * b
-  return make_float2(a.x * b.x + a.y * b.y, a.x * b.y - a.y * b.x);
-}
-__device__ __forceinline__ float2 fadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ double2 dmul(double2 a, double2 b) {
   return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
@@ -46,6 +39,9 @@ __device__ __forceinline__ double2 dadd(double2 a, double2 b) { return make_doub
 __device__ __forceinline__ double2 dsub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
 __device__ __forceinline__ double2 to_d2(float2 v) { return make_double2((double)v.x, (double)v.y); }
 __device__ __forceinline__ double2 to_d2(double2 v) { return v; }
+typedef float f32x4v __attribute__((ext_vector_type(4)));
+typedef double f64x4v __attribute__((ext_vector_type(4)));
+
 __device__ __forceinline__ double2 ddiv(double2 a, double2 b) {
   if (fabs(b.x) >= fabs(b.y)) {
     const double r = b.y / b.x, d = b.x + b.y * r;
@@ -55,174 +51,470 @@ __device__ __forceinline__ double2 ddiv(double2 a, double2 b) {
   return make_double2((a.x * r + a.y) / d, (a.y * r - a.x) / d);
 }
 
-__global__ void k_to_f32(long long n, const double2* __restrict__ in, float2* __restrict__ out) {
-  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) out[i] = make_float2((float)in[i].x, (float)in[i].y);
-}
-__global__ void k_to_f32r(long long n, const double* __restrict__ in, float* __restrict__ out) {
-  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) out[i] = (float)in[i];
+// ---------------------------------------------------------------- far field
+//
+// Every far-field kernel is a template on the scalar S of the campo arrays,
+// patterns and tables: float (complex(4) far field, FP32 accumulation) or
+// double (the reference's complex(8)).  Vectors, near field and CGS are FP64
+// in both.
+
+template <typename S> struct Cx;
+template <> struct Cx<float> {
+  typedef float2 T;
+  typedef f32x4v Acc;
+};
+template <> struct Cx<double> {
+  typedef double2 T;
+  typedef f64x4v Acc;
+};
+template <typename S>
+__device__ __forceinline__ typename Cx<S>::T mk(S x, S y) {
+  typename Cx<S>::T v;
+  v.x = x;
+  v.y = y;
+  return v;
 }
 
-// ---------------------------------------------------------------- far field (FP32)
+// 16x16x4 MFMA: lane l supplies A[l%16][l/16] and B[l/16][l%16]; result
+// register v of lane l holds D[row][l%16] with row = 4*(l/16) + v for the f32
+// form and row = l/16 + 4*v for the f64 form (both measured on gfx950).
+__device__ __forceinline__ f32x4v mma(float a, float b, f32x4v c) {
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+__device__ __forceinline__ f64x4v mma(double a, double b, f64x4v c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+template <typename S>
+__device__ __forceinline__ int mma_row(int lk, int v) {
+  return sizeof(S) == sizeof(float) ? 4 * lk + v : lk + 4 * v;
+}
 
-// Leaf aggregation: C_L[k][b][p][r] = sum_s rad(s; p,k) x(basis(s), r)
-__global__ void __launch_bounds__(256) k_b_aggregate(int R, int nk, int nb, const int* __restrict__ box_slot,
+template <typename S>
+__global__ void k_to_prec(long long n, const double2* __restrict__ in, typename Cx<S>::T* __restrict__ out) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = mk<S>((S)in[i].x, (S)in[i].y);
+}
+
+// Leaf aggregation on MFMA: per leaf box the complex GEMM
+//   C_L[m][b][r] = sum_s rad(s; m) x(basis(s), r),  m = p*nk + k,
+// over the box's nf bases.  Workgroup (box, group of 4 item tiles); a wave
+// owns 16 items and all R columns.  The box's x rows go through LDS in
+// chunks of KC bases (row stride R + 8 against bank conflicts).
+template <typename S, int R, int KC>
+__global__ void __launch_bounds__(256) k_b_aggregate(int nk, int nb, const int* __restrict__ box_slot,
                                                      const int* __restrict__ slot_basis,
-                                                     const float2* __restrict__ rad, const double2* __restrict__ X,
-                                                     float2* __restrict__ C) {
-  extern __shared__ float2 xs[];  // (n_f, R)
+                                                     const typename Cx<S>::T* __restrict__ rad,
+                                                     const double2* __restrict__ X, typename Cx<S>::T* __restrict__ C) {
+  typedef typename Cx<S>::T T2;
+  typedef typename Cx<S>::Acc Acc;
+  constexpr int RS = R + 8, NT = R / 16;
+  __shared__ T2 xs[KC * RS];
   const int b = blockIdx.x;
   const int s0 = box_slot[b], nf = box_slot[b + 1] - s0;
-  for (int e = threadIdx.x; e < nf * R; e += blockDim.x) {
-    const double2 v = X[(size_t)slot_basis[s0 + e / R] * R + (e % R)];
-    xs[e] = make_float2((float)v.x, (float)v.y);
-  }
-  __syncthreads();
-  const int r = threadIdx.x % R, per = blockDim.x / R;
   const int nk2 = 2 * nk;
-  for (int item = blockIdx.y * per + threadIdx.x / R; item < nk2; item += gridDim.y * per) {
-    float2 acc = make_float2(0.f, 0.f);
-    for (int s = 0; s < nf; ++s) acc = fadd(acc, fmul(rad[(size_t)(s0 + s) * nk2 + item], xs[s * R + r]));
-    const int p = item / nk, k = item % nk;
-    C[(((size_t)k * nb + b) * 2 + p) * R + r] = acc;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave), l = threadIdx.x % kWave;
+  const int li = l % 16, lk = l / 16;
+  const int tile = blockIdx.y * 4 + wave;
+  const int m = tile * 16 + li;
+  Acc cr[NT], ci[NT];
+#pragma unroll
+  for (int jt = 0; jt < NT; ++jt) {
+    cr[jt] = Acc{0, 0, 0, 0};
+    ci[jt] = Acc{0, 0, 0, 0};
+  }
+  for (int c0 = 0; c0 < nf; c0 += KC) {
+    const int nc = min(KC, nf - c0), nc4 = (nc + 3) & ~3;
+    __syncthreads();
+    for (int e = threadIdx.x; e < nc4 * R; e += blockDim.x) {
+      const int sl = e / R, r = e % R;
+      T2 v = mk<S>(0, 0);
+      if (sl < nc) {
+        const double2 d = X[(size_t)slot_basis[s0 + c0 + sl] * R + r];
+        v = mk<S>((S)d.x, (S)d.y);
+      }
+      xs[sl * RS + r] = v;
+    }
+    __syncthreads();
+    for (int ks = 0; ks < nc4; ks += 4) {
+      const int sl = ks + lk;
+      const T2 a = (sl < nc && m < nk2) ? rad[(size_t)(s0 + c0 + sl) * nk2 + m] : mk<S>(0, 0);
+      const T2* xr = xs + sl * RS + li;
+#pragma unroll
+      for (int jt = 0; jt < NT; ++jt) {
+        const T2 x = xr[jt * 16];
+        cr[jt] = mma(a.x, x.x, cr[jt]);
+        cr[jt] = mma(-a.y, x.y, cr[jt]);
+        ci[jt] = mma(a.x, x.y, ci[jt]);
+        ci[jt] = mma(a.y, x.x, ci[jt]);
+      }
+    }
+  }
+#pragma unroll
+  for (int v = 0; v < 4; ++v) {
+    const int mo = tile * 16 + mma_row<S>(lk, v);
+    if (mo < nk2) {
+      const int p = mo / nk, k = mo % nk;
+      T2* o = C + (((size_t)k * nb + b) * 2 + p) * R + li;
+#pragma unroll
+      for (int jt = 0; jt < NT; ++jt) o[jt * 16] = mk<S>(cr[jt][v], ci[jt][v]);
+    }
   }
 }
 
-// Upward pass: C_{n-1}[i][P] = sum_children ph(oct_c, i) sum_9 w_ij C_n[idx_j][c]
-__global__ void __launch_bounds__(256) k_b_interp_up(int R, int nkc, int nbc, int nkf, int nbf,
-                                                     const int* __restrict__ child_ptr, const int* __restrict__ oct,
-                                                     const float* __restrict__ wint, const int* __restrict__ iint,
-                                                     const float2* __restrict__ ph, const float2* __restrict__ Cf,
-                                                     float2* __restrict__ Cc) {
-  const int P = blockIdx.x;
-  const long long t = (long long)blockIdx.y * blockDim.x + threadIdx.x;
-  if (t >= (long long)nkc * 2 * R) return;
-  const int r = (int)(t % R), p = (int)((t / R) % 2), i = (int)(t / (2 * R));
-  float w[9];
-  int id[9];
-#pragma unroll
-  for (int j = 0; j < 9; ++j) {
-    w[j] = wint[9 * (size_t)i + j];
-    id[j] = iint[9 * (size_t)i + j];
-  }
-  float2 acc = make_float2(0.f, 0.f);
+// Upward pass with the factored stencil (W = Wphi (x) Wtheta), one wave per
+// (parent P, coarse phi column ip, 64-column chunk):
+//   C_{n-1}[(ip, it)][P] = sum_c ph(oct_c, (ip, it))
+//                          sum_a wphi[ip][a] sum_b wth[it][b] C_n[(fa, tb)][c]
+// The wave walks the coarse theta rows in order; the 3x3 window of fine
+// samples slides with the theta stencil, so most steps load 3 new values
+// (one per fine phi column) instead of 9.  Per-lane accumulators over the
+// coarse theta rows live in LDS.
+template <typename S>
+__global__ void __launch_bounds__(256) k_b_interp_up(int ncol, int nthc, int nkc, int nbc, int nthf, int nbf,
+                                                     int nphc, const int* __restrict__ child_ptr,
+                                                     const int* __restrict__ oct, const int* __restrict__ sth_idx,
+                                                     const S* __restrict__ sth_w, const int* __restrict__ sph_idx,
+                                                     const S* __restrict__ sph_w,
+                                                     const typename Cx<S>::T* __restrict__ ph,
+                                                     const typename Cx<S>::T* __restrict__ Cf,
+                                                     typename Cx<S>::T* __restrict__ Cc) {
+  typedef typename Cx<S>::T T2;
+  extern __shared__ unsigned char lds_raw[];
+  T2* accs = reinterpret_cast<T2*>(lds_raw);  // (waves, nthc, 64)
+  const int wpg = blockDim.x / kWave;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave), lane = threadIdx.x % kWave;
+  const int nchunk = ncol / kWave;
+  const int t = xcd_block(blockIdx.x, gridDim.x) * wpg + wave;
+  const int P = t / (nphc * nchunk);
+  if (P >= nbc) return;
+  const int ip = (t / nchunk) % nphc, col = (t % nchunk) * kWave + lane;
+  T2* acc = accs + (size_t)wave * nthc * kWave + lane;
+  for (int it = 0; it < nthc; ++it) acc[it * kWave] = mk<S>(0, 0);
+  const int f0 = sph_idx[3 * ip], f1 = sph_idx[3 * ip + 1], f2 = sph_idx[3 * ip + 2];
+  const S w0 = sph_w[3 * ip], w1 = sph_w[3 * ip + 1], w2 = sph_w[3 * ip + 2];
+  const size_t ks = (size_t)nbf * ncol;  // stride of one fine sample
   for (int c = child_ptr[P]; c < child_ptr[P + 1]; ++c) {
-    float2 a = make_float2(0.f, 0.f);
+    const T2* ph_c = ph + (size_t)oct[c] * nkc + (size_t)ip * nthc;
+    const T2* s0 = Cf + (size_t)f0 * nthf * ks + (size_t)c * ncol + col;
+    const T2* s1 = Cf + (size_t)f1 * nthf * ks + (size_t)c * ncol + col;
+    const T2* s2 = Cf + (size_t)f2 * nthf * ks + (size_t)c * ncol + col;
+    // window of step it in v (fine theta rows cur[]); the values of step
+    // it+1 are fetched before step it is computed, so that their loads
+    // overlap the arithmetic
+    int cur[3];
+    T2 v[3][3];
 #pragma unroll
-    for (int j = 0; j < 9; ++j) {
-      const float2 v = Cf[(((size_t)id[j] * nbf + c) * 2 + p) * R + r];
-      a.x += w[j] * v.x;
-      a.y += w[j] * v.y;
+    for (int b = 0; b < 3; ++b) {
+      const int tb = sth_idx[b];
+      cur[b] = tb;
+      v[0][b] = s0[(size_t)tb * ks];
+      v[1][b] = s1[(size_t)tb * ks];
+      v[2][b] = s2[(size_t)tb * ks];
     }
-    acc = fadd(acc, fmul(a, ph[(size_t)oct[c] * nkc + i]));
+    for (int it = 0; it < nthc; ++it) {
+      int tn[3] = {cur[0], cur[1], cur[2]};
+      T2 nv[3][3];
+      if (it + 1 < nthc) {
+#pragma unroll
+        for (int b = 0; b < 3; ++b) {
+          const int tb = sth_idx[3 * (it + 1) + b];
+          tn[b] = tb;
+          if (tb == cur[0]) {
+            nv[0][b] = v[0][0]; nv[1][b] = v[1][0]; nv[2][b] = v[2][0];
+          } else if (tb == cur[1]) {
+            nv[0][b] = v[0][1]; nv[1][b] = v[1][1]; nv[2][b] = v[2][1];
+          } else if (tb == cur[2]) {
+            nv[0][b] = v[0][2]; nv[1][b] = v[1][2]; nv[2][b] = v[2][2];
+          } else {
+            nv[0][b] = s0[(size_t)tb * ks];
+            nv[1][b] = s1[(size_t)tb * ks];
+            nv[2][b] = s2[(size_t)tb * ks];
+          }
+        }
+      }
+      const S u0 = sth_w[3 * it], u1 = sth_w[3 * it + 1], u2 = sth_w[3 * it + 2];
+      S vx = 0, vy = 0;
+#pragma unroll
+      for (int a = 0; a < 3; ++a) {
+        const S wa = a == 0 ? w0 : (a == 1 ? w1 : w2);
+        vx += wa * (u0 * v[a][0].x + u1 * v[a][1].x + u2 * v[a][2].x);
+        vy += wa * (u0 * v[a][0].y + u1 * v[a][1].y + u2 * v[a][2].y);
+      }
+      const T2 pz = ph_c[it];
+      T2 sa = acc[it * kWave];
+      sa.x += pz.x * vx - pz.y * vy;
+      sa.y += pz.x * vy + pz.y * vx;
+      acc[it * kWave] = sa;
+      if (it + 1 < nthc) {
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+          for (int b = 0; b < 3; ++b) v[a][b] = nv[a][b];
+        cur[0] = tn[0];
+        cur[1] = tn[1];
+        cur[2] = tn[2];
+      }
+    }
   }
-  Cc[(((size_t)i * nbc + P) * 2 + p) * R + r] = acc;
+  T2* out = Cc + ((size_t)ip * nthc * nbc + P) * ncol + col;
+  for (int it = 0; it < nthc; ++it) out[(size_t)it * nbc * ncol] = acc[it * kWave];
 }
 
-// Translation: B[k][b] = pes[k] sum_far T_t[k] C[k][far]; workgroups holding
-// the same sample k run on the same XCD (workgroup id mod 8) so the k-slice
-// of campoC stays in that XCD's L2.
-__global__ void __launch_bounds__(256) k_b_translate(int R, int nk, int nb, int nbb, const int* __restrict__ far_ptr,
-                                                     const int* __restrict__ far_box, const int* __restrict__ far_tr,
-                                                     const float2* __restrict__ T, const float* __restrict__ pes,
-                                                     const float2* __restrict__ C, float2* __restrict__ B) {
-  const int w = blockIdx.x;
-  const int g = w / 8, x = w % 8;
-  const int k = (g / nbb) * 8 + x, bb = g % nbb;
+// Translation fused with the downward pass, one wave per (sample k, parent
+// box P, 64-column chunk) at level n:
+//   A_n[k][c] = sum_{far d} pes[k] T(off(c,d))[k] C_n[k][d]
+//             + sum_{(i,w) in W^T(k)} w conj(ph_{n-1}(oct_c, i)) A_{n-1}[i][P]
+// for the children c of P.  The far boxes of every child of P are the
+// children of P's neighbour parents Q that do not touch it, so a wave loads
+// each source row C_n[k][d] once and applies it to all (up to 8) children of
+// P; the transfer function of a pair depends only on the integer offset
+// off = 2(Q - P) + oct_d - oct_c in box units, so the k-major table Tk holds
+// one entry per offset code (pes folded in).  Parent-level data come in
+// through the combined weights wp[oct][e] = w_e conj(ph(oct, row_e)).
+// Workgroups that share a sample run on the same XCD (workgroup id mod 8),
+// where the k-slice of C_n and the Tk row stay in L2.
+constexpr int kOffCodes = 343;  // (dx, dy, dz) in [-3, 3]^3
+
+template <typename S>
+__global__ void __launch_bounds__(256) k_b_far(int nk, int nb, int nP, int ncol, int nchunk, int wgs_per_k,
+                                               const int* __restrict__ pc_ptr, const int* __restrict__ bo,
+                                               const int* __restrict__ q_ptr, const int2* __restrict__ q_ent,
+                                               const typename Cx<S>::T* __restrict__ Tk,
+                                               const typename Cx<S>::T* __restrict__ C, int nbc, int nwt,
+                                               const int* __restrict__ wt_ptr, const int* __restrict__ wt_row,
+                                               const typename Cx<S>::T* __restrict__ wp,
+                                               const typename Cx<S>::T* __restrict__ Ac,
+                                               typename Cx<S>::T* __restrict__ A) {
+  typedef typename Cx<S>::T T2;
+  const int g = blockIdx.x;
+  const int k = (g / 8 / wgs_per_k) * 8 + g % 8;
   if (k >= nk) return;
-  const int per = blockDim.x / (2 * R);  // boxes per workgroup
-  const int b = bb * per + threadIdx.x / (2 * R);
-  if (b >= nb) return;
-  const int pr = threadIdx.x % (2 * R);
-  const float2* Ck = C + (size_t)k * nb * 2 * R;
-  float2 acc = make_float2(0.f, 0.f);
-  for (int m = far_ptr[b]; m < far_ptr[b + 1]; ++m)
-    acc = fadd(acc, fmul(Ck[(size_t)far_box[m] * 2 * R + pr], T[(size_t)far_tr[m] * nk + k]));
-  const float pk = pes[k];
-  B[((size_t)k * nb + b) * 2 * R + pr] = make_float2(acc.x * pk, acc.y * pk);
-}
-
-// Downward pass: A_n[s][c] = B_n[s][c] + sum_{(i,w) in W^T(s)} w conj(ph(oct_c, i)) A_{n-1}[i][P]
-__global__ void __launch_bounds__(256) k_b_interp_down(int R, int nkc, int nbc, int nkf, int nbf,
-                                                       const int* __restrict__ parent, const int* __restrict__ oct,
-                                                       const int* __restrict__ wt_ptr, const int* __restrict__ wt_row,
-                                                       const float* __restrict__ wt_w, const float2* __restrict__ ph,
-                                                       const float2* __restrict__ Ac, const float2* __restrict__ Bf,
-                                                       float2* __restrict__ Af) {
-  const int c = blockIdx.x;
-  const long long t = (long long)blockIdx.y * blockDim.x + threadIdx.x;
-  if (t >= (long long)nkf * 2 * R) return;
-  const int r = (int)(t % R), p = (int)((t / R) % 2), s = (int)(t / (2 * R));
-  const int P = parent[c], o = oct[c];
-  float2 acc = make_float2(0.f, 0.f);
-  for (int e = wt_ptr[s]; e < wt_ptr[s + 1]; ++e) {
-    const int i = wt_row[e];
-    const float2 v = fmulc(ph[(size_t)o * nkc + i], Ac[(((size_t)i * nbc + P) * 2 + p) * R + r]);
-    acc.x += wt_w[e] * v.x;
-    acc.y += wt_w[e] * v.y;
+  const int t = ((g / 8) % wgs_per_k) * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+  const int P = t / nchunk;
+  if (P >= nP) return;
+  const int col = (t % nchunk) * kWave + (threadIdx.x % kWave);
+  const int c0 = pc_ptr[P], nc = pc_ptr[P + 1] - c0;
+  int oc[8];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) oc[q] = q < nc ? bo[c0 + q] : 0;
+  T2 acc[8];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) acc[q] = mk<S>(0, 0);
+  const T2* Tkk = Tk + (size_t)k * kOffCodes;
+  const T2* Ck = C + (size_t)k * nb * ncol + col;
+  // offset code of (target child q, source child d of Q) = (2*o + bits(d) -
+  // bits(c_q) + 3) in base 7 = base(o) + cd(d) - cq[q]; near offsets and
+  // absent children need no test: Tk is zero at every code without a
+  // transfer vector and rows of absent children are never stored
+  int cq[8];
+#pragma unroll
+  for (int q = 0; q < 8; ++q) cq[q] = (oc[q] >> 2) * 49 + ((oc[q] >> 1) & 1) * 7 + (oc[q] & 1);
+  for (int e = q_ptr[P]; e < q_ptr[P + 1]; ++e) {
+    const int2 qe = q_ent[e];
+    const int ox = qe.y / 9 - 1, oy = (qe.y / 3) % 3 - 1, oz = qe.y % 3 - 1;
+    const int base = (2 * ox + 3) * 49 + (2 * oy + 3) * 7 + (2 * oz + 3);
+    const int d0 = pc_ptr[qe.x], nd = pc_ptr[qe.x + 1] - d0;
+    T2 v[8];
+    int cd[8];
+#pragma unroll
+    for (int jd = 0; jd < 8; ++jd) {
+      const int od = jd < nd ? bo[d0 + jd] : 0;
+      cd[jd] = base + (od >> 2) * 49 + ((od >> 1) & 1) * 7 + (od & 1);
+      v[jd] = jd < nd ? Ck[(size_t)(d0 + jd) * ncol] : mk<S>(0, 0);
+    }
+#pragma unroll
+    for (int jd = 0; jd < 8; ++jd) {
+      if (jd < nd) {
+#pragma unroll
+        for (int q = 0; q < 8; ++q) {
+          const T2 tt = Tkk[cd[jd] - cq[q]];
+          acc[q].x += tt.x * v[jd].x - tt.y * v[jd].y;
+          acc[q].y += tt.x * v[jd].y + tt.y * v[jd].x;
+        }
+      }
+    }
   }
-  const size_t at = (((size_t)s * nbf + c) * 2 + p) * R + r;
-  Af[at] = fadd(Bf[at], acc);
+  if (Ac) {
+    const int e0 = wt_ptr[k], e1 = wt_ptr[k + 1];
+    for (int e = e0; e < e1; e += 4) {
+      T2 a[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) a[u] = e + u < e1 ? Ac[((size_t)wt_row[e + u] * nbc + P) * ncol + col] : mk<S>(0, 0);
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        if (e + u < e1) {
+#pragma unroll
+          for (int q = 0; q < 8; ++q) {
+            const T2 w = wp[(size_t)oc[q] * nwt + e + u];
+            acc[q].x += w.x * a[u].x - w.y * a[u].y;
+            acc[q].y += w.x * a[u].y + w.y * a[u].x;
+          }
+        }
+      }
+    }
+  }
+  T2* Ak = A + (size_t)k * nb * ncol + col;
+#pragma unroll
+  for (int q = 0; q < 8; ++q)
+    if (q < nc) Ak[(size_t)(c0 + q) * ncol] = acc[q];
 }
 
-// Receive, one workgroup per leaf box: Yf[basis(s)][r] = sum_(p,k) conj(rad(s;p,k)) A_L[k][b][p][r]
-__global__ void __launch_bounds__(256) k_b_receive(int R, int nk, int nb, const int* __restrict__ box_slot,
+// Tk[k][code] = pes[k] T[tr(code)][k] (zero for codes without a transfer vector)
+template <typename S>
+__global__ void k_b_tk(int nk, const int* __restrict__ code_tr, const double2* __restrict__ T,
+                       const double* __restrict__ pes, typename Cx<S>::T* __restrict__ Tk) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)nk * kOffCodes) return;
+  const int k = (int)(t / kOffCodes), code = (int)(t % kOffCodes);
+  const int tr = code_tr[code];
+  if (tr < 0) {
+    Tk[t] = mk<S>(0, 0);
+    return;
+  }
+  const double2 v = T[(size_t)tr * nk + k];
+  Tk[t] = mk<S>((S)(v.x * pes[k]), (S)(v.y * pes[k]));
+}
+
+// wp[o][e] = w_e conj(ph(o, row_e)): anterpolation weight of transposed entry e
+// for a child in octant o
+template <typename S>
+__global__ void k_b_wp(int nkc, int nwt, const int* __restrict__ wt_row, const double* __restrict__ wt_w,
+                       const double2* __restrict__ ph, typename Cx<S>::T* __restrict__ wp) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= 8LL * nwt) return;
+  const int o = (int)(t / nwt), e = (int)(t % nwt);
+  const double2 p = ph[(size_t)o * nkc + wt_row[e]];
+  const double w = wt_w[e];
+  wp[t] = mk<S>((S)(w * p.x), (S)(-w * p.y));
+}
+
+// Receive on MFMA, one workgroup per leaf box:
+//   Yf[basis(s)][r] = sum_m conj(rad(s; m)) A_L[m][b][r],  m = p*nk + k,
+// a complex GEMM with the 2nk items as the contraction.  A wave owns 16 of
+// the box's bases and 64 columns.  Each lane loads 4 consecutive items of one
+// pattern row at once (coalesced pieces), so within a block of 16 items, MFMA
+// step q pairs lane group g with item 4g + q; the A_L operand is fetched with
+// the same assignment.
+template <typename S, int R>
+__global__ void __launch_bounds__(128) k_b_receive(int nk, int nb, const int* __restrict__ box_slot,
                                                    const int* __restrict__ slot_basis,
-                                                   const float2* __restrict__ rad, const float2* __restrict__ AL,
-                                                   float2* __restrict__ Yf) {
-  constexpr int CH = 32;
-  extern __shared__ float2 la[];  // (CH, R)
+                                                   const typename Cx<S>::T* __restrict__ rad,
+                                                   const typename Cx<S>::T* __restrict__ AL,
+                                                   typename Cx<S>::T* __restrict__ Yf) {
+  typedef typename Cx<S>::T T2;
+  typedef typename Cx<S>::Acc Acc;
+  constexpr int NT = 4, NCH = R / 64;
   const int b = blockIdx.x;
   const int s0 = box_slot[b], nf = box_slot[b + 1] - s0;
   const int nk2 = 2 * nk;
-  const int r = threadIdx.x % R, per = blockDim.x / R;
-  const int sbase = blockIdx.y * 8 * per;  // this workgroup's slice of the box's bases
-  if (sbase >= nf) return;
-  float2 acc[8];
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave), l = threadIdx.x % kWave;
+  const int li = l % 16, lk = l / 16;
+  const int nst = (nf + 15) / 16;
+  for (int task = wave; task < nst * NCH; task += blockDim.x / kWave) {
+    const int st = task / NCH, c0 = (task % NCH) * 64;
+    const int sl = st * 16 + li;  // basis row this lane loads
+    const T2* rrow = rad + (size_t)(s0 + min(sl, nf - 1)) * nk2;
+    const bool row_ok = sl < nf;
+    Acc dr[NT], di[NT];
 #pragma unroll
-  for (int q = 0; q < 8; ++q) acc[q] = make_float2(0.f, 0.f);
-  for (int base = 0; base < nk2; base += CH) {
-    __syncthreads();
-    for (int e = threadIdx.x; e < CH * R; e += blockDim.x) {
-      const int item = base + e / R;
-      if (item < nk2) {
-        const int p = item / nk, k = item % nk;
-        la[e] = AL[(((size_t)k * nb + b) * 2 + p) * R + (e % R)];
-      } else {
-        la[e] = make_float2(0.f, 0.f);
+    for (int jt = 0; jt < NT; ++jt) {
+      dr[jt] = Acc{0, 0, 0, 0};
+      di[jt] = Acc{0, 0, 0, 0};
+    }
+    for (int mb = 0; mb < nk2; mb += 16) {
+      T2 a4[4];
+      const int m0 = mb + 4 * lk;
+#pragma unroll
+      for (int q = 0; q < 4; ++q) a4[q] = (row_ok && m0 + q < nk2) ? rrow[m0 + q] : mk<S>(0, 0);
+#pragma unroll
+      for (int q = 0; q < 4; ++q) {
+        const int m = m0 + q;  // item of this lane group in step q
+        const T2* ap = nullptr;
+        if (m < nk2) {
+          const int p = m / nk, k = m % nk;
+          ap = AL + (((size_t)k * nb + b) * 2 + p) * R + c0 + li;
+        }
+#pragma unroll
+        for (int jt = 0; jt < NT; ++jt) {
+          const T2 x = ap ? ap[jt * 16] : mk<S>(0, 0);
+          dr[jt] = mma(a4[q].x, x.x, dr[jt]);
+          dr[jt] = mma(a4[q].y, x.y, dr[jt]);
+          di[jt] = mma(a4[q].x, x.y, di[jt]);
+          di[jt] = mma(-a4[q].y, x.x, di[jt]);
+        }
       }
     }
-    __syncthreads();
 #pragma unroll
-    for (int q = 0; q < 8; ++q) {
-      const int s = sbase + threadIdx.x / R + q * per;
-      if (s < nf) {
-        const float2* rs = rad + (size_t)(s0 + s) * nk2 + base;
-        const int lim = min(CH, nk2 - base);
-        for (int j = 0; j < lim; ++j) acc[q] = fadd(acc[q], fmulc(rs[j], la[j * R + r]));
+    for (int v = 0; v < 4; ++v) {
+      const int so = st * 16 + mma_row<S>(lk, v);
+      if (so < nf) {
+        T2* o = Yf + (size_t)slot_basis[s0 + so] * R + c0 + li;
+#pragma unroll
+        for (int jt = 0; jt < NT; ++jt) o[jt * 16] = mk<S>(dr[jt][v], di[jt][v]);
       }
     }
-  }
-#pragma unroll
-  for (int q = 0; q < 8; ++q) {
-    const int s = sbase + threadIdx.x / R + q * per;
-    if (s < nf) Yf[(size_t)slot_basis[s0 + s] * R + r] = acc[q];
   }
 }
 
-// Near field (FP64 accumulation of complex(4) values) plus the scaled far field.
-template <typename V>
-__global__ void __launch_bounds__(256) k_b_near(int R, int ne, const int* __restrict__ ia, const int* __restrict__ ja,
-                                                const V* __restrict__ z, const double2* __restrict__ X,
-                                                const float2* __restrict__ Yf, double2 cc, double2* __restrict__ Y) {
-  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  const int m = (int)(t / R), r = (int)(t % R);
-  if (m >= ne) return;
-  double2 acc = make_double2(0.0, 0.0);
-  for (int k = ia[m]; k < ia[m + 1]; ++k) acc = dadd(acc, dmul(to_d2(z[k]), X[(size_t)ja[k] * R + r]));
-  const double2 f = dmul(to_d2(Yf[(size_t)m * R + r]), cc);
-  Y[(size_t)m * R + r] = dadd(f, acc);
+// Near field plus the scaled far field, one workgroup per leaf box.  Every
+// row of a box has the same near columns (the bases of the box's near boxes,
+// ascending), so the box's block of the near matrix is a dense (nf x ncol)
+// matrix applied to ncol rows of X: a wave takes 16 rows and 64 right-hand
+// sides and streams the columns once, each X row feeding 16 complex FP64
+// FMAs.  The matrix entries are wave-uniform (scalar loads).  Row sums run
+// in the CSR column order, as in the single-RHS product.
+constexpr int kNearRows = 16;
+
+template <typename V, typename F>
+__global__ void __launch_bounds__(256) k_b_near_box(int R, const int* __restrict__ box_slot,
+                                                    const int* __restrict__ slot_basis,
+                                                    const int* __restrict__ col_ptr, const int* __restrict__ cols,
+                                                    const int* __restrict__ ia, const V* __restrict__ z,
+                                                    const double2* __restrict__ X, const F* __restrict__ Yf,
+                                                    double2 cc, double2* __restrict__ Y) {
+  const int b = xcd_block(blockIdx.x, gridDim.x);
+  const int s0 = box_slot[b], nf = box_slot[b + 1] - s0;
+  const int j0 = col_ptr[b], ncol = col_ptr[b + 1] - j0;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave), lane = threadIdx.x % kWave;
+  const int nchunk = R / kWave, ngroups = (nf + kNearRows - 1) / kNearRows;
+  const int* cb = cols + j0;
+  for (int task = wave; task < ngroups * nchunk; task += blockDim.x / kWave) {
+    const int g = task / nchunk, r = (task % nchunk) * kWave + lane;
+    const int nr = min(kNearRows, nf - g * kNearRows);
+    int zr[kNearRows];
+#pragma unroll
+    for (int q = 0; q < kNearRows; ++q) zr[q] = q < nr ? ia[slot_basis[s0 + g * kNearRows + q]] : 0;
+    double2 acc[kNearRows];
+#pragma unroll
+    for (int q = 0; q < kNearRows; ++q) acc[q] = make_double2(0.0, 0.0);
+    constexpr int U = 4;  // columns per step: their X loads are in flight together
+    for (int j = 0; j < ncol; j += U) {
+      double2 x[U];
+#pragma unroll
+      for (int u = 0; u < U; ++u) x[u] = j + u < ncol ? X[(size_t)cb[j + u] * R + r] : make_double2(0.0, 0.0);
+#pragma unroll
+      for (int u = 0; u < U; ++u) {
+        if (j + u < ncol) {
+#pragma unroll
+          for (int q = 0; q < kNearRows; ++q) {
+            if (q < nr) {
+              const double2 zz = to_d2(z[zr[q] + j + u]);
+              acc[q].x += zz.x * x[u].x - zz.y * x[u].y;
+              acc[q].y += zz.x * x[u].y + zz.y * x[u].x;
+            }
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < kNearRows; ++q) {
+      if (q < nr) {
+        const size_t at = (size_t)slot_basis[s0 + g * kNearRows + q] * R + r;
+        const F f = Yf[at];
+        Y[at] = dadd(dmul(make_double2((double)f.x, (double)f.y), cc), acc[q]);
+      }
+    }
+  }
 }
 
 // ---------------------------------------------------------------- ILU0 solves (FP64)
@@ -366,6 +658,59 @@ __global__ void k_b_copy_masked(int R, int n, const int* __restrict__ active, co
   if (active[t % R]) o[t] = a[t];
 }
 
+// Per-column CGS control on the device, so that iterations run back to back
+// (and replay as one captured graph) with the host looking in only every few
+// iterations.  st[c] = {active, first, converged, iterations}.
+__global__ void k_b_cgs_init(int R, const double2* __restrict__ nrm, const double2* __restrict__ rhon,
+                             double* __restrict__ nbv, double* __restrict__ rr, int4* __restrict__ stv,
+                             double2* __restrict__ rho1) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= R) return;
+  nbv[c] = sqrt(nrm[c].x);
+  rr[c] = 0.0;
+  const bool brk = rhon[c].x == 0.0 && rhon[c].y == 0.0;  // CGSold exits before its first update
+  stv[c] = make_int4(brk ? 0 : 1, 1, 0, 0);
+  rho1[c] = rhon[c];
+}
+
+// start of an iteration: a column that has used maxit iterations stops
+__global__ void k_b_cgs_start(int R, int maxit, int4* __restrict__ stv, int* __restrict__ active,
+                              int* __restrict__ first) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= R) return;
+  int4 v = stv[c];
+  if (v.x && v.w >= maxit) v.x = 0;
+  if (v.x) v.w += 1;
+  stv[c] = v;
+  active[c] = v.x;
+  first[c] = v.y;
+}
+
+// end of an iteration: the stopping test of CGSold on the recurrence residual
+// (modIterativo.f90:453-457) and the breakdown exit (:397), then the rho shift
+__global__ void k_b_cgs_check(int R, double errmax, const double2* __restrict__ nrm,
+                              const double2* __restrict__ rhon, const double* __restrict__ nbv,
+                              double* __restrict__ rr, int4* __restrict__ stv, double2* __restrict__ rho1,
+                              double2* __restrict__ rho2) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= R) return;
+  int4 v = stv[c];
+  if (v.x) {
+    const double nr = sqrt(nrm[c].x);
+    rr[c] = nr / nbv[c];
+    v.y = 0;
+    if (nr < errmax * nbv[c]) {
+      v.z = 1;
+      v.x = 0;
+    } else if (rhon[c].x == 0.0 && rhon[c].y == 0.0) {
+      v.x = 0;
+    }
+    stv[c] = v;
+  }
+  rho2[c] = rho1[c];
+  rho1[c] = rhon[c];
+}
+
 // ---------------------------------------------------------------- excitation and RCS per column
 
 __global__ void k_b_plane_wave(int R, int ne, const double* __restrict__ tb, const double* __restrict__ pc,
@@ -450,27 +795,44 @@ int nblk(long long n, int b) { return (int)((n + b - 1) / b); }
 
 // ---------------------------------------------------------------- state
 
+// Far-field arrays of one level; the scalar type (float or double) is the
+// batch's far-field precision.
 struct BatchLevel {
   int nk = 0, nb = 0;
-  float2* T = nullptr;
-  float* pes = nullptr;
-  float* wint = nullptr;  // interpolation to this level (n < L)
-  float* wt_w = nullptr;
-  float2* ph = nullptr;   // (8, nk) child-offset phases at this level's samples
-  int* oct = nullptr;     // octant of each box within its parent (n > 3)
-  float2 *C = nullptr, *B = nullptr, *A = nullptr;
+  // factored interpolation stencil to this level from level n+1 (n < L)
+  int *sth_idx = nullptr, *sph_idx = nullptr, *pcu_ptr = nullptr;  // pcu_ptr: children at n+1
+  void *sth_w = nullptr, *sph_w = nullptr;
+  void* ph = nullptr;  // (8, nk) child-offset phases at this level's samples (n < L)
+  int* oct = nullptr;  // octant of each box within its parent (n > 3)
+  // fused translation + downward pass at this level (parents at level n-1)
+  int nP = 0, nwt = 0;
+  int* pc_ptr = nullptr;  // (nP+1) children of each parent
+  int* bo = nullptr;      // (nb) octant of each box within its parent
+  int* q_ptr = nullptr;   // (nP+1) neighbour parents holding far boxes
+  int2* q_ent = nullptr;  // (Q, parent offset code (ox+1)*9 + (oy+1)*3 + (oz+1))
+  void* Tk = nullptr;     // (nk, 343) pes-scaled transfer function per offset code
+  void* wp = nullptr;     // (8, nwt) anterpolation weights from level n-1 (n > 3)
+  void *C = nullptr, *A = nullptr;
 };
 
 struct Batch {
   Handle* h = nullptr;
   int R = 0;
-  float2* rad32 = nullptr;
+  bool fp64 = false;  // far-field precision
+  int maxnf = 0;      // largest number of bases in a leaf box
+  int *ncol_ptr = nullptr, *ncols = nullptr;  // near columns of each leaf box (shared by its rows)
+  void* rad = nullptr;                        // leaf patterns in the far-field precision
   std::vector<BatchLevel> lv;
   double2 *X = nullptr, *Y = nullptr, *w[12] = {};
-  float2* Yf = nullptr;
+  void* Yf = nullptr;
   double* part = nullptr;
   double2* sc = nullptr;  // 8 per-column scalar rows of R
   int *active = nullptr, *first = nullptr;
+  int4* stv = nullptr;                     // per-column CGS state
+  double *nbv = nullptr, *rrv = nullptr;   // per-column |b| and relative residual
+  hipGraphExec_t iter_graph[2] = {nullptr, nullptr};  // one CGS iteration, without / with ILU0
+  double graph_errmax[2] = {0.0, 0.0};
+  int graph_maxit[2] = {0, 0};
   std::vector<void*> allocs;
 };
 
@@ -488,62 +850,203 @@ static int bmalloc(Batch* b, T** p, size_t n) {
   return LW_OK;
 }
 
+template <typename T>
+static int upload_b(Batch* b, T** p, const std::vector<T>& v) {
+  int rc = bmalloc(b, p, v.size());
+  if (rc) return rc;
+  if (!v.empty() && hipMemcpy(*p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice) != hipSuccess) {
+    set_error("lw_batch_create: upload failed");
+    return LW_ERR_HIP;
+  }
+  return LW_OK;
+}
+
+// Interaction lists of level n regrouped by parent pairs (see k_b_far).
+struct FarBlocks {
+  std::vector<int> bo, q_ptr, code_tr;
+  std::vector<int2> q_ent;
+};
+
+// Derives the parent-pair blocks from the plan's per-box far lists and checks
+// that they reproduce them exactly: the same far boxes for every box and one
+// transfer vector per offset code.  Returns false with msg on a mismatch.
+static bool far_blocks(const Plan& P, int n, FarBlocks& fb, std::string& msg) {
+  const LevelPlan& lp = P.lv[n];
+  const LevelPlan& pp = P.lv[n - 1];
+  const int nb = lp.nb, nP = pp.nb;
+  fb.bo.assign(nb, 0);
+  for (int c = 0; c < nb; ++c) {
+    const double* cc = &lp.centre[3 * c];
+    const double* pc = &pp.centre[3 * lp.parent[c]];
+    fb.bo[c] = ((cc[0] > pc[0]) ? 4 : 0) | ((cc[1] > pc[1]) ? 2 : 0) | ((cc[2] > pc[2]) ? 1 : 0);
+  }
+  for (int Pb = 0; Pb < nP; ++Pb)
+    for (int c = pp.child_ptr[Pb]; c < pp.child_ptr[Pb + 1]; ++c)
+      if (lp.parent[c] != Pb) {
+        msg = "children of a parent are not contiguous";
+        return false;
+      }
+  std::vector<std::vector<int>> qs(nP);
+  for (int c = 0; c < nb; ++c)
+    for (int m = lp.far_ptr[c]; m < lp.far_ptr[c + 1]; ++m) qs[lp.parent[c]].push_back(lp.parent[lp.far_box[m]]);
+  fb.q_ptr.assign(nP + 1, 0);
+  fb.q_ent.clear();
+  fb.code_tr.assign(kOffCodes, -1);
+  long long generated = 0;
+  std::vector<int> gen, ref;
+  for (int Pb = 0; Pb < nP; ++Pb) {
+    auto& q = qs[Pb];
+    std::sort(q.begin(), q.end());
+    q.erase(std::unique(q.begin(), q.end()), q.end());
+    for (int Q : q) {
+      int o[3];
+      for (int k = 0; k < 3; ++k) {
+        const double dv = (pp.centre[3 * Q + k] - pp.centre[3 * Pb + k]) / pp.lado;
+        o[k] = (int)std::lround(dv);
+        if (std::fabs(dv - o[k]) > 1e-6 || o[k] < -1 || o[k] > 1) {
+          msg = "far box outside the parent's neighbourhood";
+          return false;
+        }
+      }
+      fb.q_ent.push_back(make_int2(Q, (o[0] + 1) * 9 + (o[1] + 1) * 3 + (o[2] + 1)));
+    }
+    fb.q_ptr[Pb + 1] = (int)fb.q_ent.size();
+    // regenerate every child's far list from the blocks and compare
+    for (int c = pp.child_ptr[Pb]; c < pp.child_ptr[Pb + 1]; ++c) {
+      gen.clear();
+      for (int e = fb.q_ptr[Pb]; e < fb.q_ptr[Pb + 1]; ++e) {
+        const int Q = fb.q_ent[e].x, oc = fb.q_ent[e].y;
+        const int o[3] = {oc / 9 - 1, (oc / 3) % 3 - 1, oc % 3 - 1};
+        for (int d = pp.child_ptr[Q]; d < pp.child_ptr[Q + 1]; ++d) {
+          int off[3], far = 0;
+          for (int k = 0; k < 3; ++k) {
+            const int bit = 2 - k;
+            off[k] = 2 * o[k] + ((fb.bo[d] >> bit) & 1) - ((fb.bo[c] >> bit) & 1);
+            far |= off[k] > 1 || off[k] < -1;
+          }
+          if (far) gen.push_back(d * kOffCodes + (off[0] + 3) * 49 + (off[1] + 3) * 7 + (off[2] + 3));
+        }
+      }
+      ref.clear();
+      for (int m = lp.far_ptr[c]; m < lp.far_ptr[c + 1]; ++m) {
+        const int f = lp.far_box[m];
+        int code = 0;
+        for (int k = 0; k < 3; ++k) {
+          const int off = (int)std::lround((lp.centre[3 * f + k] - lp.centre[3 * c + k]) / lp.lado);
+          if (off < -3 || off > 3) {
+            msg = "far box offset outside [-3, 3]";
+            return false;
+          }
+          code = code * 7 + (off + 3);
+        }
+        int& slot = fb.code_tr[code];
+        if (slot < 0) slot = lp.far_tr[m];
+        if (slot != lp.far_tr[m]) {
+          msg = "two transfer vectors for one box offset";
+          return false;
+        }
+        ref.push_back(f * kOffCodes + code);
+      }
+      std::sort(gen.begin(), gen.end());
+      std::sort(ref.begin(), ref.end());
+      if (gen != ref) {
+        msg = "interaction lists are not parent-neighbour blocks";
+        return false;
+      }
+      generated += (long long)gen.size();
+    }
+  }
+  if (generated != (long long)lp.far_box.size()) {
+    msg = "interaction list count mismatch";
+    return false;
+  }
+  return true;
+}
+
 static void free_batch(Batch* b) {
   if (!b) return;
   if (b->h && b->h->stream) (void)hipStreamSynchronize(b->h->stream);
+  for (auto& g : b->iter_graph)
+    if (g) (void)hipGraphExecDestroy(g);
   for (void* p : b->allocs) (void)hipFree(p);
   delete b;
 }
 
-static int batch_matvec(Batch* bt, const double2* X, double2* Y) {
+// bases per LDS chunk of the aggregation kernel (x rows of R columns)
+template <typename S, int R>
+constexpr int agg_chunk() {
+  return std::min(64, (int)(49152 / ((R + 8) * 2 * sizeof(S))) / 4 * 4);
+}
+
+template <typename S, int R>
+static void launch_leaf(Batch* bt, bool up, const double2* X) {
+  typedef typename Cx<S>::T T2;
+  Handle* h = bt->h;
+  BatchLevel& lf = bt->lv[h->L];
+  if (up) {
+    constexpr int KC = agg_chunk<S, R>();
+    const int ntile = (2 * lf.nk + 15) / 16;
+    hipLaunchKernelGGL((k_b_aggregate<S, R, KC>), dim3(lf.nb, (ntile + 3) / 4), dim3(256), 0, h->stream, lf.nk,
+                       lf.nb, h->d_box_slot, h->d_slot_basis, (const T2*)bt->rad, X, (T2*)lf.C);
+  } else {
+    hipLaunchKernelGGL((k_b_receive<S, R>), dim3(lf.nb), dim3(128), 0, h->stream, lf.nk, lf.nb, h->d_box_slot,
+                       h->d_slot_basis, (const T2*)bt->rad, (const T2*)lf.A, (T2*)bt->Yf);
+  }
+}
+
+template <typename S>
+static int batch_matvec_t(Batch* bt, const double2* X, double2* Y) {
+  typedef typename Cx<S>::T T2;
   Handle* h = bt->h;
   const int R = bt->R, L = h->L;
   hipStream_t st = h->stream;
   BatchLevel& lf = bt->lv[L];
-  int maxnf = 0;
-  for (int b = 0; b < lf.nb; ++b) maxnf = std::max(maxnf, h->plan.box_slot[b + 1] - h->plan.box_slot[b]);
-  {
-    const int per = 256 / R;
-    dim3 grid(lf.nb, std::max(1, std::min(16, nblk(2 * lf.nk, per))));
-    hipLaunchKernelGGL(k_b_aggregate, grid, dim3(256), maxnf * R * sizeof(float2), st, R, lf.nk, lf.nb,
-                       h->d_box_slot, h->d_slot_basis, bt->rad32, X, lf.C);
-  }
+  if (R == 64)
+    launch_leaf<S, 64>(bt, true, X);
+  else
+    launch_leaf<S, 128>(bt, true, X);
+  const int ncol = 2 * R, nchunk = ncol / kWave;
   for (int n = L; n >= 4; --n) {
     BatchLevel& c = bt->lv[n];
     BatchLevel& p = bt->lv[n - 1];
-    dim3 grid(p.nb, nblk((long long)p.nk * 2 * R, 256));
-    hipLaunchKernelGGL(k_b_interp_up, grid, dim3(256), 0, st, R, p.nk, p.nb, c.nk, c.nb, h->lv[n - 1].child_ptr, c.oct,
-                       p.wint, h->lv[n - 1].iint, p.ph, c.C, p.C);
+    const LevelPlan& pp = h->plan.lv[n - 1];
+    const LevelPlan& cp = h->plan.lv[n];
+    const int wpg = std::max(1, std::min(4, 65536 / (pp.nth * kWave * (int)sizeof(T2))));
+    const long long waves = (long long)p.nb * pp.nph * nchunk;
+    hipLaunchKernelGGL(k_b_interp_up<S>, dim3(nblk(waves, wpg)), dim3(wpg * kWave),
+                       (size_t)wpg * pp.nth * kWave * sizeof(T2), st, ncol, pp.nth, p.nk, p.nb, cp.nth, c.nb, pp.nph,
+                       p.pcu_ptr, c.oct, p.sth_idx, (const S*)p.sth_w, p.sph_idx, (const S*)p.sph_w,
+                       (const T2*)p.ph, (const T2*)c.C, (T2*)p.C);
   }
   for (int n = 3; n <= L; ++n) {
     BatchLevel& d = bt->lv[n];
-    const int per = 256 / (2 * R);
-    const int nbb = nblk(d.nb, per);
-    const int groups = nblk(d.nk, 8);
-    hipLaunchKernelGGL(k_b_translate, dim3(groups * 8 * nbb), dim3(256), 0, st, R, d.nk, d.nb, nbb, h->lv[n].far_ptr,
-                       h->lv[n].far_box, h->lv[n].far_tr, d.T, d.pes, d.C, d.B);
+    const BatchLevel* pl = n > 3 ? &bt->lv[n - 1] : nullptr;
+    const int wgs_per_k = nblk((long long)d.nP * nchunk, 4);
+    const int grid = 8 * nblk(d.nk, 8) * wgs_per_k;
+    hipLaunchKernelGGL(k_b_far<S>, dim3(grid), dim3(256), 0, st, d.nk, d.nb, d.nP, ncol, nchunk, wgs_per_k, d.pc_ptr,
+                       d.bo, d.q_ptr, d.q_ent, (const T2*)d.Tk, (const T2*)d.C, pl ? pl->nb : 0, d.nwt,
+                       pl ? h->lv[n - 1].wt_ptr : nullptr, pl ? h->lv[n - 1].wt_row : nullptr, (const T2*)d.wp,
+                       pl ? (const T2*)pl->A : nullptr, (T2*)d.A);
   }
-  for (int n = 4; n <= L; ++n) {
-    BatchLevel& c = bt->lv[n];
-    BatchLevel& p = bt->lv[n - 1];
-    const float2* src = (n == 4) ? p.B : p.A;
-    dim3 grid(c.nb, nblk((long long)c.nk * 2 * R, 256));
-    hipLaunchKernelGGL(k_b_interp_down, grid, dim3(256), 0, st, R, p.nk, p.nb, c.nk, c.nb, h->lv[n].parent, c.oct,
-                       h->lv[n - 1].wt_ptr, h->lv[n - 1].wt_row, p.wt_w, p.ph, src, c.B, c.A);
-  }
-  hipLaunchKernelGGL(k_b_receive, dim3(lf.nb, nblk(maxnf, 8 * (256 / R))), dim3(256), 32 * R * sizeof(float2), st,
-                     R, lf.nk, lf.nb,
-                     h->d_box_slot, h->d_slot_basis, bt->rad32, lf.A, bt->Yf);
-  const double2 cc = make_double2(0.0, h->plan.phys.w * kMu / (4.0 * kPi));
-  const long long tot = (long long)h->ne * R;
-  if (h->fp64_near)
-    hipLaunchKernelGGL(k_b_near<double2>, dim3(nblk(tot, 256)), dim3(256), 0, st, R, h->ne, h->d_ia, h->d_ja,
-                       (const double2*)h->d_z, X, bt->Yf, cc, Y);
+  if (R == 64)
+    launch_leaf<S, 64>(bt, false, nullptr);
   else
-    hipLaunchKernelGGL(k_b_near<float2>, dim3(nblk(tot, 256)), dim3(256), 0, st, R, h->ne, h->d_ia, h->d_ja,
-                       (const float2*)h->d_z, X, bt->Yf, cc, Y);
+    launch_leaf<S, 128>(bt, false, nullptr);
+  const double2 cc = make_double2(0.0, h->plan.phys.w * kMu / (4.0 * kPi));
+  if (h->fp64_near)
+    hipLaunchKernelGGL((k_b_near_box<double2, T2>), dim3(lf.nb), dim3(128), 0, st, R, h->d_box_slot,
+                       h->d_slot_basis, bt->ncol_ptr, bt->ncols, h->d_ia, (const double2*)h->d_z, X,
+                       (const T2*)bt->Yf, cc, Y);
+  else
+    hipLaunchKernelGGL((k_b_near_box<float2, T2>), dim3(lf.nb), dim3(128), 0, st, R, h->d_box_slot,
+                       h->d_slot_basis, bt->ncol_ptr, bt->ncols, h->d_ia, (const float2*)h->d_z, X,
+                       (const T2*)bt->Yf, cc, Y);
   LW_HIP(hipGetLastError());
   return LW_OK;
+}
+
+static int batch_matvec(Batch* bt, const double2* X, double2* Y) {
+  return bt->fp64 ? batch_matvec_t<double>(bt, X, Y) : batch_matvec_t<float>(bt, X, Y);
 }
 
 static int batch_lusol(Batch* bt, const double2* y, double2* x) {
@@ -575,6 +1078,191 @@ static int batch_dot2(Batch* bt, const double2* a, const double2* b, int ca, con
   return LW_OK;
 }
 
+// One CGSold iteration for every active column (modIterativo.f90:376-486),
+// enqueued on the handle's stream; it is captured once into a graph.
+static int batch_cgs_iteration(Batch* bt, bool use_prec, double errmax, int maxit) {
+  Handle* h = bt->h;
+  const int R = bt->R, n = h->ne;
+  hipStream_t st = h->stream;
+  const int vb = nblk((long long)n * R, 256);
+  double2 *r = bt->w[1], *R_ = bt->w[2], *u = bt->w[3], *p = bt->w[4], *P_ = bt->w[5], *V = bt->w[6],
+          *q = bt->w[7], *U = bt->w[8], *Q = bt->w[9], *t = bt->w[10], *x = bt->X;
+  double2 *rho1 = bt->sc, *rho2 = bt->sc + R, *den = bt->sc + 2 * R, *alfa = bt->sc + 3 * R, *nrm = bt->sc + 4 * R,
+          *rhon = bt->sc + 5 * R;
+  int rc;
+  hipLaunchKernelGGL(k_b_cgs_start, dim3(nblk(R, 64)), dim3(64), 0, st, R, maxit, bt->stv, bt->active, bt->first);
+  hipLaunchKernelGGL(k_b_cgs_up, dim3(vb), dim3(256), 0, st, R, n, bt->active, bt->first, rho1, rho2, r, q, u, p);
+  if (use_prec) {
+    if ((rc = batch_lusol(bt, p, P_))) return rc;
+  } else {
+    hipLaunchKernelGGL(k_b_copy_masked, dim3(vb), dim3(256), 0, st, R, n, bt->active, p, P_);
+  }
+  if ((rc = batch_matvec(bt, P_, V))) return rc;
+  if ((rc = batch_dot2(bt, R_, V, 0, nullptr, nullptr, 0, den, nullptr))) return rc;
+  hipLaunchKernelGGL(k_b_cgs_q, dim3(vb), dim3(256), 0, st, R, n, bt->active, rho1, den, alfa, u, V, q, t);
+  if (use_prec) {
+    if ((rc = batch_lusol(bt, t, U))) return rc;
+  } else {
+    hipLaunchKernelGGL(k_b_copy_masked, dim3(vb), dim3(256), 0, st, R, n, bt->active, t, U);
+  }
+  if ((rc = batch_matvec(bt, U, Q))) return rc;
+  hipLaunchKernelGGL(k_b_cgs_xr, dim3(vb), dim3(256), 0, st, R, n, bt->active, rho1, den, U, Q, x, r);
+  if ((rc = batch_dot2(bt, r, r, 1, r, R_, 0, nrm, rhon))) return rc;
+  hipLaunchKernelGGL(k_b_cgs_check, dim3(nblk(R, 64)), dim3(64), 0, st, R, errmax, nrm, rhon, bt->nbv, bt->rrv,
+                     bt->stv, rho1, rho2);
+  LW_HIP(hipGetLastError());
+  return LW_OK;
+}
+
+// CGS on every column from b = w[0] and the initial guess in X: the initial
+// residual, then captured iterations replayed in groups of kGraphGroup with
+// one host check of the column states per group.
+constexpr int kGraphGroup = 8;
+
+static int batch_cgs_run(Batch* bt, double errmax, bool use_prec, int maxit) {
+  Handle* h = bt->h;
+  const int R = bt->R;
+  const long long nv = (long long)h->ne * R;
+  hipStream_t st = h->stream;
+  double2 *b = bt->w[0], *r = bt->w[1], *R_ = bt->w[2], *t = bt->w[10];
+  double2 *rho1 = bt->sc, *nrm = bt->sc + 4 * R, *rhon = bt->sc + 5 * R;
+  int rc = batch_matvec(bt, bt->X, t);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_b_sub, dim3(nblk(nv, 256)), dim3(256), 0, st, nv, b, t, r);
+  LW_HIP(hipMemcpyAsync(R_, r, sizeof(double2) * nv, hipMemcpyDeviceToDevice, st));
+  if ((rc = batch_dot2(bt, b, b, 1, r, R_, 0, nrm, rhon))) return rc;
+  hipLaunchKernelGGL(k_b_cgs_init, dim3(nblk(R, 64)), dim3(64), 0, st, R, nrm, rhon, bt->nbv, bt->rrv, bt->stv,
+                     rho1);
+  LW_HIP(hipGetLastError());
+  // the iteration graph bakes in errmax and maxit: recapture when they change
+  hipGraphExec_t& gx = bt->iter_graph[use_prec ? 1 : 0];
+  if (gx && (bt->graph_errmax[use_prec] != errmax || bt->graph_maxit[use_prec] != maxit)) {
+    LW_HIP(hipGraphExecDestroy(gx));
+    gx = nullptr;
+  }
+  if (!gx) {
+    hipGraph_t g = nullptr;
+    LW_HIP(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+    rc = batch_cgs_iteration(bt, use_prec, errmax, maxit);
+    hipError_t e = hipStreamEndCapture(st, &g);
+    if (rc) {
+      if (g) (void)hipGraphDestroy(g);
+      return rc;
+    }
+    LW_HIP(e);
+    e = hipGraphInstantiate(&gx, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    LW_HIP(e);
+    bt->graph_errmax[use_prec] = errmax;
+    bt->graph_maxit[use_prec] = maxit;
+  }
+  std::vector<int4> sv(R);
+  std::vector<double> rr(R);
+  int done = 0, next_report = progress_every();
+  while (done < maxit) {
+    const int g = std::min(kGraphGroup, maxit - done);
+    for (int j = 0; j < g; ++j) LW_HIP(hipGraphLaunch(gx, st));
+    done += g;
+    LW_HIP(hipMemcpyAsync(sv.data(), bt->stv, sizeof(int4) * R, hipMemcpyDeviceToHost, st));
+    LW_HIP(hipMemcpyAsync(rr.data(), bt->rrv, sizeof(double) * R, hipMemcpyDeviceToHost, st));
+    LW_HIP(hipStreamSynchronize(st));
+    int na = 0;
+    double worst = 0.0;
+    for (int c = 0; c < R; ++c)
+      if (sv[c].x) {
+        ++na;
+        worst = std::max(worst, rr[c]);
+      }
+    if (next_report > 0 && done >= next_report) {
+      std::fprintf(stderr, "[lw_batch_cgs] iteration %d: %d of %d columns active, worst relres %.3e\n", done, na,
+                   R, worst);
+      next_report += progress_every();
+    }
+    if (!na) break;
+  }
+  return LW_OK;
+}
+
+// Far-field tables of every level in the precision S (patterns, phases,
+// stencil weights, per-offset transfer functions, anterpolation weights)
+// and the campo arrays.
+template <typename S>
+static int setup_far(Batch* bt) {
+  typedef typename Cx<S>::T T2;
+  Handle* h = bt->h;
+  const Plan& P = h->plan;
+  const int L = h->L, R = bt->R;
+  hipStream_t st = h->stream;
+  int rc;
+  const long long nrad = (long long)P.lv[L].nk * 2 * h->ne;
+  if (sizeof(S) == sizeof(double)) {
+    bt->rad = h->d_rad;  // the operator's own FP64 patterns
+  } else {
+    T2* r = nullptr;
+    if ((rc = bmalloc(bt, &r, (size_t)nrad))) return rc;
+    hipLaunchKernelGGL(k_to_prec<S>, dim3(nblk(nrad, 256)), dim3(256), 0, st, nrad, h->d_rad, r);
+    bt->rad = r;
+  }
+  for (int n = 3; n <= L; ++n) {
+    const LevelPlan& lp = P.lv[n];
+    DevLevel& d = h->lv[n];
+    BatchLevel& b = bt->lv[n];
+    b.nk = lp.nk;
+    b.nb = lp.nb;
+    if (n < L) {
+      std::vector<S> tw(lp.sth_w.begin(), lp.sth_w.end()), pw(lp.sph_w.begin(), lp.sph_w.end());
+      S *dtw = nullptr, *dpw = nullptr;
+      if ((rc = upload_b(bt, &b.sth_idx, lp.sth_idx)) || (rc = upload_b(bt, &b.sph_idx, lp.sph_idx)) ||
+          (rc = upload_b(bt, &dtw, tw)) || (rc = upload_b(bt, &dpw, pw)) ||
+          (rc = upload_b(bt, &b.pcu_ptr, lp.child_ptr)))
+        return rc;
+      b.sth_w = dtw;
+      b.sph_w = dpw;
+      if (sizeof(S) == sizeof(double)) {
+        b.ph = d.ph;
+      } else {
+        T2* ph = nullptr;
+        if ((rc = bmalloc(bt, &ph, 8 * (size_t)lp.nk))) return rc;
+        hipLaunchKernelGGL(k_to_prec<S>, dim3(nblk(8LL * lp.nk, 256)), dim3(256), 0, st, 8LL * lp.nk, d.ph, ph);
+        b.ph = ph;
+      }
+    }
+    b.oct = d.oct;
+    FarBlocks fb;
+    std::string msg;
+    if (!far_blocks(P, n, fb, msg)) {
+      set_error("lw_batch_create: " + msg);
+      return LW_ERR_STATE;
+    }
+    b.nP = P.lv[n - 1].nb;
+    int* d_code_tr = nullptr;
+    if ((rc = upload_b(bt, &b.pc_ptr, P.lv[n - 1].child_ptr)) || (rc = upload_b(bt, &b.bo, fb.bo)) ||
+        (rc = upload_b(bt, &b.q_ptr, fb.q_ptr)) || (rc = upload_b(bt, &b.q_ent, fb.q_ent)) ||
+        (rc = upload_b(bt, &d_code_tr, fb.code_tr)))
+      return rc;
+    T2* tk = nullptr;
+    if ((rc = bmalloc(bt, &tk, (size_t)lp.nk * kOffCodes))) return rc;
+    hipLaunchKernelGGL(k_b_tk<S>, dim3(nblk((long long)lp.nk * kOffCodes, 256)), dim3(256), 0, st, lp.nk,
+                       d_code_tr, d.T, d.pes, tk);
+    b.Tk = tk;
+    if (n > 3) {
+      const LevelPlan& pp = P.lv[n - 1];
+      b.nwt = (int)pp.wt_w.size();
+      T2* wp = nullptr;
+      if ((rc = bmalloc(bt, &wp, 8 * (size_t)b.nwt))) return rc;
+      hipLaunchKernelGGL(k_b_wp<S>, dim3(nblk(8LL * b.nwt, 256)), dim3(256), 0, st, pp.nk, b.nwt,
+                         h->lv[n - 1].wt_row, h->lv[n - 1].wt_w, h->lv[n - 1].ph, wp);
+      b.wp = wp;
+    }
+    const size_t fsz = (size_t)lp.nk * lp.nb * 2 * R;
+    T2 *c = nullptr, *a = nullptr;
+    if ((rc = bmalloc(bt, &c, fsz)) || (rc = bmalloc(bt, &a, fsz))) return rc;
+    b.C = c;
+    b.A = a;
+  }
+  return LW_OK;
+}
+
 }  // namespace lw
 
 struct lw_batch {
@@ -583,10 +1271,10 @@ struct lw_batch {
 
 extern "C" {
 
-int lw_batch_create(lw_handle* hh, int nrhs, lw_batch** out) {
+int lw_batch_create(lw_handle* hh, int nrhs, int flags, lw_batch** out) {
   using namespace lw;
-  if (!hh || !out || !(nrhs == 8 || nrhs == 16 || nrhs == 32 || nrhs == 64)) {
-    set_error("lw_batch_create: nrhs must be 8, 16, 32 or 64");
+  if (!hh || !out || !(nrhs == 64 || nrhs == 128) || (flags & ~LW_BATCH_FAR_FP64)) {
+    set_error("lw_batch_create: nrhs must be 64 or 128; flags: LW_BATCH_FAR_FP64");
     return LW_ERR_ARG;
   }
   *out = nullptr;
@@ -595,6 +1283,7 @@ int lw_batch_create(lw_handle* hh, int nrhs, lw_batch** out) {
   Batch* bt = &ob->b;
   bt->h = h;
   bt->R = nrhs;
+  bt->fp64 = (flags & LW_BATCH_FAR_FP64) != 0;
   const int L = h->L, R = nrhs;
   const Plan& P = h->plan;
   int rc;
@@ -607,46 +1296,43 @@ int lw_batch_create(lw_handle* hh, int nrhs, lw_batch** out) {
     }                          \
   } while (0)
   hipStream_t st = h->stream;
-  const long long nrad = (long long)P.lv[L].nk * 2 * h->ne;
-  TRY(bmalloc(bt, &bt->rad32, (size_t)nrad));
-  hipLaunchKernelGGL(k_to_f32, dim3(nblk(nrad, 256)), dim3(256), 0, st, nrad, h->d_rad, bt->rad32);
   bt->lv.assign(L + 1, BatchLevel());
-  for (int n = 3; n <= L; ++n) {
-    const LevelPlan& lp = P.lv[n];
-    DevLevel& d = h->lv[n];
-    BatchLevel& b = bt->lv[n];
-    b.nk = lp.nk;
-    b.nb = lp.nb;
-    const long long nT = (long long)lp.nk * std::max(lp.nT, 1);
-    TRY(bmalloc(bt, &b.T, (size_t)nT));
-    hipLaunchKernelGGL(k_to_f32, dim3(nblk(nT, 256)), dim3(256), 0, st, nT, d.T, b.T);
-    TRY(bmalloc(bt, &b.pes, (size_t)lp.nk));
-    hipLaunchKernelGGL(k_to_f32r, dim3(nblk(lp.nk, 256)), dim3(256), 0, st, (long long)lp.nk, d.pes, b.pes);
-    if (n < L) {
-      TRY(bmalloc(bt, &b.wint, 9 * (size_t)lp.nk));
-      hipLaunchKernelGGL(k_to_f32r, dim3(nblk(9LL * lp.nk, 256)), dim3(256), 0, st, 9LL * lp.nk, d.wint, b.wint);
-      TRY(bmalloc(bt, &b.wt_w, lp.wt_w.size()));
-      hipLaunchKernelGGL(k_to_f32r, dim3(nblk((long long)lp.wt_w.size(), 256)), dim3(256), 0, st,
-                         (long long)lp.wt_w.size(), d.wt_w, b.wt_w);
-      // child offset phases at this level's samples (the FP64 table, rounded)
-      TRY(bmalloc(bt, &b.ph, 8 * (size_t)lp.nk));
-      hipLaunchKernelGGL(k_to_f32, dim3(nblk(8LL * lp.nk, 256)), dim3(256), 0, st, 8LL * lp.nk, d.ph, b.ph);
+  TRY(bt->fp64 ? setup_far<double>(bt) : setup_far<float>(bt));
+  for (int b = 0; b < P.lv[L].nb; ++b) bt->maxnf = std::max(bt->maxnf, P.box_slot[b + 1] - P.box_slot[b]);
+  {
+    // the near columns of each leaf box, checked to be those of every row of the box
+    std::vector<int> cp(P.lv[L].nb + 1, 0), cl;
+    for (int b = 0; b < P.lv[L].nb; ++b) {
+      const int m0 = P.slot_basis[P.box_slot[b]];
+      const int n0 = P.ia[m0 + 1] - P.ia[m0];
+      for (int s = P.box_slot[b]; s < P.box_slot[b + 1]; ++s) {
+        const int m = P.slot_basis[s];
+        bool same = P.ia[m + 1] - P.ia[m] == n0;
+        for (int j = 0; same && j < n0; ++j) same = P.ja[P.ia[m] + j] == P.ja[P.ia[m0] + j];
+        if (!same) {
+          set_error("lw_batch_create: rows of a leaf box have different near columns");
+          free_batch(bt);
+          return LW_ERR_STATE;
+        }
+      }
+      cl.insert(cl.end(), P.ja.begin() + P.ia[m0], P.ja.begin() + P.ia[m0 + 1]);
+      cp[b + 1] = (int)cl.size();
     }
-    b.oct = d.oct;
-    const size_t fsz = (size_t)lp.nk * lp.nb * 2 * R;
-    TRY(bmalloc(bt, &b.C, fsz));
-    TRY(bmalloc(bt, &b.B, fsz));
-    TRY(bmalloc(bt, &b.A, fsz));
+    TRY(upload_b(bt, &bt->ncol_ptr, cp));
+    TRY(upload_b(bt, &bt->ncols, cl));
   }
   const size_t nv = (size_t)h->ne * R;
   TRY(bmalloc(bt, &bt->X, nv));
   TRY(bmalloc(bt, &bt->Y, nv));
   for (auto& w : bt->w) TRY(bmalloc(bt, &w, nv));
-  TRY(bmalloc(bt, &bt->Yf, nv));
+  TRY(bmalloc(bt, (char**)&bt->Yf, nv * (bt->fp64 ? sizeof(double2) : sizeof(float2))));
   TRY(bmalloc(bt, &bt->part, (size_t)kRedBlocks * R * 4));
   TRY(bmalloc(bt, &bt->sc, (size_t)8 * R));
   TRY(bmalloc(bt, &bt->active, (size_t)R));
   TRY(bmalloc(bt, &bt->first, (size_t)R));
+  TRY(bmalloc(bt, &bt->stv, (size_t)R));
+  TRY(bmalloc(bt, &bt->nbv, (size_t)R));
+  TRY(bmalloc(bt, &bt->rrv, (size_t)R));
 #undef TRY
   if (hipStreamSynchronize(st) != hipSuccess || hipGetLastError() != hipSuccess) {
     set_error("lw_batch_create: conversion kernels failed");
@@ -660,6 +1346,9 @@ int lw_batch_create(lw_handle* hh, int nrhs, lw_batch** out) {
 int lw_batch_destroy(lw_batch* b) {
   if (b) {
     lw::Batch* bt = &b->b;
+    if (bt->h && bt->h->stream) (void)hipStreamSynchronize(bt->h->stream);
+    for (auto& g : bt->iter_graph)
+      if (g) (void)hipGraphExecDestroy(g);
     for (void* p : bt->allocs) (void)hipFree(p);
     bt->allocs.clear();
     delete b;
@@ -705,95 +1394,26 @@ int lw_batch_cgs(lw_batch* ob, const double* Bv, const double* guess, double err
     set_error("lw_batch_cgs: preconditioner requested but no ILU0 factor");
     return LW_ERR_STATE;
   }
-  const int R = bt->R, n = h->ne;
-  const size_t nv = (size_t)n * R, bytes = sizeof(double2) * nv;
+  const int R = bt->R;
+  const size_t bytes = sizeof(double2) * (size_t)h->ne * R;
   hipStream_t st = h->stream;
-  double2 *b = bt->w[0], *r = bt->w[1], *R_ = bt->w[2], *u = bt->w[3], *p = bt->w[4], *P_ = bt->w[5],
-          *V = bt->w[6], *q = bt->w[7], *U = bt->w[8], *Q = bt->w[9], *t = bt->w[10], *x = bt->X;
-  double2 *rho1 = bt->sc, *rho2 = bt->sc + R, *den = bt->sc + 2 * R, *alfa = bt->sc + 3 * R, *nrm = bt->sc + 4 * R,
-          *rhon = bt->sc + 5 * R;
-  LW_HIP(hipMemcpy(b, Bv, bytes, hipMemcpyHostToDevice));
+  LW_HIP(hipMemcpy(bt->w[0], Bv, bytes, hipMemcpyHostToDevice));
   if (guess)
-    LW_HIP(hipMemcpy(x, guess, bytes, hipMemcpyHostToDevice));
+    LW_HIP(hipMemcpy(bt->X, guess, bytes, hipMemcpyHostToDevice));
   else
-    LW_HIP(hipMemset(x, 0, bytes));
-  int rc = batch_matvec(bt, x, t);
+    LW_HIP(hipMemset(bt->X, 0, bytes));
+  int rc = batch_cgs_run(bt, errmax, use_prec != 0, maxit);
   if (rc) return rc;
-  const int vb = nblk((long long)nv, 256);
-  hipLaunchKernelGGL(k_b_sub, dim3(vb), dim3(256), 0, st, (long long)nv, b, t, r);
-  LW_HIP(hipMemcpyAsync(R_, r, bytes, hipMemcpyDeviceToDevice, st));
-  if ((rc = batch_dot2(bt, b, b, 1, r, R_, 0, nrm, rhon))) return rc;
-  std::vector<double2> hn(R), hr(R);
-  LW_HIP(hipMemcpyAsync(hn.data(), nrm, sizeof(double2) * R, hipMemcpyDeviceToHost, st));
-  LW_HIP(hipMemcpyAsync(hr.data(), rhon, sizeof(double2) * R, hipMemcpyDeviceToHost, st));
-  LW_HIP(hipStreamSynchronize(st));
-  std::vector<double> nb(R);
-  for (int c = 0; c < R; ++c) nb[c] = std::sqrt(hn[c].x);
-  LW_HIP(hipMemcpyAsync(rho1, rhon, sizeof(double2) * R, hipMemcpyDeviceToDevice, st));
-  std::vector<int> act(R, 1), fst(R, 1), it(R, 0), conv(R, 0);
-  std::vector<double> rr(R, 0.0);
-  for (int c = 0; c < R; ++c)
-    if (hr[c].x == 0.0 && hr[c].y == 0.0) act[c] = 0;
-  for (int k = 0; k < maxit; ++k) {
-    int nact = 0;
-    for (int c = 0; c < R; ++c) nact += act[c];
-    if (!nact) break;
-    for (int c = 0; c < R; ++c)
-      if (act[c]) it[c] += 1;
-    LW_HIP(hipMemcpyAsync(bt->active, act.data(), sizeof(int) * R, hipMemcpyHostToDevice, st));
-    LW_HIP(hipMemcpyAsync(bt->first, fst.data(), sizeof(int) * R, hipMemcpyHostToDevice, st));
-    hipLaunchKernelGGL(k_b_cgs_up, dim3(vb), dim3(256), 0, st, R, n, bt->active, bt->first, rho1, rho2, r, q, u, p);
-    if (use_prec) {
-      if ((rc = batch_lusol(bt, p, P_))) return rc;
-    } else {
-      hipLaunchKernelGGL(k_b_copy_masked, dim3(vb), dim3(256), 0, st, R, n, bt->active, p, P_);
-    }
-    if ((rc = batch_matvec(bt, P_, V))) return rc;
-    if ((rc = batch_dot2(bt, R_, V, 0, nullptr, nullptr, 0, den, nullptr))) return rc;
-    hipLaunchKernelGGL(k_b_cgs_q, dim3(vb), dim3(256), 0, st, R, n, bt->active, rho1, den, alfa, u, V, q, t);
-    if (use_prec) {
-      if ((rc = batch_lusol(bt, t, U))) return rc;
-    } else {
-      hipLaunchKernelGGL(k_b_copy_masked, dim3(vb), dim3(256), 0, st, R, n, bt->active, t, U);
-    }
-    if ((rc = batch_matvec(bt, U, Q))) return rc;
-    hipLaunchKernelGGL(k_b_cgs_xr, dim3(vb), dim3(256), 0, st, R, n, bt->active, rho1, den, U, Q, x, r);
-    if ((rc = batch_dot2(bt, r, r, 1, r, R_, 0, nrm, rhon))) return rc;
-    LW_HIP(hipMemcpyAsync(hn.data(), nrm, sizeof(double2) * R, hipMemcpyDeviceToHost, st));
-    LW_HIP(hipMemcpyAsync(hr.data(), rhon, sizeof(double2) * R, hipMemcpyDeviceToHost, st));
-    LW_HIP(hipStreamSynchronize(st));
-    for (int c = 0; c < R; ++c) {
-      if (!act[c]) continue;
-      const double nr = std::sqrt(hn[c].x);
-      rr[c] = nr / nb[c];
-      fst[c] = 0;
-      if (nr < errmax * nb[c]) {
-        conv[c] = 1;
-        act[c] = 0;
-      } else if (hr[c].x == 0.0 && hr[c].y == 0.0) {
-        act[c] = 0;  // breakdown exit of the next iteration
-      }
-    }
-    if (progress_every() > 0 && (k + 1) % progress_every() == 0) {
-      int na = 0;
-      double worst = 0.0;
-      for (int c = 0; c < R; ++c) {
-        na += act[c];
-        if (act[c]) worst = std::max(worst, rr[c]);
-      }
-      std::fprintf(stderr, "[lw_batch_cgs] iteration %d: %d of %d columns active, worst relres %.3e\n", k + 1, na, R,
-                   worst);
-    }
-    // rho2 = rho1; rho1 = next (columns that stopped no longer read them)
-    LW_HIP(hipMemcpyAsync(rho2, rho1, sizeof(double2) * R, hipMemcpyDeviceToDevice, st));
-    LW_HIP(hipMemcpyAsync(rho1, rhon, sizeof(double2) * R, hipMemcpyDeviceToDevice, st));
-  }
-  LW_HIP(hipMemcpyAsync(Xv, x, bytes, hipMemcpyDeviceToHost, st));
+  std::vector<int4> sv(R);
+  std::vector<double> rr(R);
+  LW_HIP(hipMemcpyAsync(Xv, bt->X, bytes, hipMemcpyDeviceToHost, st));
+  LW_HIP(hipMemcpyAsync(sv.data(), bt->stv, sizeof(int4) * R, hipMemcpyDeviceToHost, st));
+  LW_HIP(hipMemcpyAsync(rr.data(), bt->rrv, sizeof(double) * R, hipMemcpyDeviceToHost, st));
   LW_HIP(hipStreamSynchronize(st));
   for (int c = 0; c < R; ++c) {
-    if (iters) iters[c] = it[c];
+    if (iters) iters[c] = sv[c].w;
     if (relres) relres[c] = rr[c];
-    if (converged) converged[c] = conv[c];
+    if (converged) converged[c] = sv[c].z;
   }
   return LW_OK;
 }

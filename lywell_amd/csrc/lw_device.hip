@@ -73,10 +73,6 @@ __device__ __forceinline__ double wave_sum(double v) {
 // the linear id to a work index so that each XCD receives one contiguous range
 // (a bijection on [0, n)): neighbouring boxes, whose fields and interaction
 // lists overlap, then share one XCD's L2.
-__device__ __forceinline__ int xcd_block(int w, int n) {
-  const int q = n / 8, rem = n % 8, x = w % 8, j = w / 8;
-  return x * q + min(x, rem) + j;
-}
 
 // Leaf aggregation (modMLFMA.f90:1805-1812): campoC_L(:,:,box) =
 // sum_f funcRAD(:,:,f) x(f).  One workgroup per leaf box.

@@ -14,6 +14,15 @@ namespace lw {
 void set_error(const std::string& m);
 int progress_every();
 
+// Workgroup id w of n -> logical block: the hardware deals workgroups to the
+// 8 XCDs round-robin (XCD = w mod 8); this bijection gives each XCD one
+// contiguous range of logical blocks, so neighbouring blocks (which share
+// operands) share that XCD's L2.
+__device__ __forceinline__ int xcd_block(int w, int n) {
+  const int q = n / 8, rem = n % 8, x = w % 8, j = w / 8;
+  return x * q + min(x, rem) + j;
+}
+
 #define LW_HIP(call)                                                                 \
   do {                                                                               \
     hipError_t _e = (call);                                                          \

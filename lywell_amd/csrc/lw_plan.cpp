@@ -215,6 +215,10 @@ void fill_interp(LevelPlan& lc, const LevelPlan& lf) {
   const long long nth = lf.nth, nph = lf.nph;
   lc.wint.resize(9 * (size_t)lc.nk);
   lc.iint.resize(9 * (size_t)lc.nk);
+  lc.sth_idx.assign(3 * (size_t)lc.nth, 0);
+  lc.sth_w.assign(3 * (size_t)lc.nth, 0.0);
+  lc.sph_idx.assign(3 * (size_t)lc.nph, 0);
+  lc.sph_w.assign(3 * (size_t)lc.nph, 0.0);
   const bool phi_inc = lf.ang[2 * nth + 1] - lf.ang[1] > 0;
   const bool th_inc = lf.ang[2] - lf.ang[0] > 0;
   for (int i = 0; i < lc.nk; ++i) {
@@ -239,6 +243,15 @@ void fill_interp(LevelPlan& lc, const LevelPlan& lf) {
         lc.wint[9 * (size_t)i + 3 * a + b] = wt[b] * wp[a];
         lc.iint[9 * (size_t)i + 3 * a + b] = V[3 * a + b] - 1;
       }
+    // the stencil is a tensor product: theta part per coarse theta row, phi
+    // part per coarse phi column (sample i = iphi*nth + itheta)
+    const int it = i % lc.nth, ip = i / lc.nth;
+    for (int b = 0; b < 3; ++b) {
+      lc.sth_idx[3 * it + b] = (int)thes[b] - 1;
+      lc.sth_w[3 * it + b] = wt[b];
+      lc.sph_idx[3 * ip + b] = (int)phis[b] - 1;
+      lc.sph_w[3 * ip + b] = wp[b];
+    }
   }
   // transpose for the gather form of Anterpolar: fine sample -> (coarse row,
   // weight) in ascending coarse row order, the order the reference's scatter

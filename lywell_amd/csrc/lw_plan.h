@@ -49,6 +49,10 @@ struct LevelPlan {
   std::vector<int> iint;     // (9, nk) 0-based fine sample
   std::vector<int> wt_ptr, wt_row;  // transpose: per fine sample, coarse rows ascending
   std::vector<double> wt_w;
+  // the same stencil factored: fine theta rows (pt, lower, upper) and weights
+  // per coarse theta index, fine phi columns and weights per coarse phi index
+  std::vector<int> sth_idx, sph_idx;  // (3, nth), (3, nph), 0-based
+  std::vector<double> sth_w, sph_w;
 };
 
 struct Plan {

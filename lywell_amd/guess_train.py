@@ -1,0 +1,157 @@
+"""Training of the learned initial guess on solutions the solver generates.
+
+    python -m lywell_amd.guess_train [--meshes 12] [--subdiv 4] [--angles 64]
+                                     [--steps 3000] [--out lywell_amd/guess_mlp.npz]
+
+Training meshes are seeded random bodies: ellipsoids with semi-axes uniform
+in [amin, amax] wavelengths and spheres perturbed by real spherical
+harmonics of degree 2-4 (5% total amplitude), icosphere subdivision
+`subdiv`.  For each mesh a monostatic sweep of `angles` incidence angles
+(VV on even meshes, HH on odd ones) is solved in one batch to `errmax`; every
+angle a >= 2 of the sweep becomes one training sample per unknown, with the
+converged currents of angles a-1 and a-2 as inputs and that of angle a as
+the target.  The network (16 -> 256 x 3 -> 4) is trained by the library's
+own forward/backward kernels (lw_guess_train: Adam on random minibatches).
+The weights and a held-out evaluation are written to `out`.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+if ROOT not in sys.path:
+    sys.path.insert(0, ROOT)
+
+import lywell_amd as lw  # noqa: E402
+from lywell_amd.sweep import polarisations, sweep_directions, warm_start  # noqa: E402
+
+DEFAULT_OUT = os.path.join(os.path.dirname(os.path.abspath(__file__)), "guess_mlp.npz")
+
+
+def log(*a):
+    print(*a, flush=True)
+
+
+def random_body(rng, subdiv, amin, amax, kind):
+    if kind == "ellipsoid":
+        ax = rng.uniform(amin, amax, 3)
+        return lw.Mesh.icosphere(subdiv, *ax), {"kind": kind, "axes": ax.tolist()}
+    radius = rng.uniform(amin, amax)
+    degs = rng.integers(2, 5, 3)
+    ords = np.array([rng.integers(-d, d + 1) for d in degs])
+    coef = rng.standard_normal(3)
+    coef *= 0.05 / np.abs(coef).sum()
+    mesh = lw.Mesh.perturbed_sphere(subdiv, radius, degs.astype(np.int32), ords.astype(np.int32), coef)
+    return mesh, {"kind": kind, "radius": float(radius), "deg": degs.tolist(), "ord": ords.tolist(),
+                  "coef": coef.tolist()}
+
+
+def solve_sweep(op, n, pol, errmax, maxit, R=64):
+    """All n angles of one sweep from zero guesses, in batches of R."""
+    phi, dirs = sweep_directions(n)
+    pols = polarisations(dirs, pol)
+    X = np.zeros((op.ne, n), np.complex128)
+    its = np.zeros(n, np.int64)
+    bt = lw.Batch(op, R, far_fp64=True)
+    for a0 in range(0, n, R):
+        ang = np.minimum(np.arange(a0, a0 + R), n - 1)
+        B = bt.plane_wave(dirs[ang], pols[ang].astype(np.complex128))
+        Xb, it, rr, cv = bt.cgs(B, errmax=errmax, use_prec=False, maxit=maxit)
+        if not cv.all():
+            raise RuntimeError("training sweep did not converge")
+        keep = np.arange(a0, a0 + R) < n
+        X[:, ang[keep]] = Xb[:, keep]
+        its[ang[keep]] = it[keep]
+    bt.close()
+    return phi, dirs, pols, X, its
+
+
+def samples(net, op, dirs, pols, X):
+    """Angles 2..n-1 of a solved sweep as training rows."""
+    n = X.shape[1]
+    a = np.arange(2, n)
+    net.add_samples(op, dirs[a], pols[a], dirs[a - 1], X[:, a - 1], dirs[a - 2], X[:, a - 2], X[:, a])
+
+
+def guess_errors(net, op, dirs, pols, X):
+    """Relative error of the warm start, the linear extrapolation and the
+    learned guess on angles 2..n-1 of a solved sweep."""
+    n = X.shape[1]
+    a = np.arange(2, n)
+    k0 = 2 * np.pi / op.lam
+    xt1 = warm_start(X[:, a - 1], op.mesh.e_centro, k0, dirs[a], dirs[a - 1])
+    xt2 = warm_start(X[:, a - 2], op.mesh.e_centro, k0, dirs[a], dirs[a - 2])
+    g = net.predict_guess(op, dirs[a], pols[a], dirs[a - 1], X[:, a - 1], dirs[a - 2], X[:, a - 2])
+    nrm = np.linalg.norm(X[:, a], axis=0)
+    return {"warm": np.linalg.norm(xt1 - X[:, a], axis=0) / nrm,
+            "extrapolation": np.linalg.norm(2 * xt1 - xt2 - X[:, a], axis=0) / nrm,
+            "learned": np.linalg.norm(g - X[:, a], axis=0) / nrm}
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--meshes", type=int, default=12)
+    ap.add_argument("--subdiv", type=int, default=4)
+    ap.add_argument("--amin", type=float, default=0.5)
+    ap.add_argument("--amax", type=float, default=2.0)
+    ap.add_argument("--ladomax", type=float, default=0.2)
+    ap.add_argument("--angles", type=int, default=64)
+    ap.add_argument("--errmax", type=float, default=1e-6)
+    ap.add_argument("--maxit", type=int, default=6000)
+    ap.add_argument("--steps", type=int, default=3000)
+    ap.add_argument("--batch-rows", type=int, default=16384)
+    ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--seed", type=int, default=2024)
+    ap.add_argument("--out", default=DEFAULT_OUT)
+    args = ap.parse_args(argv)
+    if lw.device_count() < 1:
+        raise SystemExit("guess_train: no HIP device")
+    rng = np.random.default_rng(args.seed)
+    net = lw.Mlp(16, 256, 3, 4, seed=args.seed)
+    bodies = []
+    t0 = time.time()
+    for i in range(args.meshes + 1):  # the last mesh is held out
+        kind = "ellipsoid" if i % 2 == 0 else "perturbed_sphere"
+        mesh, meta = random_body(rng, args.subdiv, args.amin, args.amax, kind)
+        op = lw.MLFMA(mesh, lam=1.0, ladomax=args.ladomax)
+        pol = "VV" if i % 2 == 0 else "HH"
+        phi, dirs, pols, X, its = solve_sweep(op, args.angles, pol, args.errmax, args.maxit)
+        meta.update({"ne": mesh.ne, "levels": op.nlevels, "pol": pol, "iterations_mean": float(its.mean())})
+        if i < args.meshes:
+            samples(net, op, dirs, pols, X)
+            bodies.append(meta)
+            log(f"mesh {i}: {kind} {mesh.ne} RWG, {pol}, sweep solved (mean {its.mean():.0f} iterations), "
+                f"{net.nsamples()} rows, {time.time() - t0:.1f}s")
+            op.close()
+        else:
+            held = (op, dirs, pols, X, meta)
+    loss = net.train(args.steps, batch_rows=args.batch_rows, lr=args.lr, seed=args.seed)
+    log(f"trained {args.steps} steps: loss {loss[:10].mean():.4e} -> {loss[-50:].mean():.4e}")
+    op, dirs, pols, X, meta = held
+    err = guess_errors(net, op, dirs, pols, X)
+    ev = {k: float(np.median(v)) for k, v in err.items()}
+    log("held-out median relative error of the guess:", json.dumps(ev))
+    np.savez(args.out, params=net.get_params(), arch=np.array([16, 256, 3, 4]),
+             loss=loss, meta=json.dumps({"train_meshes": bodies, "held_out": meta, "held_out_error": ev,
+                                         "args": vars(args)}))
+    log(f"wrote {args.out}")
+    return ev
+
+
+def load(path=DEFAULT_OUT):
+    """The trained network from `path`, or None when it does not exist."""
+    if not os.path.exists(path):
+        return None
+    z = np.load(path, allow_pickle=False)
+    arch = [int(v) for v in z["arch"]]
+    net = lw.Mlp(*arch)
+    net.set_params(z["params"])
+    return net
+
+
+if __name__ == "__main__":
+    main()

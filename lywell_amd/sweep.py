@@ -1,0 +1,96 @@
+"""Monostatic RCS sweep over the batched solver.
+
+Host side of gen_rcs_monostatica (main.f90:967-1089): incidence angles
+phi_a = pi*a/n (a = 0..n-1) in the theta = 90 deg plane, propagation
+d = (-cos phi, -sin phi, 0), VV (pol = z) or HH (pol = d x z / |d x z|)
+polarisation as ajustar_polariz sets them (main.f90:828-849), one CGSold
+solve per angle to errmax, sigma = sigmaRCS(pi/2, phi) (main.f90:1050).
+
+The reference walks the angles one after another and starts each solve from
+the previous angle's current times exp(-j k (d - d_prev) . e_centro)
+(main.f90:1017-1022).  Here R chains of consecutive angles run side by side
+in one batched solve: column c of block j solves angle c*nblocks + j, so
+every chain keeps the reference's one-step warm start.  The guess of a block
+is zero (first block), that warm start, or the learned guess (from the two
+previous angles of the chain).
+"""
+import time
+
+import numpy as np
+
+
+def sweep_directions(n):
+    phi = np.pi * np.arange(n) / n
+    return phi, np.stack([-np.cos(phi), -np.sin(phi), np.zeros(n)], axis=1)
+
+
+def polarisations(dirs, kind="VV"):
+    """ajustar_polariz (main.f90:828-849): VV = z; HH = d x z normalised."""
+    if kind == "VV":
+        return np.tile(np.array([0.0, 0.0, 1.0]), (len(dirs), 1))
+    h = np.cross(dirs, np.array([0.0, 0.0, 1.0]))
+    return h / np.linalg.norm(h, axis=1, keepdims=True)
+
+
+def warm_start(x_prev, e_centro, k0, d, d_prev):
+    """x_prev * exp(-j k (d - d_prev) . e_centro) per unknown and column."""
+    E = np.asarray(e_centro).reshape(-1, 3)
+    ph = np.exp(-1j * k0 * (E @ (np.asarray(d) - np.asarray(d_prev)).T))
+    return x_prev * ph
+
+
+class Sweep:
+    """n angles through a batch of R columns (lywell_amd.Batch)."""
+
+    def __init__(self, op, batch, errmax=1e-4, maxit=20000, guess="warm", mlp=None, pol="VV", log=None):
+        if guess not in ("zero", "warm", "mlp"):
+            raise ValueError("guess: zero, warm or mlp")
+        if guess == "mlp" and mlp is None:
+            raise ValueError("the learned guess needs a trained network")
+        self.op, self.bt = op, batch
+        self.errmax, self.maxit = errmax, maxit
+        self.guess, self.mlp, self.pol = guess, mlp, pol
+        self.k0 = 2 * np.pi / op.lam
+        self.log = log or (lambda *a: None)
+
+    def run(self, n, keep_currents=False):
+        R = self.bt.R
+        nblocks = (n + R - 1) // R
+        phi, dirs = sweep_directions(n)
+        pols = polarisations(dirs, self.pol)
+        sigma = np.zeros(n)
+        iters = np.zeros(n, np.int64)
+        conv = np.zeros(n, bool)
+        currents = np.zeros((self.op.ne, n), np.complex128) if keep_currents else None
+        hist = []  # (angle indices, directions, currents) of the last two blocks
+        t0 = time.time()
+        blocks = []
+        for j in range(nblocks):
+            ang = np.minimum(np.arange(R) * nblocks + j, n - 1)  # chains; the tail repeats the last angle
+            d, p = dirs[ang], pols[ang]
+            B = self.bt.plane_wave(d, p.astype(np.complex128))
+            G = None
+            if j >= 1 and self.guess != "zero":
+                a1, d1, x1 = hist[-1]
+                if self.guess == "mlp" and j >= 2:
+                    a2, d2, x2 = hist[-2]
+                    G = self.mlp.predict_guess(self.op, d, p, d1, x1, d2, x2)
+                else:
+                    G = warm_start(x1, self.op.mesh.e_centro, self.k0, d, d1)
+            tb = time.time()
+            X, it, rr, cv = self.bt.cgs(B, guess=G, errmax=self.errmax, use_prec=False, maxit=self.maxit)
+            s = self.bt.rcs(X, p.astype(np.complex128), np.full(R, np.pi / 2), phi[ang])
+            real = np.arange(R) * nblocks + j < n
+            sigma[ang[real]] = s[real]
+            iters[ang[real]] = it[real]
+            conv[ang[real]] = cv[real]
+            if keep_currents:
+                currents[:, ang[real]] = X[:, real]
+            hist = (hist + [(ang, d, X)])[-2:]
+            blocks.append({"block": j, "seconds": time.time() - tb, "iterations_mean": float(it[real].mean()),
+                           "iterations_max": int(it[real].max()), "converged": bool(cv[real].all())})
+            self.log(f"sweep block {j + 1}/{nblocks} ({self.guess} guess): {blocks[-1]['seconds']:.2f}s, "
+                     f"iterations mean {blocks[-1]['iterations_mean']:.0f} max {blocks[-1]['iterations_max']}")
+        wall = time.time() - t0
+        return {"phi": phi, "sigma": sigma, "iterations": iters, "converged": conv, "seconds": wall,
+                "blocks": blocks, "currents": currents}

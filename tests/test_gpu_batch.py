@@ -11,14 +11,18 @@ pytestmark = pytest.mark.gpu
 _cache = {}
 
 
-def setup(R=16):
-    if R not in _cache:
-        print(f"[batch R={R}] building 7,680-unknown ellipsoid", flush=True)
-        mesh = lw.Mesh.icosphere(4, 1.6, 0.9, 0.7)
-        op = lw.MLFMA(mesh, lam=1.0, ladomax=0.2)
-        assert op.ilu0() == 0
-        _cache[R] = (mesh, op, lw.Batch(op, R))
-    return _cache[R]
+def setup(R=64, fp64=False):
+    key = (R, fp64)
+    if key not in _cache:
+        print(f"[batch R={R} far_fp64={fp64}] building 7,680-unknown ellipsoid", flush=True)
+        if "op" not in _cache:
+            mesh = lw.Mesh.icosphere(4, 1.6, 0.9, 0.7)
+            op = lw.MLFMA(mesh, lam=1.0, ladomax=0.2)
+            assert op.ilu0() == 0
+            _cache["op"] = (mesh, op)
+        mesh, op = _cache["op"]
+        _cache[key] = (mesh, op, lw.Batch(op, R, far_fp64=fp64))
+    return _cache[key]
 
 
 def rel(a, b):
@@ -31,7 +35,7 @@ def _need_device():
         pytest.skip("no HIP device")
 
 
-@pytest.mark.parametrize("R", [16, 64])
+@pytest.mark.parametrize("R", [64, 128])
 def test_batch_matvec_fp32_far_field(R):
     mesh, op, bt = setup(R)
     rng = np.random.default_rng(10)
@@ -47,33 +51,47 @@ def test_batch_matvec_fp32_far_field(R):
     assert np.all(np.isfinite(Y))
 
 
-def test_batch_matvec_against_oracle():
-    mesh, op, bt = setup(16)
-    ora = Oracle(mesh, lam=1.0, ladomax=0.2)
-    X = np.stack([lw.det_vector(mesh.ne) * (1 + 0.1j * r) for r in range(16)], axis=1)
+@pytest.mark.parametrize("R", [64, 128])
+def test_batch_matvec_fp64_far_field(R):
+    # the same kernels in FP64: only the summation order differs from the
+    # single-RHS product
+    mesh, op, bt = setup(R, fp64=True)
+    rng = np.random.default_rng(12)
+    X = rng.standard_normal((mesh.ne, R)) + 1j * rng.standard_normal((mesh.ne, R))
     Y = bt.matvec(X)
-    for r in (0, 7, 15):
+    for r in (0, R // 2, R - 1):
+        yf, yn = op.matvec_parts(X[:, r])
+        assert rel(Y[:, r], yf + yn) < 1e-12
+        assert rel(Y[:, r] - yn, yf) < 1e-11
+
+
+def test_batch_matvec_against_oracle():
+    mesh, op, bt = setup(64)
+    ora = Oracle(mesh, lam=1.0, ladomax=0.2)
+    X = np.stack([lw.det_vector(mesh.ne) * (1 + 0.1j * r) for r in range(64)], axis=1)
+    Y = bt.matvec(X)
+    for r in (0, 7, 63):
         assert rel(Y[:, r], ora.matvec(X[:, r])) < 2e-5
 
 
 def test_batch_lusol_matches_single():
-    mesh, op, bt = setup(16)
+    mesh, op, bt = setup(64)
     rng = np.random.default_rng(11)
-    Yv = rng.standard_normal((mesh.ne, 16)) + 1j * rng.standard_normal((mesh.ne, 16))
+    Yv = rng.standard_normal((mesh.ne, 64)) + 1j * rng.standard_normal((mesh.ne, 64))
     X = bt.lusol0(Yv)
-    for r in (0, 5, 15):
+    for r in (0, 5, 63):
         assert rel(X[:, r], op.lusol0(Yv[:, r])) < 1e-13
 
 
 def test_batch_plane_wave_and_rcs_match_single():
-    mesh, op, bt = setup(16)
-    ph = np.linspace(0, np.pi, 16, endpoint=False)
-    dirs = np.stack([-np.cos(ph), -np.sin(ph), np.zeros(16)], axis=1)
-    pols = np.tile(np.array([0, 0, 1.0], np.complex128), (16, 1))
+    mesh, op, bt = setup(64)
+    ph = np.linspace(0, np.pi, 64, endpoint=False)
+    dirs = np.stack([-np.cos(ph), -np.sin(ph), np.zeros(64)], axis=1)
+    pols = np.tile(np.array([0, 0, 1.0], np.complex128), (64, 1))
     B = bt.plane_wave(dirs, pols)
     for r in (0, 9):
         assert rel(B[:, r], op.plane_wave(dirs[r], pols[r])) < 1e-14
-    s = bt.rcs(B, pols, np.full(16, np.pi / 2), ph)
+    s = bt.rcs(B, pols, np.full(64, np.pi / 2), ph)
     for r in (0, 9):
         s1 = op.rcs(B[:, r], pols[r], np.pi / 2, ph[r])
         assert abs(s[r] - s1[0]) <= 1e-10 * s1[0]
@@ -82,51 +100,63 @@ def test_batch_plane_wave_and_rcs_match_single():
 _sphere = {}
 
 
-def sphere_batch():
-    if not _sphere:
-        mesh = lw.Mesh.icosphere(3, 0.5)
-        op = lw.MLFMA(mesh, lam=1.0, ladomax=0.2)
-        assert op.ilu0() == 0
-        _sphere["v"] = (mesh, op, lw.Batch(op, 16))
-    return _sphere["v"]
+def sphere_batch(fp64=False):
+    if fp64 not in _sphere:
+        if "op" not in _sphere:
+            mesh = lw.Mesh.icosphere(3, 0.5)
+            op = lw.MLFMA(mesh, lam=1.0, ladomax=0.2)
+            assert op.ilu0() == 0
+            _sphere["op"] = (mesh, op)
+        mesh, op = _sphere["op"]
+        _sphere[fp64] = (mesh, op, lw.Batch(op, 64, far_fp64=fp64))
+    return _sphere[fp64]
 
 
-def sweep(bt, R=16):
+def sweep(bt):
+    R = bt.R
     ph = np.linspace(0, np.pi, R, endpoint=False)
     dirs = np.stack([-np.cos(ph), -np.sin(ph), np.zeros(R)], axis=1)
     pols = np.tile(np.array([0, 0, 1.0], np.complex128), (R, 1))
     return ph, dirs, pols, bt.plane_wave(dirs, pols)
 
 
+@pytest.mark.parametrize("fp64", [False, True])
 @pytest.mark.parametrize("use_prec", [False, True])
-def test_batch_cgs_first_iterations_match_single(use_prec):
-    # FP32 far field: after a few iterations each column stays within the
-    # operator's FP32 error of the FP64 single-RHS iterate
-    mesh, op, bt = sphere_batch()
+def test_batch_cgs_first_iterations_match_single(use_prec, fp64):
+    # each column follows the single-RHS FP64 iterate: to the operator's FP32
+    # error with the FP32 far field, to rounding with the FP64 one
+    mesh, op, bt = sphere_batch(fp64)
     ph, dirs, pols, B = sweep(bt)
     X, it, rr, conv = bt.cgs(B, errmax=1e-12, use_prec=use_prec, maxit=3)
     assert (it == 3).all()
-    for r in (0, 5, 15):
+    for r in (0, 5, 63):
         x64, it64, rr64, c64 = op.cgs(B[:, r], errmax=1e-12, use_prec=use_prec, maxit=3)
-        assert rel(X[:, r], x64) < 1e-3, r
+        assert rel(X[:, r], x64) < (1e-10 if fp64 else 1e-3), r
 
 
-def test_batch_cgs_solves_every_column():
-    mesh, op, bt = sphere_batch()
+@pytest.mark.parametrize("fp64", [False, True])
+def test_batch_cgs_solves_every_column(fp64):
+    mesh, op, bt = sphere_batch(fp64)
     ph, dirs, pols, B = sweep(bt)
     X, it, rr, conv = bt.cgs(B, errmax=1e-4, use_prec=False, maxit=3000)
     print("iterations", it.tolist(), "relres", rr.max())
     assert conv.all() and (rr < 1e-4).all()
     # the true residual with the FP64 operator stays at the tolerance
-    for r in (0, 8, 15):
+    for r in (0, 8, 63):
         assert rel(op.matvec(X[:, r]), B[:, r]) < 3e-4
-    # against independent FP64 single-RHS solves: the FP32 far field perturbs
-    # CGS's recurrences (measured up to +35% iterations on this sphere), the
-    # converged currents give the RCS to 0.1 dB
-    for r in (0, 11):
+    # against independent FP64 single-RHS solves.  FP64 far field: the same
+    # iteration counts up to CGS's sensitivity to summation order.  FP32 far
+    # field: its rounding perturbs CGS's recurrences (measured up to 1.7x the
+    # iterations on this sphere); the converged currents give the RCS to 0.1 dB
+    for r in (0, 11, 40):
         x64, it64, rr64, c64 = op.cgs(B[:, r], errmax=1e-4, use_prec=False, maxit=3000)
         assert c64
-        assert int(it[r]) <= 1.6 * it64
-        s32 = op.rcs(X[:, r], pols[r], np.pi / 2, ph[r])[0]
+        print(f"column {r}: batched {int(it[r])} iterations, single FP64 {it64}")
+        if fp64:
+            # measured: within 10% (CGS amplifies the summation-order rounding)
+            assert abs(int(it[r]) - it64) <= max(5, it64 // 5)
+        else:
+            assert int(it[r]) <= 2.0 * it64
+        s_b = op.rcs(X[:, r], pols[r], np.pi / 2, ph[r])[0]
         s64 = op.rcs(x64, pols[r], np.pi / 2, ph[r])[0]
-        assert abs(10 * np.log10(s32 / s64)) < 0.1
+        assert abs(10 * np.log10(s_b / s64)) < (0.01 if fp64 else 0.1)
