@@ -2,18 +2,23 @@
 
 Workload (the single-GPU configuration the metric is quoted on): PEC sphere of
 radius 4 lambda (lambda = 1 m), icosphere subdivision 6 = 81,920 triangles =
-122,880 RWG unknowns, ladomax 0.25 -> 6-level MLFMA, EFIE, plane wave
-travelling along -z, x-polarised; ILU0 on the near field and CGS to relative
-residual 1e-4 (CGSold semantics, FP64 vectors, complex(4)-stored near field
-and ILU0 factor as the reference ships them).
+122,880 RWG unknowns, ladomax 0.25 -> 6-level MLFMA, EFIE.
 
-A step is one right-hand side solved from a zero initial guess.  Inputs are
-resident in HBM before the timed region (operators are built and the
-right-hand sides uploaded first).  Timed: K solves between device syncs.
+value: the plain drop-in path, one plane wave (x-polarised, travelling along
+-z) solved by CGSold semantics to relative residual 1e-4 from a zero guess:
+FP64 vectors, complex(4)-stored near field, no preconditioner (ILU0-CGS
+diverges on this mesh, as the reference's own algorithm does; the ILU0 path
+is reported by a short probe).  A step is one such solve; inputs are resident
+in HBM before the timed region.
 
 Also reported:
-  roofline: the dominant kernel of a solve, its algorithmic bytes per launch
-            over its mean duration (HIP events on the operator's stream);
+  batched: a monostatic sweep (VV, theta = 90 deg, phi in [0, pi)) solved as R
+           chains of consecutive angles per batch, each later block started
+           from the warm start or the learned guess, with RCS checks against
+           FP64 single solves;
+  fp32_far_field: the FP32-far-field operator's error and its CGS behaviour;
+  roofline: the dominant kernel of a plain solve, its algorithmic bytes per
+            launch over its mean duration (HIP events on the operator's stream);
   cpu_baseline: the C oracle (the reference's algorithm restated, 1 core) on
             a bounded sample of the same workload, scaled to RHS/s.
 
@@ -53,7 +58,14 @@ def main():
                     help="ILU0-CGS iterations run beside the timed solves to report the ILU0 path (0: skip)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
     ap.add_argument("--batch", type=int, default=64,
-                    help="right-hand sides per block of the multi-RHS FP32-far-field form (0: skip)")
+                    help="right-hand sides per block of the batched sweep (64 or 128; 0: skip)")
+    ap.add_argument("--batch-prec", choices=["fp64", "fp32"], default="fp64",
+                    help="far-field precision of the batched sweep")
+    ap.add_argument("--sweep", type=int, default=0, help="angles of the batched sweep (default: 3 blocks)")
+    ap.add_argument("--guess", choices=["zero", "warm", "mlp"], default="mlp",
+                    help="initial guess of the sweep's later blocks (mlp falls back to warm without weights)")
+    ap.add_argument("--fp32-probe-iters", type=int, default=300,
+                    help="batched CGS iterations of the FP32-far-field probe (0: skip)")
     ap.add_argument("--cpu-iters", type=int, default=3)
     ap.add_argument("--matvec-only", action="store_true",
                     help="time the single and batched matvecs only (for kernel profiles), no solves")
@@ -170,7 +182,10 @@ def main():
 
     batched = None
     if args.batch > 0:
-        batched = batched_sweep(lw, op, args, x, bs[-1], rhs_per_s)
+        batched = batched_sweep(lw, op, args, rhs_per_s)
+    probe = None
+    if args.fp32_probe_iters > 0:
+        probe = precision_probe(lw, op, args)
 
     cpu = None
     if not args.no_cpu:
@@ -212,43 +227,92 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "batched": batched,
+        "fp32_far_field": probe,
     }
     print(json.dumps(out))
 
 
-def batched_sweep(lw, op, args, x64, b64, plain_rhs_per_s):
-    """The multi-RHS form on the same mesh: a block of R monostatic incidence
-    angles (azimuth sweep in the xy-plane, VV), FP32 far field, FP64 near field
-    and CGS, solved together to the same 1e-4 tolerance.  Its RCS is compared
-    with FP64 single-RHS solves of the same angles."""
+def batched_sweep(lw, op, args, plain_rhs_per_s):
+    """The multi-RHS form on the same mesh: a monostatic sweep of
+    args.sweep angles (phi in [0, pi), theta = 90 deg, VV) solved as R chains
+    of consecutive angles in one batch (FP64 or FP32 far field, FP64 near field
+    and CGS) to the same tolerance, each block started from the chosen guess.
+    Its sigma is checked against FP64 single-RHS solves of two angles from
+    zero guesses, and its bistatic pattern at the first angle on 361 theta
+    points in the phi = 0 cut."""
+    from lywell_amd import guess_train
+    from lywell_amd.sweep import Sweep, polarisations, sweep_directions
     R = args.batch
-    bt = lw.Batch(op, R)
+    fp64 = args.batch_prec == "fp64"
+    bt = lw.Batch(op, R, far_fp64=fp64)
     mv_ms, _ = bt.bench(reps=3)
-    log(f"batched matvec (R={R}): {mv_ms:.3f} ms = {mv_ms / R * 1e3:.1f} us per RHS")
-    ph = np.pi * np.arange(R) / R
-    dirs = np.stack([-np.cos(ph), -np.sin(ph), np.zeros(R)], axis=1)
-    pols = np.tile(np.array([0, 0, 1.0], np.complex128), (R, 1))
-    B = bt.plane_wave(dirs, pols)
-    t0 = time.time()
-    X, it, rr, conv = bt.cgs(B, errmax=args.errmax, use_prec=False, maxit=args.maxit)
-    wall = time.time() - t0
-    log(f"batched CGS: {wall:.2f}s for {R} RHS, iterations {int(it.min())}-{int(it.max())}, all converged {conv.all()}")
-    sig = bt.rcs(X, pols, np.full(R, np.pi / 2), ph)
-    # FP64 single-RHS references for two of the angles
-    errs = []
-    for r in (0, R // 2):
-        x, it64, rr64, c64 = op.cgs(B[:, r], errmax=args.errmax, use_prec=False, maxit=args.maxit)
-        s64 = op.rcs(x, pols[r], np.pi / 2, ph[r])[0]
-        errs.append(abs(10 * np.log10(sig[r] / s64)))
-        log(f"  angle {r}: FP64 {it64} iterations, batched {int(it[r])}; RCS difference {errs[-1]:.4f} dB")
+    log(f"batched matvec (R={R}, far field {'FP64' if fp64 else 'FP32'}): {mv_ms:.3f} ms = "
+        f"{mv_ms / R * 1e3:.1f} us per RHS")
+    net = guess_train.load() if args.guess == "mlp" else None
+    guess = args.guess if (args.guess != "mlp" or net is not None) else "warm"
+    if guess != args.guess:
+        log("no trained guess network found: using the warm start")
+    n = args.sweep if args.sweep > 0 else 3 * R
+    sw = Sweep(op, bt, errmax=args.errmax, maxit=args.maxit, guess=guess, mlp=net, log=log)
+    res = sw.run(n, keep_currents=True)
+    value = n / res["seconds"]
+    # FP64 single-RHS references from zero guesses
+    phi, dirs = sweep_directions(n)
+    pols = polarisations(dirs, "VV").astype(np.complex128)
+    errs, ref = [], {}
+    for a in (0, n // 2):
+        b = op.plane_wave(dirs[a], pols[a])
+        x, it64, rr64, c64 = op.cgs(b, errmax=args.errmax, use_prec=False, maxit=args.maxit)
+        s64 = op.rcs(x, pols[a], np.pi / 2, phi[a])[0]
+        errs.append(abs(10 * np.log10(res["sigma"][a] / s64)))
+        ref[a] = x
+        log(f"  angle {a}: FP64 single solve {it64} iterations, batched {int(res['iterations'][a])}; "
+            f"monostatic sigma difference {errs[-1]:.4f} dB")
+    theta = np.linspace(0.0, np.pi, 361)
+    sb = op.rcs(res["currents"][:, 0], pols[0], theta, 0.0)
+    s1 = op.rcs(ref[0], pols[0], theta, 0.0)
+    bistatic = float(np.max(np.abs(10 * np.log10(sb / s1))))
+    log(f"  bistatic pattern at angle 0 (361 theta, phi = 0): max difference {bistatic:.4f} dB")
     bt.close()
-    value = R / wall
-    return {"nrhs": R, "rhs_per_s": round(value, 4), "speedup_vs_plain": round(value / plain_rhs_per_s, 2),
+    its = res["iterations"]
+    return {"rhs_per_s": round(value, 4), "speedup_vs_plain": round(value / plain_rhs_per_s, 2),
+            "angles": n, "nrhs_per_block": R, "blocks": res["blocks"], "seconds": round(res["seconds"], 2),
+            "guess": guess, "far_field": "FP64" if fp64 else "FP32",
             "matvec_ms": round(mv_ms, 3), "matvec_ms_per_rhs": round(mv_ms / R, 4),
-            "iterations_min": int(it.min()), "iterations_max": int(it.max()),
-            "converged": bool(conv.all()), "rcs_db_err_vs_fp64": [round(e, 5) for e in errs],
-            "precision": "FP32 far field (aggregation, interpolation, translation, receive), FP64 near field, "
-                         "ILU0 and CGS"}
+            "iterations_mean": float(its.mean()), "iterations_max": int(its.max()),
+            "converged": bool(res["converged"].all()),
+            "monostatic_rcs_db_err_vs_fp64": [round(e, 5) for e in errs],
+            "bistatic_rcs_db_err_vs_fp64_361": round(bistatic, 5),
+            "precision": f"{'FP64' if fp64 else 'FP32'} far field (aggregation, interpolation, translation, "
+                         "receive), FP64 near field and CGS"}
+
+
+def precision_probe(lw, op, args):
+    """FP32 far field against the FP64 operator on this mesh: matvec error and
+    the batched CGS's behaviour over a bounded number of iterations."""
+    R = 64
+    bt = lw.Batch(op, R, far_fp64=False)
+    mv_ms, _ = bt.bench(reps=3)
+    rng = np.random.default_rng(0)
+    X = rng.standard_normal((op.ne, R)) + 1j * rng.standard_normal((op.ne, R))
+    Y = bt.matvec(X)
+    tot, far = [], []
+    for r in (0, R // 2, R - 1):
+        yf, yn = op.matvec_parts(X[:, r])
+        tot.append(float(np.linalg.norm(Y[:, r] - yf - yn) / np.linalg.norm(yf + yn)))
+        far.append(float(np.linalg.norm(Y[:, r] - yn - yf) / np.linalg.norm(yf)))
+    from lywell_amd.sweep import polarisations, sweep_directions
+    phi, dirs = sweep_directions(R)
+    B = bt.plane_wave(dirs, polarisations(dirs, "VV").astype(np.complex128))
+    t0 = time.time()
+    _, it, rr, cv = bt.cgs(B, errmax=args.errmax, use_prec=False, maxit=args.fp32_probe_iters)
+    log(f"FP32 far field: matvec {mv_ms:.3f} ms (R={R}); relative error {max(tot):.2e} (far part {max(far):.2e}); "
+        f"CGS after {args.fp32_probe_iters} iterations: {int(cv.sum())} of {R} converged, "
+        f"median relres {np.median(rr):.2e} ({time.time() - t0:.1f}s)")
+    bt.close()
+    return {"matvec_ms": round(mv_ms, 3), "matvec_rel_err_vs_fp64": max(tot), "far_part_rel_err_vs_fp64": max(far),
+            "cgs_iterations": args.fp32_probe_iters, "cgs_converged": int(cv.sum()), "of": R,
+            "cgs_median_relres": float(np.median(rr)), "cgs_max_relres": float(np.max(rr))}
 
 
 def cpu_baseline(mesh, args, use_prec, gpu_iters):

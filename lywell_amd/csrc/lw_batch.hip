@@ -162,13 +162,16 @@ __global__ void __launch_bounds__(256) k_b_aggregate(int nk, int nb, const int* 
 }
 
 // Upward pass with the factored stencil (W = Wphi (x) Wtheta), one wave per
-// (parent P, coarse phi column ip, 64-column chunk):
+// (parent P, coarse phi column ip, block of kThBlock coarse theta rows,
+// 64-column chunk):
 //   C_{n-1}[(ip, it)][P] = sum_c ph(oct_c, (ip, it))
 //                          sum_a wphi[ip][a] sum_b wth[it][b] C_n[(fa, tb)][c]
-// The wave walks the coarse theta rows in order; the 3x3 window of fine
-// samples slides with the theta stencil, so most steps load 3 new values
-// (one per fine phi column) instead of 9.  Per-lane accumulators over the
-// coarse theta rows live in LDS.
+// The wave walks its theta rows in order; the 3x3 window of fine samples
+// slides with the theta stencil, so most steps load 3 new values (one per
+// fine phi column) instead of 9, and the values of the next step are fetched
+// before the current one is computed.  Accumulators stay in registers.
+constexpr int kThBlock = 8;
+
 template <typename S>
 __global__ void __launch_bounds__(256) k_b_interp_up(int ncol, int nthc, int nkc, int nbc, int nthf, int nbf,
                                                      int nphc, const int* __restrict__ child_ptr,
@@ -179,85 +182,89 @@ __global__ void __launch_bounds__(256) k_b_interp_up(int ncol, int nthc, int nkc
                                                      const typename Cx<S>::T* __restrict__ Cf,
                                                      typename Cx<S>::T* __restrict__ Cc) {
   typedef typename Cx<S>::T T2;
-  extern __shared__ unsigned char lds_raw[];
-  T2* accs = reinterpret_cast<T2*>(lds_raw);  // (waves, nthc, 64)
-  const int wpg = blockDim.x / kWave;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave), lane = threadIdx.x % kWave;
-  const int nchunk = ncol / kWave;
-  const int t = xcd_block(blockIdx.x, gridDim.x) * wpg + wave;
-  const int P = t / (nphc * nchunk);
+  const int nchunk = ncol / kWave, ntb = (nthc + kThBlock - 1) / kThBlock;
+  int t = xcd_block(blockIdx.x, gridDim.x) * (blockDim.x / kWave) + wave;
+  const int chunk = t % nchunk;
+  t /= nchunk;
+  const int tbk = t % ntb;
+  t /= ntb;
+  const int ip = t % nphc, P = t / nphc;
   if (P >= nbc) return;
-  const int ip = (t / nchunk) % nphc, col = (t % nchunk) * kWave + lane;
-  T2* acc = accs + (size_t)wave * nthc * kWave + lane;
-  for (int it = 0; it < nthc; ++it) acc[it * kWave] = mk<S>(0, 0);
+  const int col = chunk * kWave + lane;
+  const int it0 = tbk * kThBlock, nit = min(kThBlock, nthc - it0);
+  T2 acc[kThBlock];
+#pragma unroll
+  for (int u = 0; u < kThBlock; ++u) acc[u] = mk<S>(0, 0);
   const int f0 = sph_idx[3 * ip], f1 = sph_idx[3 * ip + 1], f2 = sph_idx[3 * ip + 2];
   const S w0 = sph_w[3 * ip], w1 = sph_w[3 * ip + 1], w2 = sph_w[3 * ip + 2];
   const size_t ks = (size_t)nbf * ncol;  // stride of one fine sample
   for (int c = child_ptr[P]; c < child_ptr[P + 1]; ++c) {
-    const T2* ph_c = ph + (size_t)oct[c] * nkc + (size_t)ip * nthc;
+    const T2* ph_c = ph + (size_t)oct[c] * nkc + (size_t)ip * nthc + it0;
     const T2* s0 = Cf + (size_t)f0 * nthf * ks + (size_t)c * ncol + col;
     const T2* s1 = Cf + (size_t)f1 * nthf * ks + (size_t)c * ncol + col;
     const T2* s2 = Cf + (size_t)f2 * nthf * ks + (size_t)c * ncol + col;
-    // window of step it in v (fine theta rows cur[]); the values of step
-    // it+1 are fetched before step it is computed, so that their loads
-    // overlap the arithmetic
     int cur[3];
     T2 v[3][3];
 #pragma unroll
     for (int b = 0; b < 3; ++b) {
-      const int tb = sth_idx[b];
+      const int tb = sth_idx[3 * it0 + b];
       cur[b] = tb;
       v[0][b] = s0[(size_t)tb * ks];
       v[1][b] = s1[(size_t)tb * ks];
       v[2][b] = s2[(size_t)tb * ks];
     }
-    for (int it = 0; it < nthc; ++it) {
-      int tn[3] = {cur[0], cur[1], cur[2]};
-      T2 nv[3][3];
-      if (it + 1 < nthc) {
 #pragma unroll
-        for (int b = 0; b < 3; ++b) {
-          const int tb = sth_idx[3 * (it + 1) + b];
-          tn[b] = tb;
-          if (tb == cur[0]) {
-            nv[0][b] = v[0][0]; nv[1][b] = v[1][0]; nv[2][b] = v[2][0];
-          } else if (tb == cur[1]) {
-            nv[0][b] = v[0][1]; nv[1][b] = v[1][1]; nv[2][b] = v[2][1];
-          } else if (tb == cur[2]) {
-            nv[0][b] = v[0][2]; nv[1][b] = v[1][2]; nv[2][b] = v[2][2];
-          } else {
-            nv[0][b] = s0[(size_t)tb * ks];
-            nv[1][b] = s1[(size_t)tb * ks];
-            nv[2][b] = s2[(size_t)tb * ks];
+    for (int u = 0; u < kThBlock; ++u) {
+      if (u < nit) {
+        const int it = it0 + u;
+        int tn[3] = {cur[0], cur[1], cur[2]};
+        T2 nv[3][3];
+        if (u + 1 < nit) {
+#pragma unroll
+          for (int b = 0; b < 3; ++b) {
+            const int tb = sth_idx[3 * (it + 1) + b];
+            tn[b] = tb;
+            if (tb == cur[0]) {
+              nv[0][b] = v[0][0]; nv[1][b] = v[1][0]; nv[2][b] = v[2][0];
+            } else if (tb == cur[1]) {
+              nv[0][b] = v[0][1]; nv[1][b] = v[1][1]; nv[2][b] = v[2][1];
+            } else if (tb == cur[2]) {
+              nv[0][b] = v[0][2]; nv[1][b] = v[1][2]; nv[2][b] = v[2][2];
+            } else {
+              nv[0][b] = s0[(size_t)tb * ks];
+              nv[1][b] = s1[(size_t)tb * ks];
+              nv[2][b] = s2[(size_t)tb * ks];
+            }
           }
         }
-      }
-      const S u0 = sth_w[3 * it], u1 = sth_w[3 * it + 1], u2 = sth_w[3 * it + 2];
-      S vx = 0, vy = 0;
+        const S u0 = sth_w[3 * it], u1 = sth_w[3 * it + 1], u2 = sth_w[3 * it + 2];
+        S vx = 0, vy = 0;
 #pragma unroll
-      for (int a = 0; a < 3; ++a) {
-        const S wa = a == 0 ? w0 : (a == 1 ? w1 : w2);
-        vx += wa * (u0 * v[a][0].x + u1 * v[a][1].x + u2 * v[a][2].x);
-        vy += wa * (u0 * v[a][0].y + u1 * v[a][1].y + u2 * v[a][2].y);
-      }
-      const T2 pz = ph_c[it];
-      T2 sa = acc[it * kWave];
-      sa.x += pz.x * vx - pz.y * vy;
-      sa.y += pz.x * vy + pz.y * vx;
-      acc[it * kWave] = sa;
-      if (it + 1 < nthc) {
+        for (int a = 0; a < 3; ++a) {
+          const S wa = a == 0 ? w0 : (a == 1 ? w1 : w2);
+          vx += wa * (u0 * v[a][0].x + u1 * v[a][1].x + u2 * v[a][2].x);
+          vy += wa * (u0 * v[a][0].y + u1 * v[a][1].y + u2 * v[a][2].y);
+        }
+        const T2 pz = ph_c[u];
+        acc[u].x += pz.x * vx - pz.y * vy;
+        acc[u].y += pz.x * vy + pz.y * vx;
+        if (u + 1 < nit) {
 #pragma unroll
-        for (int a = 0; a < 3; ++a)
+          for (int a = 0; a < 3; ++a)
 #pragma unroll
-          for (int b = 0; b < 3; ++b) v[a][b] = nv[a][b];
-        cur[0] = tn[0];
-        cur[1] = tn[1];
-        cur[2] = tn[2];
+            for (int b = 0; b < 3; ++b) v[a][b] = nv[a][b];
+          cur[0] = tn[0];
+          cur[1] = tn[1];
+          cur[2] = tn[2];
+        }
       }
     }
   }
-  T2* out = Cc + ((size_t)ip * nthc * nbc + P) * ncol + col;
-  for (int it = 0; it < nthc; ++it) out[(size_t)it * nbc * ncol] = acc[it * kWave];
+  T2* out = Cc + ((size_t)(ip * nthc + it0) * nbc + P) * ncol + col;
+#pragma unroll
+  for (int u = 0; u < kThBlock; ++u)
+    if (u < nit) out[(size_t)u * nbc * ncol] = acc[u];
 }
 
 // Translation fused with the downward pass, one wave per (sample k, parent
@@ -275,40 +282,25 @@ __global__ void __launch_bounds__(256) k_b_interp_up(int ncol, int nthc, int nkc
 // where the k-slice of C_n and the Tk row stay in L2.
 constexpr int kOffCodes = 343;  // (dx, dy, dz) in [-3, 3]^3
 
-template <typename S>
-__global__ void __launch_bounds__(256) k_b_far(int nk, int nb, int nP, int ncol, int nchunk, int wgs_per_k,
-                                               const int* __restrict__ pc_ptr, const int* __restrict__ bo,
-                                               const int* __restrict__ q_ptr, const int2* __restrict__ q_ent,
-                                               const typename Cx<S>::T* __restrict__ Tk,
-                                               const typename Cx<S>::T* __restrict__ C, int nbc, int nwt,
-                                               const int* __restrict__ wt_ptr, const int* __restrict__ wt_row,
-                                               const typename Cx<S>::T* __restrict__ wp,
-                                               const typename Cx<S>::T* __restrict__ Ac,
-                                               typename Cx<S>::T* __restrict__ A) {
+// The work of one wave for a parent with NC children (acc[NC] in registers).
+template <typename S, int NC>
+__device__ __forceinline__ void far_wave(int P, int k, int nb, int ncol, int col, int c0,
+                                         const int* __restrict__ pc_ptr, const int* __restrict__ bo,
+                                         const int* __restrict__ q_ptr, const int2* __restrict__ q_ent,
+                                         const typename Cx<S>::T* tks, const typename Cx<S>::T* __restrict__ C,
+                                         int nbc, int nwt, const int* __restrict__ wt_ptr,
+                                         const int* __restrict__ wt_row, const typename Cx<S>::T* __restrict__ wp,
+                                         const typename Cx<S>::T* __restrict__ Ac, typename Cx<S>::T* __restrict__ A) {
   typedef typename Cx<S>::T T2;
-  const int g = blockIdx.x;
-  const int k = (g / 8 / wgs_per_k) * 8 + g % 8;
-  if (k >= nk) return;
-  const int t = ((g / 8) % wgs_per_k) * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
-  const int P = t / nchunk;
-  if (P >= nP) return;
-  const int col = (t % nchunk) * kWave + (threadIdx.x % kWave);
-  const int c0 = pc_ptr[P], nc = pc_ptr[P + 1] - c0;
-  int oc[8];
+  int oc[NC], cq[NC];
+  T2 acc[NC];
 #pragma unroll
-  for (int q = 0; q < 8; ++q) oc[q] = q < nc ? bo[c0 + q] : 0;
-  T2 acc[8];
-#pragma unroll
-  for (int q = 0; q < 8; ++q) acc[q] = mk<S>(0, 0);
-  const T2* Tkk = Tk + (size_t)k * kOffCodes;
+  for (int q = 0; q < NC; ++q) {
+    oc[q] = bo[c0 + q];
+    cq[q] = (oc[q] >> 2) * 49 + ((oc[q] >> 1) & 1) * 7 + (oc[q] & 1);
+    acc[q] = mk<S>(0, 0);
+  }
   const T2* Ck = C + (size_t)k * nb * ncol + col;
-  // offset code of (target child q, source child d of Q) = (2*o + bits(d) -
-  // bits(c_q) + 3) in base 7 = base(o) + cd(d) - cq[q]; near offsets and
-  // absent children need no test: Tk is zero at every code without a
-  // transfer vector and rows of absent children are never stored
-  int cq[8];
-#pragma unroll
-  for (int q = 0; q < 8; ++q) cq[q] = (oc[q] >> 2) * 49 + ((oc[q] >> 1) & 1) * 7 + (oc[q] & 1);
   for (int e = q_ptr[P]; e < q_ptr[P + 1]; ++e) {
     const int2 qe = q_ent[e];
     const int ox = qe.y / 9 - 1, oy = (qe.y / 3) % 3 - 1, oz = qe.y % 3 - 1;
@@ -326,8 +318,8 @@ __global__ void __launch_bounds__(256) k_b_far(int nk, int nb, int nP, int ncol,
     for (int jd = 0; jd < 8; ++jd) {
       if (jd < nd) {
 #pragma unroll
-        for (int q = 0; q < 8; ++q) {
-          const T2 tt = Tkk[cd[jd] - cq[q]];
+        for (int q = 0; q < NC; ++q) {
+          const T2 tt = tks[cd[jd] - cq[q]];
           acc[q].x += tt.x * v[jd].x - tt.y * v[jd].y;
           acc[q].y += tt.x * v[jd].y + tt.y * v[jd].x;
         }
@@ -344,7 +336,7 @@ __global__ void __launch_bounds__(256) k_b_far(int nk, int nb, int nP, int ncol,
       for (int u = 0; u < 4; ++u) {
         if (e + u < e1) {
 #pragma unroll
-          for (int q = 0; q < 8; ++q) {
+          for (int q = 0; q < NC; ++q) {
             const T2 w = wp[(size_t)oc[q] * nwt + e + u];
             acc[q].x += w.x * a[u].x - w.y * a[u].y;
             acc[q].y += w.x * a[u].y + w.y * a[u].x;
@@ -355,8 +347,53 @@ __global__ void __launch_bounds__(256) k_b_far(int nk, int nb, int nP, int ncol,
   }
   T2* Ak = A + (size_t)k * nb * ncol + col;
 #pragma unroll
-  for (int q = 0; q < 8; ++q)
-    if (q < nc) Ak[(size_t)(c0 + q) * ncol] = acc[q];
+  for (int q = 0; q < NC; ++q) Ak[(size_t)(c0 + q) * ncol] = acc[q];
+}
+
+template <typename S>
+__global__ void __launch_bounds__(256) k_b_far(int nk, int nb, int nP, int ncol, int nchunk, int wgs_per_k,
+                                               const int* __restrict__ pc_ptr, const int* __restrict__ bo,
+                                               const int* __restrict__ q_ptr, const int2* __restrict__ q_ent,
+                                               const typename Cx<S>::T* __restrict__ Tk,
+                                               const typename Cx<S>::T* __restrict__ C, int nbc, int nwt,
+                                               const int* __restrict__ wt_ptr, const int* __restrict__ wt_row,
+                                               const typename Cx<S>::T* __restrict__ wp,
+                                               const typename Cx<S>::T* __restrict__ Ac,
+                                               typename Cx<S>::T* __restrict__ A) {
+  typedef typename Cx<S>::T T2;
+  __shared__ T2 tks[kOffCodes];  // this sample's transfer functions, read as broadcasts
+  const int g = blockIdx.x;
+  const int k = (g / 8 / wgs_per_k) * 8 + g % 8;
+  if (k >= nk) return;  // uniform over the workgroup
+  for (int e = threadIdx.x; e < kOffCodes; e += blockDim.x) tks[e] = Tk[(size_t)k * kOffCodes + e];
+  __syncthreads();
+  // within a sample, all parents of one 64-column chunk before the next
+  // chunk: the chunk's slice of C_n (nb x 64 columns) is what the XCD's L2
+  // has to hold
+  const int t = ((g / 8) % wgs_per_k) * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+  const int nPr = wgs_per_k * 4 / nchunk;  // parents per chunk, rounded up to whole workgroups
+  const int P = t % nPr;
+  if (P >= nP) return;
+  const int col = (t / nPr) * kWave + (threadIdx.x % kWave);
+  const int c0 = pc_ptr[P], nc = pc_ptr[P + 1] - c0;
+#define LW_FAR_CASE(NC)                                                                                      \
+  case NC:                                                                                                   \
+    far_wave<S, NC>(P, k, nb, ncol, col, c0, pc_ptr, bo, q_ptr, q_ent, tks, C, nbc, nwt, wt_ptr, wt_row, wp, \
+                    Ac, A);                                                                                  \
+    break;
+  switch (nc) {
+    LW_FAR_CASE(1)
+    LW_FAR_CASE(2)
+    LW_FAR_CASE(3)
+    LW_FAR_CASE(4)
+    LW_FAR_CASE(5)
+    LW_FAR_CASE(6)
+    LW_FAR_CASE(7)
+    LW_FAR_CASE(8)
+    default:
+      break;
+  }
+#undef LW_FAR_CASE
 }
 
 // Tk[k][code] = pes[k] T[tr(code)][k] (zero for codes without a transfer vector)
@@ -459,59 +496,73 @@ __global__ void __launch_bounds__(128) k_b_receive(int nk, int nb, const int* __
 // Near field plus the scaled far field, one workgroup per leaf box.  Every
 // row of a box has the same near columns (the bases of the box's near boxes,
 // ascending), so the box's block of the near matrix is a dense (nf x ncol)
-// matrix applied to ncol rows of X: a wave takes 16 rows and 64 right-hand
-// sides and streams the columns once, each X row feeding 16 complex FP64
-// FMAs.  The matrix entries are wave-uniform (scalar loads).  Row sums run
-// in the CSR column order, as in the single-RHS product.
-constexpr int kNearRows = 16;
-
+// matrix applied to ncol rows of X: a complex GEMM on FP64 MFMA.  A wave owns
+// 16 of the box's rows and 64 right-hand sides; each lane loads 4
+// consecutive matrix entries of its row at once, so within a block of 16
+// columns MFMA step q pairs lane group g with column 4g + q, and the X rows
+// are fetched with the same assignment.  The stored values (complex(4) or
+// complex(8)) are widened to FP64; accumulation is FP64.
 template <typename V, typename F>
-__global__ void __launch_bounds__(256) k_b_near_box(int R, const int* __restrict__ box_slot,
+__global__ void __launch_bounds__(128) k_b_near_box(int R, const int* __restrict__ box_slot,
                                                     const int* __restrict__ slot_basis,
                                                     const int* __restrict__ col_ptr, const int* __restrict__ cols,
                                                     const int* __restrict__ ia, const V* __restrict__ z,
                                                     const double2* __restrict__ X, const F* __restrict__ Yf,
                                                     double2 cc, double2* __restrict__ Y) {
+  constexpr int NT = 4;
   const int b = xcd_block(blockIdx.x, gridDim.x);
   const int s0 = box_slot[b], nf = box_slot[b + 1] - s0;
   const int j0 = col_ptr[b], ncol = col_ptr[b + 1] - j0;
-  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave), lane = threadIdx.x % kWave;
-  const int nchunk = R / kWave, ngroups = (nf + kNearRows - 1) / kNearRows;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave), l = threadIdx.x % kWave;
+  const int li = l % 16, lk = l / 16;
+  const int nch = R / 64, nst = (nf + 15) / 16;
   const int* cb = cols + j0;
-  for (int task = wave; task < ngroups * nchunk; task += blockDim.x / kWave) {
-    const int g = task / nchunk, r = (task % nchunk) * kWave + lane;
-    const int nr = min(kNearRows, nf - g * kNearRows);
-    int zr[kNearRows];
+  for (int task = wave; task < nst * nch; task += blockDim.x / kWave) {
+    const int st = task / nch, c0 = (task % nch) * 64;
+    const int sl = st * 16 + li;  // matrix row this lane loads
+    const bool row_ok = sl < nf;
+    const V* zrow = z + ia[slot_basis[s0 + min(sl, nf - 1)]];
+    f64x4v dr[NT], di[NT];
 #pragma unroll
-    for (int q = 0; q < kNearRows; ++q) zr[q] = q < nr ? ia[slot_basis[s0 + g * kNearRows + q]] : 0;
-    double2 acc[kNearRows];
+    for (int jt = 0; jt < NT; ++jt) {
+      dr[jt] = f64x4v{0, 0, 0, 0};
+      di[jt] = f64x4v{0, 0, 0, 0};
+    }
+    for (int jb = 0; jb < ncol; jb += 16) {
+      const int jl = jb + 4 * lk;
+      double2 a4[4];
+      const double2* xp[4];
 #pragma unroll
-    for (int q = 0; q < kNearRows; ++q) acc[q] = make_double2(0.0, 0.0);
-    constexpr int U = 4;  // columns per step: their X loads are in flight together
-    for (int j = 0; j < ncol; j += U) {
-      double2 x[U];
+      for (int q = 0; q < 4; ++q) {
+        const bool ok = jl + q < ncol;
+        a4[q] = (row_ok && ok) ? to_d2(zrow[jl + q]) : make_double2(0.0, 0.0);
+        xp[q] = ok ? X + (size_t)cb[jl + q] * R + c0 + li : nullptr;
+      }
 #pragma unroll
-      for (int u = 0; u < U; ++u) x[u] = j + u < ncol ? X[(size_t)cb[j + u] * R + r] : make_double2(0.0, 0.0);
+      for (int q = 0; q < 4; ++q) {
+        double2 x[NT];
 #pragma unroll
-      for (int u = 0; u < U; ++u) {
-        if (j + u < ncol) {
+        for (int jt = 0; jt < NT; ++jt) x[jt] = xp[q] ? xp[q][jt * 16] : make_double2(0.0, 0.0);
 #pragma unroll
-          for (int q = 0; q < kNearRows; ++q) {
-            if (q < nr) {
-              const double2 zz = to_d2(z[zr[q] + j + u]);
-              acc[q].x += zz.x * x[u].x - zz.y * x[u].y;
-              acc[q].y += zz.x * x[u].y + zz.y * x[u].x;
-            }
-          }
+        for (int jt = 0; jt < NT; ++jt) {
+          dr[jt] = mma(a4[q].x, x[jt].x, dr[jt]);
+          dr[jt] = mma(-a4[q].y, x[jt].y, dr[jt]);
+          di[jt] = mma(a4[q].x, x[jt].y, di[jt]);
+          di[jt] = mma(a4[q].y, x[jt].x, di[jt]);
         }
       }
     }
 #pragma unroll
-    for (int q = 0; q < kNearRows; ++q) {
-      if (q < nr) {
-        const size_t at = (size_t)slot_basis[s0 + g * kNearRows + q] * R + r;
-        const F f = Yf[at];
-        Y[at] = dadd(dmul(make_double2((double)f.x, (double)f.y), cc), acc[q]);
+    for (int v = 0; v < 4; ++v) {
+      const int so = st * 16 + mma_row<double>(lk, v);
+      if (so < nf) {
+        const size_t at0 = (size_t)slot_basis[s0 + so] * R + c0 + li;
+#pragma unroll
+        for (int jt = 0; jt < NT; ++jt) {
+          const F f = Yf[at0 + jt * 16];
+          const double2 fs = dmul(make_double2((double)f.x, (double)f.y), cc);
+          Y[at0 + jt * 16] = make_double2(fs.x + dr[jt][v], fs.y + di[jt][v]);
+        }
       }
     }
   }
@@ -1011,17 +1062,15 @@ static int batch_matvec_t(Batch* bt, const double2* X, double2* Y) {
     BatchLevel& p = bt->lv[n - 1];
     const LevelPlan& pp = h->plan.lv[n - 1];
     const LevelPlan& cp = h->plan.lv[n];
-    const int wpg = std::max(1, std::min(4, 65536 / (pp.nth * kWave * (int)sizeof(T2))));
-    const long long waves = (long long)p.nb * pp.nph * nchunk;
-    hipLaunchKernelGGL(k_b_interp_up<S>, dim3(nblk(waves, wpg)), dim3(wpg * kWave),
-                       (size_t)wpg * pp.nth * kWave * sizeof(T2), st, ncol, pp.nth, p.nk, p.nb, cp.nth, c.nb, pp.nph,
-                       p.pcu_ptr, c.oct, p.sth_idx, (const S*)p.sth_w, p.sph_idx, (const S*)p.sph_w,
+    const long long waves = (long long)p.nb * pp.nph * ((pp.nth + kThBlock - 1) / kThBlock) * nchunk;
+    hipLaunchKernelGGL(k_b_interp_up<S>, dim3(nblk(waves, 4)), dim3(256), 0, st, ncol, pp.nth, p.nk, p.nb, cp.nth,
+                       c.nb, pp.nph, p.pcu_ptr, c.oct, p.sth_idx, (const S*)p.sth_w, p.sph_idx, (const S*)p.sph_w,
                        (const T2*)p.ph, (const T2*)c.C, (T2*)p.C);
   }
   for (int n = 3; n <= L; ++n) {
     BatchLevel& d = bt->lv[n];
     const BatchLevel* pl = n > 3 ? &bt->lv[n - 1] : nullptr;
-    const int wgs_per_k = nblk((long long)d.nP * nchunk, 4);
+    const int wgs_per_k = nblk(d.nP, 4) * nchunk;  // each chunk covers the parents in whole workgroups
     const int grid = 8 * nblk(d.nk, 8) * wgs_per_k;
     hipLaunchKernelGGL(k_b_far<S>, dim3(grid), dim3(256), 0, st, d.nk, d.nb, d.nP, ncol, nchunk, wgs_per_k, d.pc_ptr,
                        d.bo, d.q_ptr, d.q_ent, (const T2*)d.Tk, (const T2*)d.C, pl ? pl->nb : 0, d.nwt,

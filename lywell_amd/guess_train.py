@@ -1,13 +1,16 @@
 """Training of the learned initial guess on solutions the solver generates.
 
-    python -m lywell_amd.guess_train [--meshes 12] [--subdiv 4] [--angles 64]
+    python -m lywell_amd.guess_train [--meshes 12] [--subdiv 3,4] [--angles 64]
                                      [--steps 3000] [--out lywell_amd/guess_mlp.npz]
 
 Training meshes are seeded random bodies: ellipsoids with semi-axes uniform
 in [amin, amax] wavelengths and spheres perturbed by real spherical
-harmonics of degree 2-4 (5% total amplitude), icosphere subdivision
-`subdiv`.  For each mesh a monostatic sweep of `angles` incidence angles
-(VV on even meshes, HH on odd ones) is solved in one batch to `errmax`; every
+harmonics of degree 2-4 (5% total amplitude), icosphere subdivisions cycled
+from `subdiv` (1,920 and 7,680 RWG by default).  For each mesh a monostatic
+sweep of `angles` incidence angles (VV on even meshes, HH on odd ones) is
+solved in one batch with the FP64 far field to `errmax`; a body whose sweep
+does not converge within `maxit` unpreconditioned CGS iterations is replaced
+by a new draw.  Every
 angle a >= 2 of the sweep becomes one training sample per unknown, with the
 converged currents of angles a-1 and a-2 as inputs and that of angle a as
 the target.  The network (16 -> 256 x 3 -> 4) is trained by the library's
@@ -62,7 +65,8 @@ def solve_sweep(op, n, pol, errmax, maxit, R=64):
         B = bt.plane_wave(dirs[ang], pols[ang].astype(np.complex128))
         Xb, it, rr, cv = bt.cgs(B, errmax=errmax, use_prec=False, maxit=maxit)
         if not cv.all():
-            raise RuntimeError("training sweep did not converge")
+            bt.close()
+            return None
         keep = np.arange(a0, a0 + R) < n
         X[:, ang[keep]] = Xb[:, keep]
         its[ang[keep]] = it[keep]
@@ -95,13 +99,13 @@ def guess_errors(net, op, dirs, pols, X):
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--meshes", type=int, default=12)
-    ap.add_argument("--subdiv", type=int, default=4)
+    ap.add_argument("--subdiv", default="3,4", help="icosphere subdivisions, cycled over the meshes")
     ap.add_argument("--amin", type=float, default=0.5)
     ap.add_argument("--amax", type=float, default=2.0)
     ap.add_argument("--ladomax", type=float, default=0.2)
     ap.add_argument("--angles", type=int, default=64)
-    ap.add_argument("--errmax", type=float, default=1e-6)
-    ap.add_argument("--maxit", type=int, default=6000)
+    ap.add_argument("--errmax", type=float, default=1e-5)
+    ap.add_argument("--maxit", type=int, default=4000)
     ap.add_argument("--steps", type=int, default=3000)
     ap.add_argument("--batch-rows", type=int, default=16384)
     ap.add_argument("--lr", type=float, default=1e-3)
@@ -112,14 +116,25 @@ def main(argv=None):
         raise SystemExit("guess_train: no HIP device")
     rng = np.random.default_rng(args.seed)
     net = lw.Mlp(16, 256, 3, 4, seed=args.seed)
-    bodies = []
+    bodies, skipped = [], 0
+    subdivs = [int(v) for v in str(args.subdiv).split(",")]
     t0 = time.time()
-    for i in range(args.meshes + 1):  # the last mesh is held out
+    i = 0
+    while i <= args.meshes:  # the last mesh is held out
         kind = "ellipsoid" if i % 2 == 0 else "perturbed_sphere"
-        mesh, meta = random_body(rng, args.subdiv, args.amin, args.amax, kind)
+        mesh, meta = random_body(rng, subdivs[i % len(subdivs)], args.amin, args.amax, kind)
         op = lw.MLFMA(mesh, lam=1.0, ladomax=args.ladomax)
         pol = "VV" if i % 2 == 0 else "HH"
-        phi, dirs, pols, X, its = solve_sweep(op, args.angles, pol, args.errmax, args.maxit)
+        sol = solve_sweep(op, args.angles, pol, args.errmax, args.maxit)
+        if sol is None:
+            # unpreconditioned CGS stalls on some bodies: draw another one
+            log(f"mesh {i}: {kind} {mesh.ne} RWG, {pol}: sweep not converged in {args.maxit} iterations, redrawn")
+            op.close()
+            skipped += 1
+            if skipped > 2 * args.meshes:
+                raise RuntimeError("too many training sweeps failed to converge")
+            continue
+        phi, dirs, pols, X, its = sol
         meta.update({"ne": mesh.ne, "levels": op.nlevels, "pol": pol, "iterations_mean": float(its.mean())})
         if i < args.meshes:
             samples(net, op, dirs, pols, X)
@@ -129,6 +144,7 @@ def main(argv=None):
             op.close()
         else:
             held = (op, dirs, pols, X, meta)
+        i += 1
     loss = net.train(args.steps, batch_rows=args.batch_rows, lr=args.lr, seed=args.seed)
     log(f"trained {args.steps} steps: loss {loss[:10].mean():.4e} -> {loss[-50:].mean():.4e}")
     op, dirs, pols, X, meta = held
@@ -137,7 +153,7 @@ def main(argv=None):
     log("held-out median relative error of the guess:", json.dumps(ev))
     np.savez(args.out, params=net.get_params(), arch=np.array([16, 256, 3, 4]),
              loss=loss, meta=json.dumps({"train_meshes": bodies, "held_out": meta, "held_out_error": ev,
-                                         "args": vars(args)}))
+                                         "redrawn": skipped, "args": vars(args)}))
     log(f"wrote {args.out}")
     return ev
 
