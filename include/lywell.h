@@ -190,6 +190,20 @@ int lw_batch_lusol0(lw_batch *b, const double *y, double *x);
  * (its later updates are masked); iters/relres/converged are per column. */
 int lw_batch_cgs(lw_batch *b, const double *rhs, const double *guess /* NULL: zero */, double errmax,
                  int use_prec, int maxit, double *x, int *iters, double *relres, int *converged);
+/* Persistent form of lw_batch_cgs for sweeps whose columns run independent
+ * chains of problems: lw_batch_cgs_start sets every column up (initial
+ * residuals); lw_batch_cgs_advance runs up to `iterations` iterations (fewer
+ * once no column is active) and returns per column state(4,nrhs) = (active,
+ * first, converged, iterations) and relres(nrhs); lw_batch_column copies out
+ * the current iterate of one column; lw_batch_restart_column gives column col
+ * a new right-hand side and guess (NULL: zero), its initial residual taken
+ * with the single-RHS FP64 operator, while the other columns keep iterating
+ * in later calls. */
+int lw_batch_cgs_start(lw_batch *b, const double *rhs, const double *guess, double errmax, int use_prec,
+                       int maxit);
+int lw_batch_cgs_advance(lw_batch *b, int iterations, int *state, double *relres);
+int lw_batch_column(lw_batch *b, int col, double *x);
+int lw_batch_restart_column(lw_batch *b, int col, const double *rhs, const double *guess);
 /* generar_onda per column: dirs (3,nrhs) real, pols (3,nrhs) complex(8). */
 int lw_batch_plane_wave(lw_batch *b, const double *dirs, const double *pols, const double *ctte, double *rhs);
 /* sigmaRCS per column at one (theta(r), phi(r)) each: the monostatic sweep. */

@@ -353,6 +353,33 @@ class Batch:
                                   _P(it), _P(rr), _P(cv)))
         return X, it, rr, cv.astype(bool)
 
+    # -- persistent solves: columns restarted one at a time
+    def cgs_start(self, B, guess=None, errmax=1e-4, use_prec=False, maxit=10000):
+        """Initial residuals of every column; iterate with advance()."""
+        B = self._mat(B)
+        G = None if guess is None else self._mat(guess)
+        _check(lib().lw_batch_cgs_start(self.b, _P(B), _P(G), _C.c_double(errmax), int(bool(use_prec)), int(maxit)))
+
+    def advance(self, iterations):
+        """Up to `iterations` CGS iterations (fewer when every column has
+        stopped).  Returns per-column (active, first, converged, iterations)
+        as an (R, 4) int array and the relative residuals."""
+        st = _np.zeros((self.R, 4), _np.int32)
+        rr = _np.zeros(self.R)
+        _check(lib().lw_batch_cgs_advance(self.b, int(iterations), _P(st), _P(rr)))
+        return st, rr
+
+    def column(self, c):
+        x = _np.zeros(self.ne, _np.complex128)
+        _check(lib().lw_batch_column(self.b, int(c), _P(x)))
+        return x
+
+    def restart(self, c, b, guess=None):
+        """Give column c a new right-hand side (and initial guess)."""
+        b = self.op._vec(b)
+        g = None if guess is None else self.op._vec(guess)
+        _check(lib().lw_batch_restart_column(self.b, int(c), _P(b), _P(g)))
+
     def plane_wave(self, dirs, pols, ctte=1.0):
         d = _np.ascontiguousarray(dirs, _np.float64).reshape(self.R, 3)
         p = _np.ascontiguousarray(pols, _np.complex128).reshape(self.R, 3)

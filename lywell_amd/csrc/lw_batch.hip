@@ -762,6 +762,57 @@ __global__ void k_b_cgs_check(int R, double errmax, const double2* __restrict__ 
   rho1[c] = rhon[c];
 }
 
+// Restart of column c (persistent sweeps): b and x of the new problem come in
+// contiguous (n); tmp holds Z x.  Sets b, x, r = R~ = b - Z x in the batch
+// layout, then (one workgroup) |b|, rho1 = r.R~ and a fresh state.
+__global__ void k_b_col_set(int R, int n, int c, const double2* __restrict__ bc, const double2* __restrict__ xc,
+                            const double2* __restrict__ zx, double2* __restrict__ b, double2* __restrict__ x,
+                            double2* __restrict__ r, double2* __restrict__ rt) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const size_t at = (size_t)i * R + c;
+  const double2 ri = dsub(bc[i], zx[i]);
+  b[at] = bc[i];
+  x[at] = xc[i];
+  r[at] = ri;
+  rt[at] = ri;
+}
+
+__global__ void __launch_bounds__(256) k_b_col_init(int R, int n, int c, const double2* __restrict__ bc,
+                                                    const double2* __restrict__ zx, double* __restrict__ nbv,
+                                                    double* __restrict__ rr, int4* __restrict__ stv,
+                                                    double2* __restrict__ rho1) {
+  __shared__ double sh[3][256];
+  double nb = 0.0, qr = 0.0, qi = 0.0;
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    const double2 bi = bc[i], ri = dsub(bc[i], zx[i]);
+    nb += bi.x * bi.x + bi.y * bi.y;
+    qr += ri.x * ri.x - ri.y * ri.y;  // unconjugated r . r
+    qi += 2.0 * ri.x * ri.y;
+  }
+  sh[0][threadIdx.x] = nb;
+  sh[1][threadIdx.x] = qr;
+  sh[2][threadIdx.x] = qi;
+  __syncthreads();
+  for (int o = blockDim.x / 2; o > 0; o >>= 1) {
+    if ((int)threadIdx.x < o)
+      for (int q = 0; q < 3; ++q) sh[q][threadIdx.x] += sh[q][threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    nbv[c] = sqrt(sh[0][0]);
+    rr[c] = 0.0;
+    const bool brk = sh[1][0] == 0.0 && sh[2][0] == 0.0;
+    stv[c] = make_int4(brk ? 0 : 1, 1, 0, 0);
+    rho1[c] = make_double2(sh[1][0], sh[2][0]);
+  }
+}
+
+__global__ void k_b_col_get(int R, int n, int c, const double2* __restrict__ x, double2* __restrict__ xc) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) xc[i] = x[(size_t)i * R + c];
+}
+
 // ---------------------------------------------------------------- excitation and RCS per column
 
 __global__ void k_b_plane_wave(int R, int ne, const double* __restrict__ tb, const double* __restrict__ pc,
@@ -884,6 +935,8 @@ struct Batch {
   hipGraphExec_t iter_graph[2] = {nullptr, nullptr};  // one CGS iteration, without / with ILU0
   double graph_errmax[2] = {0.0, 0.0};
   int graph_maxit[2] = {0, 0};
+  hipGraphExec_t cur_graph = nullptr;  // the graph of the solve in progress
+  long long done = 0, next_report = 0;  // iterations replayed since the start
   std::vector<void*> allocs;
 };
 
@@ -1127,6 +1180,8 @@ static int batch_dot2(Batch* bt, const double2* a, const double2* b, int ca, con
   return LW_OK;
 }
 
+constexpr int kGraphGroup = 8;
+
 // One CGSold iteration for every active column (modIterativo.f90:376-486),
 // enqueued on the handle's stream; it is captured once into a graph.
 static int batch_cgs_iteration(Batch* bt, bool use_prec, double errmax, int maxit) {
@@ -1166,9 +1221,9 @@ static int batch_cgs_iteration(Batch* bt, bool use_prec, double errmax, int maxi
 // CGS on every column from b = w[0] and the initial guess in X: the initial
 // residual, then captured iterations replayed in groups of kGraphGroup with
 // one host check of the column states per group.
-constexpr int kGraphGroup = 8;
-
-static int batch_cgs_run(Batch* bt, double errmax, bool use_prec, int maxit) {
+// Initial residual of every column from b = w[0] and the guess in X, and
+// the iteration graph for (use_prec, errmax, maxit).
+static int batch_cgs_start(Batch* bt, double errmax, bool use_prec, int maxit) {
   Handle* h = bt->h;
   const int R = bt->R;
   const long long nv = (long long)h->ne * R;
@@ -1205,13 +1260,30 @@ static int batch_cgs_run(Batch* bt, double errmax, bool use_prec, int maxit) {
     bt->graph_errmax[use_prec] = errmax;
     bt->graph_maxit[use_prec] = maxit;
   }
-  std::vector<int4> sv(R);
-  std::vector<double> rr(R);
-  int done = 0, next_report = progress_every();
-  while (done < maxit) {
-    const int g = std::min(kGraphGroup, maxit - done);
-    for (int j = 0; j < g; ++j) LW_HIP(hipGraphLaunch(gx, st));
-    done += g;
+  bt->cur_graph = gx;
+  bt->done = 0;
+  return LW_OK;
+}
+
+// Up to `iterations` replays of the iteration graph, in groups of
+// kGraphGroup with one host check of the column states per group; stops
+// early when no column is active.  sv/rr receive the states.
+static int batch_cgs_advance(Batch* bt, int iterations, std::vector<int4>& sv, std::vector<double>& rr) {
+  Handle* h = bt->h;
+  const int R = bt->R;
+  hipStream_t st = h->stream;
+  if (!bt->cur_graph) {
+    set_error("lw_batch: no solve started");
+    return LW_ERR_STATE;
+  }
+  sv.resize(R);
+  rr.resize(R);
+  int left = iterations;
+  while (left > 0) {
+    const int g = std::min(kGraphGroup, left);
+    for (int j = 0; j < g; ++j) LW_HIP(hipGraphLaunch(bt->cur_graph, st));
+    left -= g;
+    bt->done += g;
     LW_HIP(hipMemcpyAsync(sv.data(), bt->stv, sizeof(int4) * R, hipMemcpyDeviceToHost, st));
     LW_HIP(hipMemcpyAsync(rr.data(), bt->rrv, sizeof(double) * R, hipMemcpyDeviceToHost, st));
     LW_HIP(hipStreamSynchronize(st));
@@ -1222,14 +1294,24 @@ static int batch_cgs_run(Batch* bt, double errmax, bool use_prec, int maxit) {
         ++na;
         worst = std::max(worst, rr[c]);
       }
-    if (next_report > 0 && done >= next_report) {
-      std::fprintf(stderr, "[lw_batch_cgs] iteration %d: %d of %d columns active, worst relres %.3e\n", done, na,
-                   R, worst);
-      next_report += progress_every();
+    if (progress_every() > 0 && bt->done >= bt->next_report) {
+      std::fprintf(stderr, "[lw_batch_cgs] iteration %lld: %d of %d columns active, worst relres %.3e\n",
+                   bt->done, na, R, worst);
+      bt->next_report += progress_every();
     }
     if (!na) break;
   }
   return LW_OK;
+}
+
+// CGS on every column from b = w[0] and the initial guess in X.
+static int batch_cgs_run(Batch* bt, double errmax, bool use_prec, int maxit) {
+  int rc = batch_cgs_start(bt, errmax, use_prec, maxit);
+  if (rc) return rc;
+  bt->next_report = progress_every();
+  std::vector<int4> sv;
+  std::vector<double> rr;
+  return batch_cgs_advance(bt, maxit, sv, rr);
 }
 
 // Far-field tables of every level in the precision S (patterns, phases,
@@ -1464,6 +1546,94 @@ int lw_batch_cgs(lw_batch* ob, const double* Bv, const double* guess, double err
     if (relres) relres[c] = rr[c];
     if (converged) converged[c] = sv[c].z;
   }
+  return LW_OK;
+}
+
+int lw_batch_cgs_start(lw_batch* ob, const double* Bv, const double* guess, double errmax, int use_prec,
+                       int maxit) {
+  using namespace lw;
+  if (!ob || !Bv || maxit < 1) return LW_ERR_ARG;
+  Batch* bt = &ob->b;
+  Handle* h = bt->h;
+  if (use_prec && !h->have_ilu) {
+    set_error("lw_batch_cgs_start: preconditioner requested but no ILU0 factor");
+    return LW_ERR_STATE;
+  }
+  const size_t bytes = sizeof(double2) * (size_t)h->ne * bt->R;
+  LW_HIP(hipMemcpy(bt->w[0], Bv, bytes, hipMemcpyHostToDevice));
+  if (guess)
+    LW_HIP(hipMemcpy(bt->X, guess, bytes, hipMemcpyHostToDevice));
+  else
+    LW_HIP(hipMemset(bt->X, 0, bytes));
+  int rc = batch_cgs_start(bt, errmax, use_prec != 0, maxit);
+  bt->next_report = progress_every();
+  return rc;
+}
+
+int lw_batch_cgs_advance(lw_batch* ob, int iterations, int* state, double* relres) {
+  using namespace lw;
+  if (!ob || iterations < 0) return LW_ERR_ARG;
+  Batch* bt = &ob->b;
+  std::vector<int4> sv;
+  std::vector<double> rr;
+  int rc = batch_cgs_advance(bt, iterations, sv, rr);
+  if (rc) return rc;
+  if (iterations == 0) {
+    sv.resize(bt->R);
+    rr.resize(bt->R);
+    LW_HIP(hipMemcpy(sv.data(), bt->stv, sizeof(int4) * bt->R, hipMemcpyDeviceToHost));
+    LW_HIP(hipMemcpy(rr.data(), bt->rrv, sizeof(double) * bt->R, hipMemcpyDeviceToHost));
+  }
+  for (int c = 0; c < bt->R; ++c) {
+    if (state) {
+      state[4 * c] = sv[c].x;
+      state[4 * c + 1] = sv[c].y;
+      state[4 * c + 2] = sv[c].z;
+      state[4 * c + 3] = sv[c].w;
+    }
+    if (relres) relres[c] = rr[c];
+  }
+  return LW_OK;
+}
+
+int lw_batch_column(lw_batch* ob, int col, double* x) {
+  using namespace lw;
+  if (!ob || !x || col < 0 || col >= ob->b.R) return LW_ERR_ARG;
+  Batch* bt = &ob->b;
+  Handle* h = bt->h;
+  hipLaunchKernelGGL(k_b_col_get, dim3(nblk(h->ne, 256)), dim3(256), 0, h->stream, bt->R, h->ne, col, bt->X,
+                     h->d_x);
+  LW_HIP(hipGetLastError());
+  LW_HIP(hipMemcpyAsync(x, h->d_x, sizeof(double2) * h->ne, hipMemcpyDeviceToHost, h->stream));
+  LW_HIP(hipStreamSynchronize(h->stream));
+  return LW_OK;
+}
+
+int lw_batch_restart_column(lw_batch* ob, int col, const double* b, const double* guess) {
+  using namespace lw;
+  if (!ob || !b || col < 0 || col >= ob->b.R) return LW_ERR_ARG;
+  Batch* bt = &ob->b;
+  Handle* h = bt->h;
+  if (!bt->cur_graph) {
+    set_error("lw_batch_restart_column: no solve started");
+    return LW_ERR_STATE;
+  }
+  const size_t bytes = sizeof(double2) * h->ne;
+  hipStream_t st = h->stream;
+  // the handle's single-RHS work vectors: x (guess), y (Z x), w[0] (b)
+  LW_HIP(hipMemcpyAsync(h->d_w[0], b, bytes, hipMemcpyHostToDevice, st));
+  if (guess)
+    LW_HIP(hipMemcpyAsync(h->d_x, guess, bytes, hipMemcpyHostToDevice, st));
+  else
+    LW_HIP(hipMemsetAsync(h->d_x, 0, bytes, st));
+  int rc = run_matvec(h, h->d_x, h->d_y, nullptr, nullptr);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_b_col_set, dim3(nblk(h->ne, 256)), dim3(256), 0, st, bt->R, h->ne, col, h->d_w[0], h->d_x,
+                     h->d_y, bt->w[0], bt->X, bt->w[1], bt->w[2]);
+  hipLaunchKernelGGL(k_b_col_init, dim3(1), dim3(256), 0, st, bt->R, h->ne, col, h->d_w[0], h->d_y, bt->nbv,
+                     bt->rrv, bt->stv, bt->sc);
+  LW_HIP(hipGetLastError());
+  LW_HIP(hipStreamSynchronize(st));
   return LW_OK;
 }
 

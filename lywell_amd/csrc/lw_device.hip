@@ -558,7 +558,9 @@ int blocks_for(long long n, int b) { return (int)((n + b - 1) / b); }
 constexpr int kDownLdsBytes = 64 * 1024;
 
 // one full matvec on device vectors x -> y (y_near: optional split output)
-int run_matvec(Handle* h, const double2* x, double2* y, double2* y_near, hipEvent_t* ev = nullptr) {
+}  // namespace
+
+int run_matvec(Handle* h, const double2* x, double2* y, double2* y_near, hipEvent_t* ev) {
   const int L = h->L;
   hipStream_t st = h->stream;
   DevLevel& lf = h->lv[L];
@@ -607,6 +609,8 @@ int run_matvec(Handle* h, const double2* x, double2* y, double2* y_near, hipEven
   LW_HIP(hipGetLastError());
   return LW_OK;
 }
+
+namespace {
 
 int run_lusol(Handle* h, const double2* y, double2* x) {
   if (!h->have_ilu) {
@@ -871,7 +875,7 @@ int lw_mlfma_level_info(lw_handle* hh, int level, int* nboxes, int* nmult, int* 
 
 int lw_matvec_dev(lw_handle* hh, const void* x_dev, void* y_dev) {
   if (lw::check_handle(hh) || !x_dev || !y_dev) return LW_ERR_ARG;
-  return lw::run_matvec(&hh->h, (const double2*)x_dev, (double2*)y_dev, nullptr);
+  return lw::run_matvec(&hh->h, (const double2*)x_dev, (double2*)y_dev, nullptr, nullptr);
 }
 
 int lw_matvec(lw_handle* hh, const double* x, double* y) {
@@ -879,7 +883,7 @@ int lw_matvec(lw_handle* hh, const double* x, double* y) {
   Handle* h = &hh->h;
   const size_t bytes = sizeof(double2) * h->ne;
   LW_HIP(hipMemcpyAsync(h->d_x, x, bytes, hipMemcpyHostToDevice, h->stream));
-  int rc = lw::run_matvec(h, h->d_x, h->d_y, nullptr);
+  int rc = lw::run_matvec(h, h->d_x, h->d_y, nullptr, nullptr);
   if (rc) return rc;
   LW_HIP(hipMemcpyAsync(y, h->d_y, bytes, hipMemcpyDeviceToHost, h->stream));
   LW_HIP(hipStreamSynchronize(h->stream));
@@ -891,7 +895,7 @@ int lw_matvec_parts(lw_handle* hh, const double* x, double* y_far, double* y_nea
   Handle* h = &hh->h;
   const size_t bytes = sizeof(double2) * h->ne;
   LW_HIP(hipMemcpyAsync(h->d_x, x, bytes, hipMemcpyHostToDevice, h->stream));
-  int rc = lw::run_matvec(h, h->d_x, h->d_y, h->d_w[0]);
+  int rc = lw::run_matvec(h, h->d_x, h->d_y, h->d_w[0], nullptr);
   if (rc) return rc;
   LW_HIP(hipMemcpyAsync(y_far, h->d_y, bytes, hipMemcpyDeviceToHost, h->stream));
   LW_HIP(hipMemcpyAsync(y_near, h->d_w[0], bytes, hipMemcpyDeviceToHost, h->stream));
@@ -1102,7 +1106,7 @@ int lw_cgs(lw_handle* hh, const double* b, const double* guess, double errmax, i
     LW_HIP(hipMemcpyAsync(xv, guess, bytes, hipMemcpyHostToDevice, st));
   else
     LW_HIP(hipMemsetAsync(xv, 0, bytes, st));
-  int rc = lw::run_matvec(h, xv, t, nullptr);
+  int rc = lw::run_matvec(h, xv, t, nullptr, nullptr);
   if (rc) return rc;
   const int vb = lw::blocks_for(n, 256);
   hipLaunchKernelGGL(lw::k_sub, dim3(vb), dim3(256), 0, st, n, d_b, t, r);
@@ -1127,7 +1131,7 @@ int lw_cgs(lw_handle* hh, const double* b, const double* guess, double errmax, i
     } else {
       LW_HIP(hipMemcpyAsync(P_, p, bytes, hipMemcpyDeviceToDevice, st));
     }
-    if ((rc = lw::run_matvec(h, P_, V, nullptr))) return rc;
+    if ((rc = lw::run_matvec(h, P_, V, nullptr, nullptr))) return rc;
     if ((rc = lw::run_dot2(h, R_, V, 0, nullptr, nullptr, 0, s + 2))) return rc;
     hipLaunchKernelGGL(lw::k_cgs_q, dim3(vb), dim3(256), 0, st, n, s + 0, s + 2, s + 3, u, V, q, t);
     if (use_prec) {
@@ -1135,7 +1139,7 @@ int lw_cgs(lw_handle* hh, const double* b, const double* guess, double errmax, i
     } else {
       LW_HIP(hipMemcpyAsync(U, t, bytes, hipMemcpyDeviceToDevice, st));
     }
-    if ((rc = lw::run_matvec(h, U, Q, nullptr))) return rc;
+    if ((rc = lw::run_matvec(h, U, Q, nullptr, nullptr))) return rc;
     hipLaunchKernelGGL(lw::k_cgs_xr, dim3(vb), dim3(256), 0, st, n, s + 3, U, Q, xv, r);
     // |r|^2 and the next rho = sum(r*R_)
     if ((rc = lw::run_dot2(h, r, r, 1, r, R_, 0, s + 4))) return rc;
@@ -1272,7 +1276,7 @@ int lw_bench_matvec(lw_handle* hh, int reps, double* ms, double* stage_ms) {
   std::vector<double2> xin(h->ne);
   for (int i = 0; i < h->ne; ++i) xin[i] = make_double2((i + 1) / 1.3823, 57.382 / (i + 1));
   LW_HIP(hipMemcpyAsync(h->d_x, xin.data(), sizeof(double2) * h->ne, hipMemcpyHostToDevice, h->stream));
-  int rc = lw::run_matvec(h, h->d_x, h->d_y, nullptr);
+  int rc = lw::run_matvec(h, h->d_x, h->d_y, nullptr, nullptr);
   if (rc) return rc;
   LW_HIP(hipStreamSynchronize(h->stream));
   double acc[5] = {0, 0, 0, 0, 0}, tot = 0.0;

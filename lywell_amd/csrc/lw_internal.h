@@ -138,6 +138,10 @@ int launch_setup_transfer(Handle* h, int level, const double* d_binom2, const do
                           const int* d_tL, const double* d_th, int hstride);
 int launch_setup_near(Handle* h);
 
+// One single-RHS matvec (MatxVec_MLFMA) on device vectors, on the handle's
+// stream (lw_device.hip).
+int run_matvec(Handle* h, const double2* x, double2* y, double2* y_near, hipEvent_t* ev);
+
 }  // namespace lw
 
 // The opaque handle of the C ABI.

@@ -40,7 +40,14 @@ def warm_start(x_prev, e_centro, k0, d, d_prev):
 
 
 class Sweep:
-    """n angles through a batch of R columns (lywell_amd.Batch)."""
+    """n angles through a batch of R columns (lywell_amd.Batch).
+
+    run(): blocks of R angles, each block solved to completion before the
+    next starts (the slowest column of a block sets its length).
+    run_persistent(): every column works through its own chain of
+    consecutive angles and is restarted with the next angle of its chain as
+    soon as it has converged, while the other columns keep iterating
+    (lw_batch_restart_column), so no column waits for the block's slowest."""
 
     def __init__(self, op, batch, errmax=1e-4, maxit=20000, guess="warm", mlp=None, pol="VV", log=None):
         if guess not in ("zero", "warm", "mlp"):
@@ -94,3 +101,65 @@ class Sweep:
         wall = time.time() - t0
         return {"phi": phi, "sigma": sigma, "iterations": iters, "converged": conv, "seconds": wall,
                 "blocks": blocks, "currents": currents}
+
+    def _guess(self, hist, d, p):
+        """Initial guess for direction d from the chain's last two solutions."""
+        if self.guess == "zero" or not hist:
+            return None
+        d1, x1 = hist[-1]
+        if self.guess == "mlp" and len(hist) >= 2:
+            d2, x2 = hist[-2]
+            return self.mlp.predict_guess(self.op, d, p, d1, x1, d2, x2)[:, 0]
+        return warm_start(x1[:, None], self.op.mesh.e_centro, self.k0, d[None, :], d1[None, :])[:, 0]
+
+    def run_persistent(self, n, keep_currents=False, check_every=8):
+        R = self.bt.R
+        m = (n + R - 1) // R
+        chains = [list(range(c * m, min((c + 1) * m, n))) for c in range(R)]
+        phi, dirs = sweep_directions(n)
+        pols = polarisations(dirs, self.pol)
+        cp = pols.astype(np.complex128)
+        sigma = np.zeros(n)
+        iters = np.zeros(n, np.int64)
+        conv = np.zeros(n, bool)
+        currents = np.zeros((self.op.ne, n), np.complex128) if keep_currents else None
+        first = np.array([ch[0] if ch else n - 1 for ch in chains])
+        t0 = time.time()
+        B = self.bt.plane_wave(dirs[first], cp[first])
+        self.bt.cgs_start(B, errmax=self.errmax, maxit=self.maxit)
+        pos = [0] * R
+        hist = [[] for _ in range(R)]
+        done = [len(ch) == 0 for ch in chains]
+        solved = 0
+        restarts = 0
+        while not all(done):
+            st, rr = self.bt.advance(check_every)
+            for c in range(R):
+                if done[c] or st[c, 0]:
+                    continue
+                a = chains[c][pos[c]]
+                x = self.bt.column(c)
+                iters[a] = st[c, 3]
+                conv[a] = bool(st[c, 2])
+                sigma[a] = self.op.rcs(x, cp[a], np.pi / 2, phi[a])[0]
+                if keep_currents:
+                    currents[:, a] = x
+                hist[c] = (hist[c] + [(dirs[a], x)])[-2:]
+                solved += 1
+                pos[c] += 1
+                if pos[c] < len(chains[c]):
+                    a2 = chains[c][pos[c]]
+                    b = self.op.plane_wave(dirs[a2], cp[a2])
+                    self.bt.restart(c, b, self._guess(hist[c], dirs[a2], pols[a2]))
+                    restarts += 1
+                else:
+                    done[c] = True
+            if solved and solved % R == 0:
+                self.log(f"sweep: {solved}/{n} angles solved, {time.time() - t0:.1f}s")
+        wall = time.time() - t0
+        self.log(f"sweep ({self.guess} guess, persistent): {n} angles in {wall:.2f}s, "
+                 f"mean {iters.mean():.0f} iterations per angle")
+        return {"phi": phi, "sigma": sigma, "iterations": iters, "converged": conv, "seconds": wall,
+                "restarts": restarts, "currents": currents,
+                "blocks": [{"chain_length": m, "columns": R}]}
+

@@ -159,3 +159,35 @@ def test_batch_cgs_solves_every_column():
         s_b = op.rcs(X[:, r], pols[r], np.pi / 2, ph[r])[0]
         s64 = op.rcs(x64, pols[r], np.pi / 2, ph[r])[0]
         assert abs(10 * np.log10(s_b / s64)) < 0.01
+
+
+def test_batch_restart_column():
+    # persistent form: a converged column takes a new problem while the
+    # others keep their solutions
+    mesh, op, bt = sphere_batch(True)
+    ph, dirs, pols, B = sweep(bt)
+    bt.cgs_start(B, errmax=1e-4, maxit=3000)
+    st, rr = bt.advance(3000)
+    assert (st[:, 0] == 0).all() and (st[:, 2] == 1).all()
+    X0 = np.stack([bt.column(c) for c in range(bt.R)], axis=1)
+    d_new = np.array([-np.cos(0.123), -np.sin(0.123), 0.0])
+    p_new = np.array([0, 0, 1.0], np.complex128)
+    b_new = op.plane_wave(d_new, p_new)
+    guess = X0[:, 5] * np.exp(-1j * 2 * np.pi * (mesh.e_centro.reshape(-1, 3) @ (d_new - dirs[5])))
+    bt.restart(5, b_new, guess)
+    st2, rr2 = bt.advance(0)
+    assert st2[5, 0] == 1 and st2[5, 3] == 0 and st2[5, 2] == 0
+    st2, rr2 = bt.advance(3000)
+    assert st2[5, 2] == 1 and rr2[5] < 1e-4
+    # the warm start saved iterations over the column's first solve from zero
+    print("restarted column: iterations", st2[5, 3], "from zero", st[5, 3])
+    assert st2[5, 3] < st[5, 3]
+    x5 = bt.column(5)
+    assert rel(op.matvec(x5), b_new) < 3e-4
+    x64, it64, rr64, c64 = op.cgs(b_new, errmax=1e-4, use_prec=False, maxit=3000)
+    s_b = op.rcs(x5, p_new, np.pi / 2, 0.123)[0]
+    s64 = op.rcs(x64, p_new, np.pi / 2, 0.123)[0]
+    assert abs(10 * np.log10(s_b / s64)) < 0.01
+    for c in (0, 4, 6, 63):
+        assert np.array_equal(bt.column(c), X0[:, c])
+        assert st2[c, 2] == 1 and st2[c, 3] == st[c, 3]
