@@ -58,12 +58,13 @@ def main():
                     help="ILU0-CGS iterations run beside the timed solves to report the ILU0 path (0: skip)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
     ap.add_argument("--batch", type=int, default=64,
-                    help="right-hand sides per block of the batched sweep (64 or 128; 0: skip)")
+                    help="columns of the batched sweep (64 or 128; -1: autotuned; 0: skip the sweep)")
+    ap.add_argument("--chain", type=int, default=3, help="angles per column chain of the sweep")
     ap.add_argument("--batch-prec", choices=["fp64", "fp32"], default="fp64",
                     help="far-field precision of the batched sweep")
-    ap.add_argument("--sweep", type=int, default=0, help="angles of the batched sweep (default: 3 blocks)")
+    ap.add_argument("--sweep", type=int, default=0, help="angles of the batched sweep (default: chain x columns)")
     ap.add_argument("--guess", choices=["zero", "warm", "mlp"], default="mlp",
-                    help="initial guess of the sweep's later blocks (mlp falls back to warm without weights)")
+                    help="initial guess of each later angle of a chain (mlp falls back to warm without weights)")
     ap.add_argument("--fp32-probe-iters", type=int, default=300,
                     help="batched CGS iterations of the FP32-far-field probe (0: skip)")
     ap.add_argument("--cpu-iters", type=int, default=3)
@@ -115,6 +116,8 @@ def main():
     log(f"matvec {mv_ms:.3f} ms (stages {np.round(stages, 3).tolist()}), lusol {lus_ms:.3f} ms")
 
     if args.matvec_only:
+        cms, cgbs = op.bench_copy(512 << 20, reps=5)
+        log(f"device copy of 512 MiB: {cms:.3f} ms = {cgbs:.0f} GB/s (read + write)")
         if args.batch > 0:
             for fp64 in (False, True):
                 bt = lw.Batch(op, args.batch, far_fp64=fp64)
@@ -177,11 +180,15 @@ def main():
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                 "algorithmic_bytes_per_launch": alg / nl, "mean_launch_ms": round(dom_ms / nl, 4),
                 "launches_per_matvec": nl}
+    copy_ms, copy_gbs = op.bench_copy(1 << 30, reps=5)
+    roofline["measured_copy_GBps"] = round(copy_gbs, 1)
+    roofline["frac_of_measured_copy"] = round(achieved / copy_gbs, 4)
+    log(f"device copy 1 GiB: {copy_gbs:.0f} GB/s; dominant kernel {kernel_of[dom]} at {achieved:.0f} GB/s")
     stage_gbs = {n: round(float(b) / (float(t) * 1e-3) / 1e9, 1) for n, b, t in zip(stage_names, sbytes, stages)}
     matvec_gbs = bytes_["total"] / (mv_ms * 1e-3) / 1e9
 
     batched = None
-    if args.batch > 0:
+    if args.batch != 0:
         batched = batched_sweep(lw, op, args, rhs_per_s)
     probe = None
     if args.fp32_probe_iters > 0:
@@ -232,30 +239,57 @@ def main():
     print(json.dumps(out))
 
 
+def plain_sweep(op, args, n, chain):
+    """The plain drop-in on the sweep, as the reference runs it
+    (gen_rcs_monostatica, main.f90:995-1057): angles of one chain solved one
+    after another, each from the previous angle's current times the phase
+    correction (zero guess for the first)."""
+    from lywell_amd.sweep import polarisations, sweep_directions, warm_start
+    phi, dirs = sweep_directions(n)
+    pols = polarisations(dirs, "VV").astype(np.complex128)
+    k0 = 2 * np.pi / op.lam
+    x_prev, d_prev, its = None, None, []
+    t0 = time.time()
+    for a in chain:
+        b = op.plane_wave(dirs[a], pols[a])
+        g = None if x_prev is None else warm_start(x_prev[:, None], op.mesh.e_centro, k0, dirs[a][None, :],
+                                                      d_prev[None, :])[:, 0]
+        x, it, rr, cv = op.cgs(b, guess=g, errmax=args.errmax, use_prec=False, maxit=args.maxit)
+        its.append(int(it))
+        x_prev, d_prev = x, dirs[a]
+    wall = time.time() - t0
+    log(f"plain drop-in on sweep angles {list(chain)}: iterations {its}, {wall:.2f}s")
+    return len(chain) / wall, its
+
+
 def batched_sweep(lw, op, args, plain_rhs_per_s):
-    """The multi-RHS form on the same mesh: a monostatic sweep of
-    args.sweep angles (phi in [0, pi), theta = 90 deg, VV) solved as R chains
-    of consecutive angles in one batch (FP64 or FP32 far field, FP64 near field
-    and CGS) to the same tolerance, each block started from the chosen guess.
-    Its sigma is checked against FP64 single-RHS solves of two angles from
-    zero guesses, and its bistatic pattern at the first angle on 361 theta
-    points in the phi = 0 cut."""
-    from lywell_amd import guess_train
+    """The chosen form on the same mesh: a monostatic sweep of args.sweep
+    angles (phi in [0, pi), theta = 90 deg, VV) solved as R chains of
+    consecutive angles in one persistent batch (FP64 far field by default,
+    FP64 near field and CGS) to the same tolerance; each chain restarts its
+    column with the next angle as soon as one converges, from the learned
+    guess (or the warm start).  Its sigma is checked against FP64 single-RHS
+    solves of two angles from zero guesses, and its bistatic pattern at the
+    first angle on 361 theta points in the phi = 0 cut."""
+    from lywell_amd import autotune, guess_train
     from lywell_amd.sweep import Sweep, polarisations, sweep_directions
-    R = args.batch
     fp64 = args.batch_prec == "fp64"
+    best, times = autotune.tune(op, far_fp64=fp64, log=log)
+    R = args.batch if args.batch in (64, 128) else best
     bt = lw.Batch(op, R, far_fp64=fp64)
-    mv_ms, _ = bt.bench(reps=3)
+    mv_ms = times.get(R) or bt.bench(reps=3)[0]
     log(f"batched matvec (R={R}, far field {'FP64' if fp64 else 'FP32'}): {mv_ms:.3f} ms = "
         f"{mv_ms / R * 1e3:.1f} us per RHS")
     net = guess_train.load() if args.guess == "mlp" else None
     guess = args.guess if (args.guess != "mlp" or net is not None) else "warm"
     if guess != args.guess:
         log("no trained guess network found: using the warm start")
-    n = args.sweep if args.sweep > 0 else 3 * R
+    n = args.sweep if args.sweep > 0 else args.chain * R
     sw = Sweep(op, bt, errmax=args.errmax, maxit=args.maxit, guess=guess, mlp=net, log=log)
-    res = sw.run(n, keep_currents=True)
+    res = sw.run_persistent(n, keep_currents=True)
     value = n / res["seconds"]
+    m = (n + R - 1) // R
+    plain_sweep_rhs_s, plain_its = plain_sweep(op, args, n, range(0, min(m, n)))
     # FP64 single-RHS references from zero guesses
     phi, dirs = sweep_directions(n)
     pols = polarisations(dirs, "VV").astype(np.complex128)
@@ -275,11 +309,16 @@ def batched_sweep(lw, op, args, plain_rhs_per_s):
     log(f"  bistatic pattern at angle 0 (361 theta, phi = 0): max difference {bistatic:.4f} dB")
     bt.close()
     its = res["iterations"]
-    return {"rhs_per_s": round(value, 4), "speedup_vs_plain": round(value / plain_rhs_per_s, 2),
-            "angles": n, "nrhs_per_block": R, "blocks": res["blocks"], "seconds": round(res["seconds"], 2),
-            "guess": guess, "far_field": "FP64" if fp64 else "FP32",
+    pos = np.arange(n) % m  # position of each angle in its chain
+    by_pos = {int(q): float(its[pos == q].mean()) for q in range(m)}
+    return {"rhs_per_s": round(value, 4), "speedup_vs_plain_sweep": round(value / plain_sweep_rhs_s, 2),
+            "speedup_vs_plain_single": round(value / plain_rhs_per_s, 2),
+            "plain_sweep_rhs_per_s": round(plain_sweep_rhs_s, 4), "plain_sweep_iterations": plain_its,
+            "angles": n, "nrhs_per_batch": R, "chain_length": m, "seconds": round(res["seconds"], 2),
+            "guess": guess, "far_field": "FP64" if fp64 else "FP32", "autotune_ms": times,
             "matvec_ms": round(mv_ms, 3), "matvec_ms_per_rhs": round(mv_ms / R, 4),
             "iterations_mean": float(its.mean()), "iterations_max": int(its.max()),
+            "iterations_mean_by_chain_position": by_pos, "restarts": res["restarts"],
             "converged": bool(res["converged"].all()),
             "monostatic_rcs_db_err_vs_fp64": [round(e, 5) for e in errs],
             "bistatic_rcs_db_err_vs_fp64_361": round(bistatic, 5),

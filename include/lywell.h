@@ -156,6 +156,10 @@ int lw_bench_lusol(lw_handle *h, int reps, double *ms);
  * (pattern arrays twice, campo arrays 6x, transfer tables once, near CSR,
  * x and y), and of the dominant kernels. */
 int lw_matvec_bytes(lw_handle *h, double *total, double *aggregation, double *receive_near);
+/* Mean time in ms of a device-to-device copy of `bytes` bytes (a multiple of
+ * 16) by the library's copy kernel on the handle's stream: the measured
+ * bandwidth reference of the roofline figures. */
+int lw_bench_copy(lw_handle *h, long long bytes, int reps, double *ms);
 /* Algorithmic bytes of each matvec stage, in the order of lw_bench_matvec's
  * stage_ms: aggregation, upward passes, translations, downward passes,
  * receive + near field. */

@@ -276,6 +276,12 @@ class MLFMA:
         _check(lib().lw_rcs_sigma(self.h, th.size, _P(th), _P(ph), _P(s)))
         return s
 
+    def bench_copy(self, nbytes=1 << 30, reps=5):
+        """(ms per copy, GB/s counting read + write) of the library's device copy kernel."""
+        ms = _C.c_double()
+        _check(lib().lw_bench_copy(self.h, _C.c_longlong(nbytes), int(reps), _C.byref(ms)))
+        return ms.value, 2 * nbytes / (ms.value * 1e-3) / 1e9
+
     def bench_matvec(self, reps=10):
         ms = _C.c_double()
         st = _np.zeros(5)

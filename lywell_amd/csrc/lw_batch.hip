@@ -97,9 +97,11 @@ __global__ void k_to_prec(long long n, const double2* __restrict__ in, typename 
 
 // Leaf aggregation on MFMA: per leaf box the complex GEMM
 //   C_L[m][b][r] = sum_s rad(s; m) x(basis(s), r),  m = p*nk + k,
-// over the box's nf bases.  Workgroup (box, group of 4 item tiles); a wave
-// owns 16 items and all R columns.  The box's x rows go through LDS in
-// chunks of KC bases (row stride R + 8 against bank conflicts).
+// over the box's nf bases.  One workgroup per box; its 4 waves take the
+// 16-item tiles in rounds, each wave owning 16 items and all R columns.  The
+// box's x rows go through LDS (row stride R + 8 against bank conflicts),
+// once per box when they fit (nf <= KC), else in chunks of KC bases per tile
+// round.
 template <typename S, int R, int KC>
 __global__ void __launch_bounds__(256) k_b_aggregate(int nk, int nb, const int* __restrict__ box_slot,
                                                      const int* __restrict__ slot_basis,
@@ -111,20 +113,12 @@ __global__ void __launch_bounds__(256) k_b_aggregate(int nk, int nb, const int* 
   __shared__ T2 xs[KC * RS];
   const int b = blockIdx.x;
   const int s0 = box_slot[b], nf = box_slot[b + 1] - s0;
-  const int nk2 = 2 * nk;
+  const int nk2 = 2 * nk, ntile = (nk2 + 15) / 16;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave), l = threadIdx.x % kWave;
   const int li = l % 16, lk = l / 16;
-  const int tile = blockIdx.y * 4 + wave;
-  const int m = tile * 16 + li;
-  Acc cr[NT], ci[NT];
-#pragma unroll
-  for (int jt = 0; jt < NT; ++jt) {
-    cr[jt] = Acc{0, 0, 0, 0};
-    ci[jt] = Acc{0, 0, 0, 0};
-  }
-  for (int c0 = 0; c0 < nf; c0 += KC) {
+  const bool one_chunk = nf <= KC;
+  auto stage = [&](int c0) {
     const int nc = min(KC, nf - c0), nc4 = (nc + 3) & ~3;
-    __syncthreads();
     for (int e = threadIdx.x; e < nc4 * R; e += blockDim.x) {
       const int sl = e / R, r = e % R;
       T2 v = mk<S>(0, 0);
@@ -134,29 +128,52 @@ __global__ void __launch_bounds__(256) k_b_aggregate(int nk, int nb, const int* 
       }
       xs[sl * RS + r] = v;
     }
+  };
+  if (one_chunk) {
+    stage(0);
     __syncthreads();
-    for (int ks = 0; ks < nc4; ks += 4) {
-      const int sl = ks + lk;
-      const T2 a = (sl < nc && m < nk2) ? rad[(size_t)(s0 + c0 + sl) * nk2 + m] : mk<S>(0, 0);
-      const T2* xr = xs + sl * RS + li;
+  }
+  for (int round = 0; round < (ntile + 3) / 4; ++round) {
+    const int tile = round * 4 + wave;
+    const int m = tile * 16 + li;
+    Acc cr[NT], ci[NT];
 #pragma unroll
-      for (int jt = 0; jt < NT; ++jt) {
-        const T2 x = xr[jt * 16];
-        cr[jt] = mma(a.x, x.x, cr[jt]);
-        cr[jt] = mma(-a.y, x.y, cr[jt]);
-        ci[jt] = mma(a.x, x.y, ci[jt]);
-        ci[jt] = mma(a.y, x.x, ci[jt]);
+    for (int jt = 0; jt < NT; ++jt) {
+      cr[jt] = Acc{0, 0, 0, 0};
+      ci[jt] = Acc{0, 0, 0, 0};
+    }
+    for (int c0 = 0; c0 < nf; c0 += KC) {
+      if (!one_chunk) {
+        __syncthreads();
+        stage(c0);
+        __syncthreads();
+      }
+      const int nc = min(KC, nf - c0), nc4 = (nc + 3) & ~3;
+      for (int ks = 0; ks < nc4; ks += 4) {
+        const int sl = ks + lk;
+        const T2 a = (sl < nc && m < nk2) ? rad[(size_t)(s0 + c0 + sl) * nk2 + m] : mk<S>(0, 0);
+        const T2* xr = xs + sl * RS + li;
+#pragma unroll
+        for (int jt = 0; jt < NT; ++jt) {
+          const T2 x = xr[jt * 16];
+          cr[jt] = mma(a.x, x.x, cr[jt]);
+          cr[jt] = mma(-a.y, x.y, cr[jt]);
+          ci[jt] = mma(a.x, x.y, ci[jt]);
+          ci[jt] = mma(a.y, x.x, ci[jt]);
+        }
       }
     }
-  }
+    if (tile < ntile) {
 #pragma unroll
-  for (int v = 0; v < 4; ++v) {
-    const int mo = tile * 16 + mma_row<S>(lk, v);
-    if (mo < nk2) {
-      const int p = mo / nk, k = mo % nk;
-      T2* o = C + (((size_t)k * nb + b) * 2 + p) * R + li;
+      for (int v = 0; v < 4; ++v) {
+        const int mo = tile * 16 + mma_row<S>(lk, v);
+        if (mo < nk2) {
+          const int p = mo / nk, k = mo % nk;
+          T2* o = C + (((size_t)k * nb + b) * 2 + p) * R + li;
 #pragma unroll
-      for (int jt = 0; jt < NT; ++jt) o[jt * 16] = mk<S>(cr[jt][v], ci[jt][v]);
+          for (int jt = 0; jt < NT; ++jt) o[jt * 16] = mk<S>(cr[jt][v], ci[jt][v]);
+        }
+      }
     }
   }
 }
@@ -282,49 +299,106 @@ __global__ void __launch_bounds__(256) k_b_interp_up(int ncol, int nthc, int nkc
 // where the k-slice of C_n and the Tk row stay in L2.
 constexpr int kOffCodes = 343;  // (dx, dy, dz) in [-3, 3]^3
 
-// The work of one wave for a parent with NC children (acc[NC] in registers).
-template <typename S, int NC>
-__device__ __forceinline__ void far_wave(int P, int k, int nb, int ncol, int col, int c0,
-                                         const int* __restrict__ pc_ptr, const int* __restrict__ bo,
-                                         const int* __restrict__ q_ptr, const int2* __restrict__ q_ent,
-                                         const typename Cx<S>::T* tks, const typename Cx<S>::T* __restrict__ C,
-                                         int nbc, int nwt, const int* __restrict__ wt_ptr,
-                                         const int* __restrict__ wt_row, const typename Cx<S>::T* __restrict__ wp,
-                                         const typename Cx<S>::T* __restrict__ Ac, typename Cx<S>::T* __restrict__ A) {
+// Offset-code part of a child octant o (bits x,y,z = 4,2,1): 49 bx + 7 by + bz.
+__host__ __device__ constexpr int oct_code(int o) { return (o >> 2) * 49 + ((o >> 1) & 1) * 7 + (o & 1); }
+
+template <typename S>
+__global__ void __launch_bounds__(256) k_b_far(int nk, int nb, int nP, int ncol, int nchunk, int wpx, int split_p,
+                                               const int* __restrict__ pc_ptr, const int* __restrict__ p_mask,
+                                               const int* __restrict__ p_child, const int* __restrict__ q_ptr,
+                                               const int2* __restrict__ q_ent, const int* __restrict__ bcd,
+                                               const typename Cx<S>::T* __restrict__ Tk,
+                                               const typename Cx<S>::T* __restrict__ C, int nbc, int nwt,
+                                               const int* __restrict__ wt_ptr, const int* __restrict__ wt_row,
+                                               const typename Cx<S>::T* __restrict__ wp,
+                                               const typename Cx<S>::T* __restrict__ Ac,
+                                               typename Cx<S>::T* __restrict__ A) {
   typedef typename Cx<S>::T T2;
-  int oc[NC], cq[NC];
-  T2 acc[NC];
-#pragma unroll
-  for (int q = 0; q < NC; ++q) {
-    oc[q] = bo[c0 + q];
-    cq[q] = (oc[q] >> 2) * 49 + ((oc[q] >> 1) & 1) * 7 + (oc[q] & 1);
-    acc[q] = mk<S>(0, 0);
+  __shared__ T2 tks[kOffCodes];  // this sample's transfer functions, read as broadcasts
+  // Two workgroup maps (both: within a workgroup one sample k and one
+  // 64-column chunk, 4 waves = 4 consecutive parents):
+  //  split_p: XCD x owns the contiguous parent range [x*nPx, (x+1)*nPx) for
+  //           every sample; its L2 then holds the sources of a box range
+  //           (~1/8 of a k-slice plus a halo) instead of whole k-slices;
+  //  else:    (few parents) XCD x owns the samples k = x mod 8.
+  const int g = blockIdx.x, x = g % 8, j = g / 8;
+  int k, P0, P1, pb, chunk;
+  if (split_p) {
+    const int nPx = (nP + 7) / 8, nPBx = wpx / nchunk;
+    k = j / wpx;
+    chunk = (j % wpx) / nPBx;
+    pb = (j % wpx) % nPBx;
+    P0 = x * nPx + pb * 4;
+    P1 = min((x + 1) * nPx, nP);
+  } else {
+    k = (j / wpx) * 8 + x;
+    const int nPB = wpx / nchunk;
+    chunk = (j % wpx) / nPB;
+    pb = (j % wpx) % nPB;
+    P0 = pb * 4;
+    P1 = nP;
   }
-  const T2* Ck = C + (size_t)k * nb * ncol + col;
-  for (int e = q_ptr[P]; e < q_ptr[P + 1]; ++e) {
-    const int2 qe = q_ent[e];
-    const int ox = qe.y / 9 - 1, oy = (qe.y / 3) % 3 - 1, oz = qe.y % 3 - 1;
-    const int base = (2 * ox + 3) * 49 + (2 * oy + 3) * 7 + (2 * oz + 3);
-    const int d0 = pc_ptr[qe.x], nd = pc_ptr[qe.x + 1] - d0;
-    T2 v[8];
-    int cd[8];
+  if (k >= nk) return;  // uniform over the workgroup
+  for (int e = threadIdx.x; e < kOffCodes; e += blockDim.x) tks[e] = Tk[(size_t)k * kOffCodes + e];
+  __syncthreads();
+  const int P = P0 + __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+  if (P >= P1) return;
+  const int col = chunk * kWave + (threadIdx.x % kWave);
+  // accumulators by child octant: the code offset of an octant is then a
+  // compile-time constant, so each transfer value is one LDS broadcast read
+  // at an immediate offset from the source's code; absent octants are skipped
+  // by the (uniform) mask test and never stored
+  const int mask = p_mask[P];
+  T2 acc[8];
 #pragma unroll
-    for (int jd = 0; jd < 8; ++jd) {
-      const int od = jd < nd ? bo[d0 + jd] : 0;
-      cd[jd] = base + (od >> 2) * 49 + ((od >> 1) & 1) * 7 + (od & 1);
-      v[jd] = jd < nd ? Ck[(size_t)(d0 + jd) * ncol] : mk<S>(0, 0);
+  for (int o = 0; o < 8; ++o) acc[o] = mk<S>(0, 0);
+  const T2* Ck = C + (size_t)k * nb * ncol + col;
+  constexpr int kMaxCq = oct_code(7);  // 57: code - oct_code(o) >= 0 for every source
+  // source rows of neighbour parent Q (entry e) are fetched one entry ahead
+  // of their use, so that a wave has the next entry's loads in flight while
+  // it computes the current one (the loop is otherwise one memory round trip
+  // per entry)
+  const int e0 = q_ptr[P], e1 = q_ptr[P + 1];
+  T2 v[8];
+  int base = 0, d0 = 0, nd = 0;
+  if (e0 < e1) {
+    const int2 qe = q_ent[e0];  // (Q, 49 (2ox+3) + 7 (2oy+3) + (2oz+3))
+    base = qe.y;
+    d0 = pc_ptr[qe.x];
+    nd = pc_ptr[qe.x + 1] - d0;
+#pragma unroll
+    for (int jd = 0; jd < 8; ++jd) v[jd] = jd < nd ? Ck[(size_t)(d0 + jd) * ncol] : mk<S>(0, 0);
+  }
+  for (int e = e0; e < e1; ++e) {
+    T2 vn[8];
+    int basen = 0, d0n = 0, ndn = 0;
+    if (e + 1 < e1) {
+      const int2 qn = q_ent[e + 1];
+      basen = qn.y;
+      d0n = pc_ptr[qn.x];
+      ndn = pc_ptr[qn.x + 1] - d0n;
+#pragma unroll
+      for (int jd = 0; jd < 8; ++jd) vn[jd] = jd < ndn ? Ck[(size_t)(d0n + jd) * ncol] : mk<S>(0, 0);
     }
 #pragma unroll
     for (int jd = 0; jd < 8; ++jd) {
       if (jd < nd) {
+        const T2* tb = tks + (base + bcd[d0 + jd] - kMaxCq);
 #pragma unroll
-        for (int q = 0; q < NC; ++q) {
-          const T2 tt = tks[cd[jd] - cq[q]];
-          acc[q].x += tt.x * v[jd].x - tt.y * v[jd].y;
-          acc[q].y += tt.x * v[jd].y + tt.y * v[jd].x;
+        for (int o = 0; o < 8; ++o) {
+          if (mask & (1 << o)) {
+            const T2 tt = tb[kMaxCq - oct_code(o)];
+            acc[o].x += tt.x * v[jd].x - tt.y * v[jd].y;
+            acc[o].y += tt.x * v[jd].y + tt.y * v[jd].x;
+          }
         }
       }
     }
+#pragma unroll
+    for (int jd = 0; jd < 8; ++jd) v[jd] = vn[jd];
+    base = basen;
+    d0 = d0n;
+    nd = ndn;
   }
   if (Ac) {
     const int e0 = wt_ptr[k], e1 = wt_ptr[k + 1];
@@ -336,10 +410,12 @@ __device__ __forceinline__ void far_wave(int P, int k, int nb, int ncol, int col
       for (int u = 0; u < 4; ++u) {
         if (e + u < e1) {
 #pragma unroll
-          for (int q = 0; q < NC; ++q) {
-            const T2 w = wp[(size_t)oc[q] * nwt + e + u];
-            acc[q].x += w.x * a[u].x - w.y * a[u].y;
-            acc[q].y += w.x * a[u].y + w.y * a[u].x;
+          for (int o = 0; o < 8; ++o) {
+            if (mask & (1 << o)) {
+              const T2 w = wp[(size_t)o * nwt + e + u];
+              acc[o].x += w.x * a[u].x - w.y * a[u].y;
+              acc[o].y += w.x * a[u].y + w.y * a[u].x;
+            }
           }
         }
       }
@@ -347,53 +423,8 @@ __device__ __forceinline__ void far_wave(int P, int k, int nb, int ncol, int col
   }
   T2* Ak = A + (size_t)k * nb * ncol + col;
 #pragma unroll
-  for (int q = 0; q < NC; ++q) Ak[(size_t)(c0 + q) * ncol] = acc[q];
-}
-
-template <typename S>
-__global__ void __launch_bounds__(256) k_b_far(int nk, int nb, int nP, int ncol, int nchunk, int wgs_per_k,
-                                               const int* __restrict__ pc_ptr, const int* __restrict__ bo,
-                                               const int* __restrict__ q_ptr, const int2* __restrict__ q_ent,
-                                               const typename Cx<S>::T* __restrict__ Tk,
-                                               const typename Cx<S>::T* __restrict__ C, int nbc, int nwt,
-                                               const int* __restrict__ wt_ptr, const int* __restrict__ wt_row,
-                                               const typename Cx<S>::T* __restrict__ wp,
-                                               const typename Cx<S>::T* __restrict__ Ac,
-                                               typename Cx<S>::T* __restrict__ A) {
-  typedef typename Cx<S>::T T2;
-  __shared__ T2 tks[kOffCodes];  // this sample's transfer functions, read as broadcasts
-  const int g = blockIdx.x;
-  const int k = (g / 8 / wgs_per_k) * 8 + g % 8;
-  if (k >= nk) return;  // uniform over the workgroup
-  for (int e = threadIdx.x; e < kOffCodes; e += blockDim.x) tks[e] = Tk[(size_t)k * kOffCodes + e];
-  __syncthreads();
-  // within a sample, all parents of one 64-column chunk before the next
-  // chunk: the chunk's slice of C_n (nb x 64 columns) is what the XCD's L2
-  // has to hold
-  const int t = ((g / 8) % wgs_per_k) * 4 + __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
-  const int nPr = wgs_per_k * 4 / nchunk;  // parents per chunk, rounded up to whole workgroups
-  const int P = t % nPr;
-  if (P >= nP) return;
-  const int col = (t / nPr) * kWave + (threadIdx.x % kWave);
-  const int c0 = pc_ptr[P], nc = pc_ptr[P + 1] - c0;
-#define LW_FAR_CASE(NC)                                                                                      \
-  case NC:                                                                                                   \
-    far_wave<S, NC>(P, k, nb, ncol, col, c0, pc_ptr, bo, q_ptr, q_ent, tks, C, nbc, nwt, wt_ptr, wt_row, wp, \
-                    Ac, A);                                                                                  \
-    break;
-  switch (nc) {
-    LW_FAR_CASE(1)
-    LW_FAR_CASE(2)
-    LW_FAR_CASE(3)
-    LW_FAR_CASE(4)
-    LW_FAR_CASE(5)
-    LW_FAR_CASE(6)
-    LW_FAR_CASE(7)
-    LW_FAR_CASE(8)
-    default:
-      break;
-  }
-#undef LW_FAR_CASE
+  for (int o = 0; o < 8; ++o)
+    if (mask & (1 << o)) Ak[(size_t)p_child[8 * P + o] * ncol] = acc[o];
 }
 
 // Tk[k][code] = pes[k] T[tr(code)][k] (zero for codes without a transfer vector)
@@ -909,9 +940,11 @@ struct BatchLevel {
   // fused translation + downward pass at this level (parents at level n-1)
   int nP = 0, nwt = 0;
   int* pc_ptr = nullptr;  // (nP+1) children of each parent
-  int* bo = nullptr;      // (nb) octant of each box within its parent
+  int* p_mask = nullptr;  // (nP) child octants present
+  int* p_child = nullptr; // (8, nP) child box of each octant
+  int* bcd = nullptr;     // (nb) offset-code part of each box's octant
   int* q_ptr = nullptr;   // (nP+1) neighbour parents holding far boxes
-  int2* q_ent = nullptr;  // (Q, parent offset code (ox+1)*9 + (oy+1)*3 + (oz+1))
+  int2* q_ent = nullptr;  // (Q, 49 (2ox+3) + 7 (2oy+3) + (2oz+3)) for parent offset o = Q - P
   void* Tk = nullptr;     // (nk, 343) pes-scaled transfer function per offset code
   void* wp = nullptr;     // (8, nwt) anterpolation weights from level n-1 (n > 3)
   void *C = nullptr, *A = nullptr;
@@ -958,7 +991,11 @@ template <typename T>
 static int upload_b(Batch* b, T** p, const std::vector<T>& v) {
   int rc = bmalloc(b, p, v.size());
   if (rc) return rc;
-  if (!v.empty() && hipMemcpy(*p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice) != hipSuccess) {
+  // ordered on the handle's (non-blocking) stream and completed before
+  // returning: callers pass temporaries
+  hipStream_t st = b->h->stream;
+  if (!v.empty() && (hipMemcpyAsync(*p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, st) != hipSuccess ||
+                     hipStreamSynchronize(st) != hipSuccess)) {
     set_error("lw_batch_create: upload failed");
     return LW_ERR_HIP;
   }
@@ -969,6 +1006,7 @@ static int upload_b(Batch* b, T** p, const std::vector<T>& v) {
 struct FarBlocks {
   std::vector<int> bo, q_ptr, code_tr;
   std::vector<int2> q_ent;
+  std::vector<int> p_mask, p_child, bcd;  // per parent: child octants present, child per octant; per box: code part
 };
 
 // Derives the parent-pair blocks from the plan's per-box far lists and checks
@@ -1012,7 +1050,7 @@ static bool far_blocks(const Plan& P, int n, FarBlocks& fb, std::string& msg) {
           return false;
         }
       }
-      fb.q_ent.push_back(make_int2(Q, (o[0] + 1) * 9 + (o[1] + 1) * 3 + (o[2] + 1)));
+      fb.q_ent.push_back(make_int2(Q, (2 * o[0] + 3) * 49 + (2 * o[1] + 3) * 7 + (2 * o[2] + 3)));
     }
     fb.q_ptr[Pb + 1] = (int)fb.q_ent.size();
     // regenerate every child's far list from the blocks and compare
@@ -1020,7 +1058,7 @@ static bool far_blocks(const Plan& P, int n, FarBlocks& fb, std::string& msg) {
       gen.clear();
       for (int e = fb.q_ptr[Pb]; e < fb.q_ptr[Pb + 1]; ++e) {
         const int Q = fb.q_ent[e].x, oc = fb.q_ent[e].y;
-        const int o[3] = {oc / 9 - 1, (oc / 3) % 3 - 1, oc % 3 - 1};
+        const int o[3] = {(oc / 49 - 3) / 2, ((oc / 7) % 7 - 3) / 2, (oc % 7 - 3) / 2};
         for (int d = pp.child_ptr[Q]; d < pp.child_ptr[Q + 1]; ++d) {
           int off[3], far = 0;
           for (int k = 0; k < 3; ++k) {
@@ -1064,6 +1102,19 @@ static bool far_blocks(const Plan& P, int n, FarBlocks& fb, std::string& msg) {
     msg = "interaction list count mismatch";
     return false;
   }
+  fb.bcd.resize(nb);
+  for (int c = 0; c < nb; ++c) fb.bcd[c] = oct_code(fb.bo[c]);
+  fb.p_mask.assign(nP, 0);
+  fb.p_child.assign(8 * (size_t)nP, 0);
+  for (int Pb = 0; Pb < nP; ++Pb)
+    for (int c = pp.child_ptr[Pb]; c < pp.child_ptr[Pb + 1]; ++c) {
+      if (fb.p_mask[Pb] & (1 << fb.bo[c])) {
+        msg = "two children of one parent in the same octant";
+        return false;
+      }
+      fb.p_mask[Pb] |= 1 << fb.bo[c];
+      fb.p_child[8 * (size_t)Pb + fb.bo[c]] = c;
+    }
   return true;
 }
 
@@ -1089,8 +1140,7 @@ static void launch_leaf(Batch* bt, bool up, const double2* X) {
   BatchLevel& lf = bt->lv[h->L];
   if (up) {
     constexpr int KC = agg_chunk<S, R>();
-    const int ntile = (2 * lf.nk + 15) / 16;
-    hipLaunchKernelGGL((k_b_aggregate<S, R, KC>), dim3(lf.nb, (ntile + 3) / 4), dim3(256), 0, h->stream, lf.nk,
+    hipLaunchKernelGGL((k_b_aggregate<S, R, KC>), dim3(lf.nb), dim3(256), 0, h->stream, lf.nk,
                        lf.nb, h->d_box_slot, h->d_slot_basis, (const T2*)bt->rad, X, (T2*)lf.C);
   } else {
     hipLaunchKernelGGL((k_b_receive<S, R>), dim3(lf.nb), dim3(128), 0, h->stream, lf.nk, lf.nb, h->d_box_slot,
@@ -1123,10 +1173,14 @@ static int batch_matvec_t(Batch* bt, const double2* X, double2* Y) {
   for (int n = 3; n <= L; ++n) {
     BatchLevel& d = bt->lv[n];
     const BatchLevel* pl = n > 3 ? &bt->lv[n - 1] : nullptr;
-    const int wgs_per_k = nblk(d.nP, 4) * nchunk;  // each chunk covers the parents in whole workgroups
-    const int grid = 8 * nblk(d.nk, 8) * wgs_per_k;
-    hipLaunchKernelGGL(k_b_far<S>, dim3(grid), dim3(256), 0, st, d.nk, d.nb, d.nP, ncol, nchunk, wgs_per_k, d.pc_ptr,
-                       d.bo, d.q_ptr, d.q_ent, (const T2*)d.Tk, (const T2*)d.C, pl ? pl->nb : 0, d.nwt,
+    // workgroups per (XCD, sample) with the parents split over the XCDs, or
+    // per sample when there are too few parents to split
+    const bool split_p = d.nP >= 64;
+    const int wpx = split_p ? nblk(nblk(d.nP, 8), 4) * nchunk : nblk(d.nP, 4) * nchunk;
+    const long long grid = split_p ? 8LL * d.nk * wpx : 8LL * nblk(d.nk, 8) * wpx;
+    hipLaunchKernelGGL(k_b_far<S>, dim3((unsigned)grid), dim3(256), 0, st, d.nk, d.nb, d.nP, ncol, nchunk, wpx,
+                       split_p ? 1 : 0, d.pc_ptr, d.p_mask, d.p_child, d.q_ptr, d.q_ent, d.bcd, (const T2*)d.Tk,
+                       (const T2*)d.C, pl ? pl->nb : 0, d.nwt,
                        pl ? h->lv[n - 1].wt_ptr : nullptr, pl ? h->lv[n - 1].wt_row : nullptr, (const T2*)d.wp,
                        pl ? (const T2*)pl->A : nullptr, (T2*)d.A);
   }
@@ -1367,7 +1421,8 @@ static int setup_far(Batch* bt) {
     }
     b.nP = P.lv[n - 1].nb;
     int* d_code_tr = nullptr;
-    if ((rc = upload_b(bt, &b.pc_ptr, P.lv[n - 1].child_ptr)) || (rc = upload_b(bt, &b.bo, fb.bo)) ||
+    if ((rc = upload_b(bt, &b.pc_ptr, P.lv[n - 1].child_ptr)) || (rc = upload_b(bt, &b.p_mask, fb.p_mask)) ||
+        (rc = upload_b(bt, &b.p_child, fb.p_child)) || (rc = upload_b(bt, &b.bcd, fb.bcd)) ||
         (rc = upload_b(bt, &b.q_ptr, fb.q_ptr)) || (rc = upload_b(bt, &b.q_ent, fb.q_ent)) ||
         (rc = upload_b(bt, &d_code_tr, fb.code_tr)))
       return rc;
@@ -1494,7 +1549,7 @@ int lw_batch_matvec(lw_batch* ob, const double* X, double* Y) {
   lw::Batch* bt = &ob->b;
   lw::Handle* h = bt->h;
   const size_t bytes = sizeof(double2) * (size_t)h->ne * bt->R;
-  LW_HIP(hipMemcpy(bt->X, X, bytes, hipMemcpyHostToDevice));
+  LW_HIP(hipMemcpyAsync(bt->X, X, bytes, hipMemcpyHostToDevice, h->stream));
   int rc = lw::batch_matvec(bt, bt->X, bt->Y);
   if (rc) return rc;
   LW_HIP(hipMemcpyAsync(Y, bt->Y, bytes, hipMemcpyDeviceToHost, h->stream));
@@ -1507,7 +1562,7 @@ int lw_batch_lusol0(lw_batch* ob, const double* Yv, double* Xv) {
   lw::Batch* bt = &ob->b;
   lw::Handle* h = bt->h;
   const size_t bytes = sizeof(double2) * (size_t)h->ne * bt->R;
-  LW_HIP(hipMemcpy(bt->X, Yv, bytes, hipMemcpyHostToDevice));
+  LW_HIP(hipMemcpyAsync(bt->X, Yv, bytes, hipMemcpyHostToDevice, h->stream));
   int rc = lw::batch_lusol(bt, bt->X, bt->Y);
   if (rc) return rc;
   LW_HIP(hipMemcpyAsync(Xv, bt->Y, bytes, hipMemcpyDeviceToHost, h->stream));
@@ -1528,11 +1583,12 @@ int lw_batch_cgs(lw_batch* ob, const double* Bv, const double* guess, double err
   const int R = bt->R;
   const size_t bytes = sizeof(double2) * (size_t)h->ne * R;
   hipStream_t st = h->stream;
-  LW_HIP(hipMemcpy(bt->w[0], Bv, bytes, hipMemcpyHostToDevice));
+  // on the handle's stream: it does not synchronise with the null stream
+  LW_HIP(hipMemcpyAsync(bt->w[0], Bv, bytes, hipMemcpyHostToDevice, h->stream));
   if (guess)
-    LW_HIP(hipMemcpy(bt->X, guess, bytes, hipMemcpyHostToDevice));
+    LW_HIP(hipMemcpyAsync(bt->X, guess, bytes, hipMemcpyHostToDevice, h->stream));
   else
-    LW_HIP(hipMemset(bt->X, 0, bytes));
+    LW_HIP(hipMemsetAsync(bt->X, 0, bytes, h->stream));
   int rc = batch_cgs_run(bt, errmax, use_prec != 0, maxit);
   if (rc) return rc;
   std::vector<int4> sv(R);
@@ -1560,11 +1616,12 @@ int lw_batch_cgs_start(lw_batch* ob, const double* Bv, const double* guess, doub
     return LW_ERR_STATE;
   }
   const size_t bytes = sizeof(double2) * (size_t)h->ne * bt->R;
-  LW_HIP(hipMemcpy(bt->w[0], Bv, bytes, hipMemcpyHostToDevice));
+  // on the handle's stream: it does not synchronise with the null stream
+  LW_HIP(hipMemcpyAsync(bt->w[0], Bv, bytes, hipMemcpyHostToDevice, h->stream));
   if (guess)
-    LW_HIP(hipMemcpy(bt->X, guess, bytes, hipMemcpyHostToDevice));
+    LW_HIP(hipMemcpyAsync(bt->X, guess, bytes, hipMemcpyHostToDevice, h->stream));
   else
-    LW_HIP(hipMemset(bt->X, 0, bytes));
+    LW_HIP(hipMemsetAsync(bt->X, 0, bytes, h->stream));
   int rc = batch_cgs_start(bt, errmax, use_prec != 0, maxit);
   bt->next_report = progress_every();
   return rc;
@@ -1581,8 +1638,10 @@ int lw_batch_cgs_advance(lw_batch* ob, int iterations, int* state, double* relre
   if (iterations == 0) {
     sv.resize(bt->R);
     rr.resize(bt->R);
-    LW_HIP(hipMemcpy(sv.data(), bt->stv, sizeof(int4) * bt->R, hipMemcpyDeviceToHost));
-    LW_HIP(hipMemcpy(rr.data(), bt->rrv, sizeof(double) * bt->R, hipMemcpyDeviceToHost));
+    hipStream_t st = bt->h->stream;
+    LW_HIP(hipMemcpyAsync(sv.data(), bt->stv, sizeof(int4) * bt->R, hipMemcpyDeviceToHost, st));
+    LW_HIP(hipMemcpyAsync(rr.data(), bt->rrv, sizeof(double) * bt->R, hipMemcpyDeviceToHost, st));
+    LW_HIP(hipStreamSynchronize(st));
   }
   for (int c = 0; c < bt->R; ++c) {
     if (state) {
@@ -1654,8 +1713,8 @@ int lw_batch_plane_wave(lw_batch* ob, const double* dirs, const double* pols, co
   double2* d_p = nullptr;
   LW_HIP(hipMalloc(&d_d, sizeof(double) * d.size()));
   LW_HIP(hipMalloc(&d_p, sizeof(double2) * 3 * R));
-  LW_HIP(hipMemcpy(d_d, d.data(), sizeof(double) * d.size(), hipMemcpyHostToDevice));
-  LW_HIP(hipMemcpy(d_p, pols, sizeof(double2) * 3 * R, hipMemcpyHostToDevice));
+  LW_HIP(hipMemcpyAsync(d_d, d.data(), sizeof(double) * d.size(), hipMemcpyHostToDevice, h->stream));
+  LW_HIP(hipMemcpyAsync(d_p, pols, sizeof(double2) * 3 * R, hipMemcpyHostToDevice, h->stream));
   const long long tot = (long long)h->ne * R;
   hipLaunchKernelGGL(k_b_plane_wave, dim3(nblk(tot, 256)), dim3(256), 0, h->stream, R, h->ne, h->d_tb, h->d_p,
                      h->d_et, h->d_epo, h->d_el, d_d, d_p, make_double2(ctte[0], ctte[1]),
@@ -1678,7 +1737,8 @@ int lw_batch_rcs(lw_batch* ob, const double* cur, const double* pols, const doub
   const Plan& P = h->plan;
   // far-zone distance and centre from the edge-centre bounding box (modRCS.f90:120-147)
   std::vector<double> ec(3 * (size_t)h->ne);
-  LW_HIP(hipMemcpy(ec.data(), h->d_ec, sizeof(double) * ec.size(), hipMemcpyDeviceToHost));
+  LW_HIP(hipMemcpyAsync(ec.data(), h->d_ec, sizeof(double) * ec.size(), hipMemcpyDeviceToHost, h->stream));
+  LW_HIP(hipStreamSynchronize(h->stream));
   int lim[6] = {0, 0, 0, 0, 0, 0};
   for (int i = 1; i < h->ne; ++i) {
     const double* c = &ec[3 * (size_t)i];
@@ -1710,8 +1770,8 @@ int lw_batch_rcs(lw_batch* ob, const double* cur, const double* pols, const doub
   double2* d_E = nullptr;
   LW_HIP(hipMalloc(&d_ro, sizeof(double) * ro.size()));
   LW_HIP(hipMalloc(&d_E, sizeof(double2) * 3 * R));
-  LW_HIP(hipMemcpy(d_ro, ro.data(), sizeof(double) * ro.size(), hipMemcpyHostToDevice));
-  LW_HIP(hipMemcpy(bt->Y, cur, bytes, hipMemcpyHostToDevice));
+  LW_HIP(hipMemcpyAsync(d_ro, ro.data(), sizeof(double) * ro.size(), hipMemcpyHostToDevice, h->stream));
+  LW_HIP(hipMemcpyAsync(bt->Y, cur, bytes, hipMemcpyHostToDevice, h->stream));
   hipLaunchKernelGGL(k_b_rcs, dim3(R), dim3(256), 0, h->stream, R, h->ne, h->d_tb, h->d_et, h->d_el, bt->Y, d_ro, kfs,
                      d_E);
   std::vector<double2> E(3 * (size_t)R);
@@ -1745,7 +1805,7 @@ int lw_bench_batch(lw_batch* ob, int reps, double* matvec_ms, double* lusol_ms) 
   const size_t nv = (size_t)h->ne * bt->R;
   std::vector<double2> xin(nv);
   for (size_t i = 0; i < nv; ++i) xin[i] = make_double2(1.0 / (1.0 + (double)(i % 977)), 0.5 - (double)(i % 13) / 13);
-  LW_HIP(hipMemcpy(bt->X, xin.data(), sizeof(double2) * nv, hipMemcpyHostToDevice));
+  LW_HIP(hipMemcpyAsync(bt->X, xin.data(), sizeof(double2) * nv, hipMemcpyHostToDevice, h->stream));
   hipEvent_t e0, e1;
   LW_HIP(hipEventCreate(&e0));
   LW_HIP(hipEventCreate(&e1));

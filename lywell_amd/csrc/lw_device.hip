@@ -305,6 +305,13 @@ __global__ void __launch_bounds__(256) k_receive_near(int ne, int nk2, const int
   }
 }
 
+// Device copy of n complex(8) values (grid-stride): the bandwidth reference
+// for the roofline figures and the HBM-traffic counter calibration.
+__global__ void __launch_bounds__(256) k_copy(long long n, const double2* __restrict__ a, double2* __restrict__ b) {
+  for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (long long)gridDim.x * blockDim.x)
+    b[i] = a[i];
+}
+
 // ---------------------------------------------------------------- ILU0 solves
 
 // lusol0C (modIterativo.f90:504-560): forward solve with the unit-diagonal L
@@ -1316,6 +1323,36 @@ int lw_bench_lusol(lw_handle* hh, int reps, double* ms) {
   LW_HIP(hipEventElapsedTime(&t, e0, e1));
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
+  *ms = t / reps;
+  return LW_OK;
+}
+
+int lw_bench_copy(lw_handle* hh, long long bytes, int reps, double* ms) {
+  if (lw::check_handle(hh) || reps < 1 || !ms || bytes < 16 || bytes % 16) return LW_ERR_ARG;
+  Handle* h = &hh->h;
+  const long long n = bytes / 16;
+  double2 *a = nullptr, *b = nullptr;
+  LW_HIP(hipMalloc(&a, bytes));
+  if (hipMalloc(&b, bytes) != hipSuccess) {
+    (void)hipFree(a);
+    return LW_ERR_NOMEM;
+  }
+  hipEvent_t e0, e1;
+  LW_HIP(hipEventCreate(&e0));
+  LW_HIP(hipEventCreate(&e1));
+  const int blocks = (int)std::min<long long>((n + 255) / 256, 1 << 20);
+  hipLaunchKernelGGL(lw::k_copy, dim3(blocks), dim3(256), 0, h->stream, n, (const double2*)a, b);
+  LW_HIP(hipEventRecord(e0, h->stream));
+  for (int r = 0; r < reps; ++r)
+    hipLaunchKernelGGL(lw::k_copy, dim3(blocks), dim3(256), 0, h->stream, n, (const double2*)a, b);
+  LW_HIP(hipEventRecord(e1, h->stream));
+  LW_HIP(hipEventSynchronize(e1));
+  float t = 0.f;
+  LW_HIP(hipEventElapsedTime(&t, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  (void)hipFree(a);
+  (void)hipFree(b);
   *ms = t / reps;
   return LW_OK;
 }

@@ -127,8 +127,13 @@ template <typename T>
 int upload(Handle* h, T** p, const std::vector<T>& v) {
   int rc = dmalloc(h, p, v.size());
   if (rc) return rc;
-  // synchronous copy: callers may pass temporaries
-  if (!v.empty()) LW_HIP(hipMemcpy(*p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+  // ordered on the handle's stream (a non-blocking stream: null-stream
+  // copies would not be ordered with its kernels) and completed before
+  // returning, since callers may pass temporaries
+  if (!v.empty()) {
+    LW_HIP(hipMemcpyAsync(*p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, h->stream));
+    LW_HIP(hipStreamSynchronize(h->stream));
+  }
   return LW_OK;
 }
 

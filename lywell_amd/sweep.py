@@ -132,6 +132,7 @@ class Sweep:
         done = [len(ch) == 0 for ch in chains]
         solved = 0
         restarts = 0
+        next_log = R
         while not all(done):
             st, rr = self.bt.advance(check_every)
             for c in range(R):
@@ -154,8 +155,9 @@ class Sweep:
                     restarts += 1
                 else:
                     done[c] = True
-            if solved and solved % R == 0:
+            if solved >= next_log:
                 self.log(f"sweep: {solved}/{n} angles solved, {time.time() - t0:.1f}s")
+                next_log += R
         wall = time.time() - t0
         self.log(f"sweep ({self.guess} guess, persistent): {n} angles in {wall:.2f}s, "
                  f"mean {iters.mean():.0f} iterations per angle")
