@@ -346,44 +346,24 @@ __global__ void __launch_bounds__(256) k_b_far(int nk, int nb, int nP, int ncol,
   const int col = chunk * kWave + (threadIdx.x % kWave);
   // accumulators by child octant: the code offset of an octant is then a
   // compile-time constant, so each transfer value is one LDS broadcast read
-  // at an immediate offset from the source's code; absent octants are skipped
-  // by the (uniform) mask test and never stored
+  // at an immediate offset from the source's code; absent octants are
+  // skipped by the (uniform) mask test and never stored
   const int mask = p_mask[P];
   T2 acc[8];
 #pragma unroll
   for (int o = 0; o < 8; ++o) acc[o] = mk<S>(0, 0);
   const T2* Ck = C + (size_t)k * nb * ncol + col;
   constexpr int kMaxCq = oct_code(7);  // 57: code - oct_code(o) >= 0 for every source
-  // source rows of neighbour parent Q (entry e) are fetched one entry ahead
-  // of their use, so that a wave has the next entry's loads in flight while
-  // it computes the current one (the loop is otherwise one memory round trip
-  // per entry)
-  const int e0 = q_ptr[P], e1 = q_ptr[P + 1];
-  T2 v[8];
-  int base = 0, d0 = 0, nd = 0;
-  if (e0 < e1) {
-    const int2 qe = q_ent[e0];  // (Q, 49 (2ox+3) + 7 (2oy+3) + (2oz+3))
-    base = qe.y;
-    d0 = pc_ptr[qe.x];
-    nd = pc_ptr[qe.x + 1] - d0;
+  for (int e = q_ptr[P]; e < q_ptr[P + 1]; ++e) {
+    const int2 qe = q_ent[e];  // (Q, 49 (2ox+3) + 7 (2oy+3) + (2oz+3))
+    const int d0 = pc_ptr[qe.x], nd = pc_ptr[qe.x + 1] - d0;
+    T2 v[8];
 #pragma unroll
     for (int jd = 0; jd < 8; ++jd) v[jd] = jd < nd ? Ck[(size_t)(d0 + jd) * ncol] : mk<S>(0, 0);
-  }
-  for (int e = e0; e < e1; ++e) {
-    T2 vn[8];
-    int basen = 0, d0n = 0, ndn = 0;
-    if (e + 1 < e1) {
-      const int2 qn = q_ent[e + 1];
-      basen = qn.y;
-      d0n = pc_ptr[qn.x];
-      ndn = pc_ptr[qn.x + 1] - d0n;
-#pragma unroll
-      for (int jd = 0; jd < 8; ++jd) vn[jd] = jd < ndn ? Ck[(size_t)(d0n + jd) * ncol] : mk<S>(0, 0);
-    }
 #pragma unroll
     for (int jd = 0; jd < 8; ++jd) {
       if (jd < nd) {
-        const T2* tb = tks + (base + bcd[d0 + jd] - kMaxCq);
+        const T2* tb = tks + (qe.y + bcd[d0 + jd] - kMaxCq);
 #pragma unroll
         for (int o = 0; o < 8; ++o) {
           if (mask & (1 << o)) {
@@ -394,11 +374,6 @@ __global__ void __launch_bounds__(256) k_b_far(int nk, int nb, int nP, int ncol,
         }
       }
     }
-#pragma unroll
-    for (int jd = 0; jd < 8; ++jd) v[jd] = vn[jd];
-    base = basen;
-    d0 = d0n;
-    nd = ndn;
   }
   if (Ac) {
     const int e0 = wt_ptr[k], e1 = wt_ptr[k + 1];
