@@ -311,7 +311,17 @@ def batched_sweep(lw, op, args, plain_rhs_per_s):
     its = res["iterations"]
     pos = np.arange(n) % m  # position of each angle in its chain
     by_pos = {int(q): float(its[pos == q].mean()) for q in range(m)}
+    # steady state of a long sweep: every angle past the second of its chain
+    # starts from the learned guess; time per batched iteration from this run
+    s_per_iter = res["seconds"] / max(1.0, float(its.sum()) / R)
+    late = its[pos >= 2] if m > 2 else its
+    steady = R / (float(late.mean()) * s_per_iter)
+    log(f"steady state (chain positions >= 2): {late.mean():.0f} iterations per angle, "
+        f"{s_per_iter * 1e3:.1f} ms per batched iteration -> {steady:.3f} RHS/s")
     return {"rhs_per_s": round(value, 4), "speedup_vs_plain_sweep": round(value / plain_sweep_rhs_s, 2),
+            "steady_state_rhs_per_s": round(steady, 4),
+            "steady_state_speedup_vs_plain_sweep": round(steady / plain_sweep_rhs_s, 2),
+            "ms_per_batched_iteration": round(s_per_iter * 1e3, 2),
             "speedup_vs_plain_single": round(value / plain_rhs_per_s, 2),
             "plain_sweep_rhs_per_s": round(plain_sweep_rhs_s, 4), "plain_sweep_iterations": plain_its,
             "angles": n, "nrhs_per_batch": R, "chain_length": m, "seconds": round(res["seconds"], 2),

@@ -14,6 +14,35 @@ namespace lw {
 void set_error(const std::string& m);
 int progress_every();
 
+// Per-basis geometry of the radiation pattern (calcRad, modMLFMA.f90:669-699),
+// in leaf-slot order: dp = c+ - C, dm = c- - C (triangle centroids relative to
+// the leaf-box centre C), rp = c+ - v+, rm = v- - c- (RWG vectors at the
+// centroids), and the edge length.
+struct RadGeom {
+  double dp[3], dm[3], rp[3], rm[3], len;
+};
+
+// funcRAD of one basis at one sample (theta and phi components), the
+// reference's non-maxPrecision formula with its operation order; used both
+// to tabulate the patterns and to regenerate them inside the matvec, with
+// contraction off so that both give the same bits.
+__device__ inline void rad_eval(const RadGeom& g, const double* __restrict__ vk, const double* __restrict__ at,
+                                const double* __restrict__ ap, double ka, double2& r0, double2& r1) {
+#pragma clang fp contract(off)
+  double sp, cp, sm, cm;
+  sincos(ka * ((vk[0] * g.dp[0] + vk[1] * g.dp[1]) + vk[2] * g.dp[2]), &sp, &cp);
+  sincos(ka * ((vk[0] * g.dm[0] + vk[1] * g.dm[1]) + vk[2] * g.dm[2]), &sm, &cm);
+  double vx[3], vy[3];
+  for (int d = 0; d < 3; ++d) {
+    vx[d] = ((cp * g.rp[d] + cm * g.rm[d]) * g.len) * 0.5;
+    vy[d] = ((sp * g.rp[d] + sm * g.rm[d]) * g.len) * 0.5;
+  }
+  r0.x = (vx[0] * at[0] + vx[1] * at[1]) + vx[2] * at[2];
+  r0.y = (vy[0] * at[0] + vy[1] * at[1]) + vy[2] * at[2];
+  r1.x = (vx[0] * ap[0] + vx[1] * ap[1]) + vx[2] * ap[2];
+  r1.y = (vy[0] * ap[0] + vy[1] * ap[1]) + vy[2] * ap[2];
+}
+
 // Workgroup id w of n -> logical block: the hardware deals workgroups to the
 // 8 XCDs round-robin (XCD = w mod 8); this bijection gives each XCD one
 // contiguous range of logical blocks, so neighbouring blocks (which share
@@ -83,6 +112,8 @@ struct Handle {
   int *d_et = nullptr, *d_epo = nullptr;
   // leaf patterns and ordering
   double2* d_rad = nullptr;  // (nk_L, 2, ne) in slot order
+  RadGeom* d_rgeo = nullptr;  // (ne) per-basis pattern geometry, slot order
+  double *d_ath = nullptr, *d_aph = nullptr;  // (3, nk_L) theta and phi unit vectors of the leaf samples
   int *d_slot_basis = nullptr, *d_basis_slot = nullptr, *d_basis_box = nullptr, *d_box_slot = nullptr;
   // near field CSR (natural order)
   int *d_ia = nullptr, *d_ja = nullptr;

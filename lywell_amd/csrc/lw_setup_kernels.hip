@@ -31,47 +31,40 @@ __device__ inline double ipow(double x, int n) {
   return r;
 }
 
-// calcRad, non-maxPrecision branch (modMLFMA.f90:669-699): radiation pattern
-// of basis f about its leaf-box centre, theta and phi components.
-__global__ void k_rad(int ne, int nk, const double* __restrict__ kap, const double* __restrict__ ath,
-                      const double* __restrict__ aph, const double* __restrict__ box_c,
-                      const int* __restrict__ slot_basis, const int* __restrict__ basis_box,
-                      const double* __restrict__ tb, const double* __restrict__ pc,
-                      const int* __restrict__ et, const int* __restrict__ epo,
-                      const double* __restrict__ el, double ka, double2* __restrict__ rad) {
-  const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= (long long)ne * nk) return;
-  const int s = (int)(idx / nk), k = (int)(idx % nk);
+// Per-basis geometry of calcRad (modMLFMA.f90:669-699) in slot order.
+__global__ void k_rad_geom(int ne, const int* __restrict__ slot_basis, const int* __restrict__ basis_box,
+                           const double* __restrict__ box_c, const double* __restrict__ tb,
+                           const double* __restrict__ pc, const int* __restrict__ et, const int* __restrict__ epo,
+                           const double* __restrict__ el, RadGeom* __restrict__ g) {
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= ne) return;
   const int f = slot_basis[s];
   const double* c = box_c + 3 * basis_box[f];
   const double* bp = tb + 3 * (et[2 * f] - 1);
   const double* vp = pc + 3 * (epo[2 * f] - 1);
   const double* bm = tb + 3 * (et[2 * f + 1] - 1);
   const double* vm = pc + 3 * (epo[2 * f + 1] - 1);
-  const double lon = el[f];
-  const double* vk = kap + 3 * k;
-  double dp[3], dm[3], rp[3], rm[3];
+  RadGeom r;
   for (int d = 0; d < 3; ++d) {
-    rp[d] = bp[d] - vp[d];
-    rm[d] = vm[d] - bm[d];
-    dp[d] = bp[d] - c[d];
-    dm[d] = bm[d] - c[d];
+    r.rp[d] = bp[d] - vp[d];
+    r.rm[d] = vm[d] - bm[d];
+    r.dp[d] = bp[d] - c[d];
+    r.dm[d] = bm[d] - c[d];
   }
-  double sp, cp, sm, cm;
-  sincos(ka * ((vk[0] * dp[0] + vk[1] * dp[1]) + vk[2] * dp[2]), &sp, &cp);
-  sincos(ka * ((vk[0] * dm[0] + vk[1] * dm[1]) + vk[2] * dm[2]), &sm, &cm);
-  double2 vt[3];
-  for (int d = 0; d < 3; ++d) {
-    vt[d].x = ((cp * rp[d] + cm * rm[d]) * lon) * 0.5;
-    vt[d].y = ((sp * rp[d] + sm * rm[d]) * lon) * 0.5;
-  }
-  const double* at = ath + 3 * k;
-  const double* ap = aph + 3 * k;
+  r.len = el[f];
+  g[s] = r;
+}
+
+// calcRad, non-maxPrecision branch: radiation pattern of basis slot s about
+// its leaf-box centre, theta and phi components.
+__global__ void k_rad(int ne, int nk, const double* __restrict__ kap, const double* __restrict__ ath,
+                      const double* __restrict__ aph, const RadGeom* __restrict__ geo, double ka,
+                      double2* __restrict__ rad) {
+  const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long long)ne * nk) return;
+  const int s = (int)(idx / nk), k = (int)(idx % nk);
   double2 r0, r1;
-  r0.x = (vt[0].x * at[0] + vt[1].x * at[1]) + vt[2].x * at[2];
-  r0.y = (vt[0].y * at[0] + vt[1].y * at[1]) + vt[2].y * at[2];
-  r1.x = (vt[0].x * ap[0] + vt[1].x * ap[1]) + vt[2].x * ap[2];
-  r1.y = (vt[0].y * ap[0] + vt[1].y * ap[1]) + vt[2].y * ap[2];
+  rad_eval(geo[s], kap + 3 * k, ath + 3 * k, aph + 3 * k, ka, r0, r1);
   rad[((size_t)s * 2 + 0) * nk + k] = r0;
   rad[((size_t)s * 2 + 1) * nk + k] = r1;
 }
@@ -193,16 +186,19 @@ __global__ void k_near(int ne, const int* __restrict__ ia, const int* __restrict
 int launch_setup_rad(Handle* h) {
   const LevelPlan& lf = h->plan.lv[h->L];
   DevLevel& d = h->lv[h->L];
-  double *ath = nullptr, *aph = nullptr;
-  int rc = upload(h, &ath, lf.ath);
+  int rc = upload(h, &h->d_ath, lf.ath);
   if (rc) return rc;
-  rc = upload(h, &aph, lf.aph);
+  rc = upload(h, &h->d_aph, lf.aph);
   if (rc) return rc;
+  rc = dmalloc(h, &h->d_rgeo, (size_t)h->ne);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_rad_geom, dim3((unsigned)((h->ne + 255) / 256)), dim3(256), 0, h->stream, h->ne,
+                     h->d_slot_basis, h->d_basis_box, d.centre, h->d_tb, h->d_p, h->d_et, h->d_epo, h->d_el,
+                     h->d_rgeo);
   const long long tot = (long long)h->ne * lf.nk;
   const int blk = 256;
   hipLaunchKernelGGL(k_rad, dim3((unsigned)((tot + blk - 1) / blk)), dim3(blk), 0, h->stream, h->ne, lf.nk,
-                     d.kap, ath, aph, d.centre, h->d_slot_basis, h->d_basis_box, h->d_tb, h->d_p, h->d_et,
-                     h->d_epo, h->d_el, (2.0 * kPi) / h->plan.phys.lambda, h->d_rad);
+                     d.kap, h->d_ath, h->d_aph, h->d_rgeo, (2.0 * kPi) / h->plan.phys.lambda, h->d_rad);
   LW_HIP(hipGetLastError());
   return LW_OK;
 }
