@@ -152,10 +152,9 @@ int lw_rcs_sigma(lw_handle *h, int n, const double *theta, const double *phi, do
 int lw_bench_matvec(lw_handle *h, int reps, double *ms, double *stage_ms);
 /* Time `reps` ILU0 triangular solves on device-resident vectors. */
 int lw_bench_lusol(lw_handle *h, int reps, double *ms);
-/* Algorithmic HBM bytes of one matvec as this library moves them: the
- * radiation patterns regenerated from per-basis geometry (instead of the
- * reference's two passes over funcRAD), campo arrays 6x, transfer tables
- * once, near CSR, x and y; and of the two leaf kernels. */
+/* Algorithmic HBM bytes of one matvec in the reference data layout
+ * (pattern arrays twice, campo arrays 6x, transfer tables once, near CSR,
+ * x and y), and of the dominant kernels. */
 int lw_matvec_bytes(lw_handle *h, double *total, double *aggregation, double *receive_near);
 /* Mean time in ms of a device-to-device copy of `bytes` bytes (a multiple of
  * 16) by the library's copy kernel on the handle's stream: the measured

@@ -75,51 +75,26 @@ __device__ __forceinline__ double wave_sum(double v) {
 // lists overlap, then share one XCD's L2.
 
 // Leaf aggregation (modMLFMA.f90:1805-1812): campoC_L(:,:,box) =
-// sum_f funcRAD(:,:,f) x(f), with funcRAD regenerated from the basis
-// geometry (rad_eval: the reference's formula, bit-identical to the tabulated
-// patterns) instead of streaming 2*K_L complex(8) values per basis from HBM.
-// One workgroup per leaf box, one sample per thread; the box's geometry and
-// x values are staged in LDS.
-constexpr int kAggMaxStage = 96;  // bases staged per pass
-
-__global__ void __launch_bounds__(256) k_aggregate(int nk, const int* __restrict__ box_slot,
+// sum_f funcRAD(:,:,f) x(f).  One workgroup per leaf box.
+__global__ void __launch_bounds__(256) k_aggregate(int nk2, const int* __restrict__ box_slot,
                                                    const int* __restrict__ slot_basis,
-                                                   const RadGeom* __restrict__ geo, const double* __restrict__ kap,
-                                                   const double* __restrict__ ath, const double* __restrict__ aph,
-                                                   double ka, const double2* __restrict__ x,
-                                                   double2* __restrict__ C) {
-  __shared__ RadGeom gs[kAggMaxStage];
-  __shared__ double2 xs[kAggMaxStage];
+                                                   const double2* __restrict__ rad,
+                                                   const double2* __restrict__ x, double2* __restrict__ C) {
   const int b = xcd_block(blockIdx.x, gridDim.x);
   const int s0 = box_slot[b], s1 = box_slot[b + 1];
-  const int nk2 = 2 * nk;
-  for (int k0 = 0; k0 < nk; k0 += blockDim.x) {
-    const int k = k0 + threadIdx.x;
-    double2 a0 = make_double2(0.0, 0.0), a1 = make_double2(0.0, 0.0);
-    for (int c0 = s0; c0 < s1; c0 += kAggMaxStage) {
-      const int nc = min(kAggMaxStage, s1 - c0);
-      __syncthreads();
-      for (int j = threadIdx.x; j < nc; j += blockDim.x) {
-        gs[j] = geo[c0 + j];
-        xs[j] = x[slot_basis[c0 + j]];
-      }
-      __syncthreads();
-      if (k < nk) {
-        const double* vk = kap + 3 * k;
-        const double* at = ath + 3 * k;
-        const double* ap = aph + 3 * k;
-        for (int j = 0; j < nc; ++j) {
-          double2 r0, r1;
-          rad_eval(gs[j], vk, at, ap, ka, r0, r1);
-          a0 = cadd(a0, cmul(r0, xs[j]));
-          a1 = cadd(a1, cmul(r1, xs[j]));
-        }
-      }
+  // two samples per thread per pass: independent accumulators keep more
+  // pattern loads in flight
+  for (int e = threadIdx.x; e < nk2; e += 2 * blockDim.x) {
+    const int e2 = e + blockDim.x;
+    const bool two = e2 < nk2;
+    double2 acc = make_double2(0.0, 0.0), acc2 = make_double2(0.0, 0.0);
+    for (int s = s0; s < s1; ++s) {
+      const double2 xv = x[slot_basis[s]];
+      acc = cadd(acc, cmul(rad[(size_t)s * nk2 + e], xv));
+      if (two) acc2 = cadd(acc2, cmul(rad[(size_t)s * nk2 + e2], xv));
     }
-    if (k < nk) {
-      C[(size_t)b * nk2 + k] = a0;
-      C[(size_t)b * nk2 + nk + k] = a1;
-    }
+    C[(size_t)b * nk2 + e] = acc;
+    if (two) C[(size_t)b * nk2 + e2] = acc2;
   }
 }
 
@@ -296,15 +271,11 @@ __global__ void __launch_bounds__(512) k_interp_down_lds(int nkc, int nkf, const
 }
 
 // Receive (modMLFMA.f90:1915-1933, funcREC = conjg(funcRAD) under EFIE) and
-// near-field CSR product (:1935-1942), one wavefront per row.  funcREC is
-// regenerated from the basis geometry (rad_eval), both polarisations of a
-// sample from one pair of phase factors.
+// near-field CSR product (:1935-1942), one wavefront per row.
 template <typename V>
-__global__ void __launch_bounds__(256) k_receive_near(int ne, int nk, const int* __restrict__ slot_basis,
+__global__ void __launch_bounds__(256) k_receive_near(int ne, int nk2, const int* __restrict__ slot_basis,
                                                       const int* __restrict__ basis_box,
-                                                      const RadGeom* __restrict__ geo,
-                                                      const double* __restrict__ kap, const double* __restrict__ ath,
-                                                      const double* __restrict__ aph, double ka,
+                                                      const double2* __restrict__ rad,
                                                       const double2* __restrict__ AL, const int* __restrict__ ia,
                                                       const int* __restrict__ ja, const V* __restrict__ z,
                                                       const double2* __restrict__ x, double2 cc,
@@ -315,15 +286,10 @@ __global__ void __launch_bounds__(256) k_receive_near(int ne, int nk, const int*
   const int lane = threadIdx.x % kWave;
   if (slot >= ne) return;
   const int m = slot_basis[slot];
-  const RadGeom g = geo[slot];
-  const double2* a = AL + (size_t)basis_box[m] * 2 * nk;
+  const double2* r = rad + (size_t)slot * nk2;
+  const double2* a = AL + (size_t)basis_box[m] * nk2;
   double2 f = make_double2(0.0, 0.0);
-  for (int k = lane; k < nk; k += kWave) {
-    double2 r0, r1;
-    rad_eval(g, kap + 3 * k, ath + 3 * k, aph + 3 * k, ka, r0, r1);
-    f = cadd(f, cmulc(r0, a[k]));
-    f = cadd(f, cmulc(r1, a[nk + k]));
-  }
+  for (int e = lane; e < nk2; e += kWave) f = cadd(f, cmulc(r[e], a[e]));
   f = wave_sum(f);
   double2 nsum = make_double2(0.0, 0.0);
   for (int k = ia[m] + lane; k < ia[m + 1]; k += kWave) nsum = cadd(nsum, cmul(x[ja[k]], to_d2(z[k])));
@@ -606,9 +572,8 @@ int run_matvec(Handle* h, const double2* x, double2* y, double2* y_near, hipEven
   hipStream_t st = h->stream;
   DevLevel& lf = h->lv[L];
   if (ev) LW_HIP(hipEventRecord(ev[0], st));
-  const double ka = 2.0 * kPi / h->plan.phys.lambda;
-  hipLaunchKernelGGL(k_aggregate, dim3(lf.nb), dim3(256), 0, st, lf.nk, h->d_box_slot, h->d_slot_basis, h->d_rgeo,
-                     lf.kap, h->d_ath, h->d_aph, ka, x, lf.C);
+  hipLaunchKernelGGL(k_aggregate, dim3(lf.nb), dim3(256), 0, st, 2 * lf.nk, h->d_box_slot, h->d_slot_basis,
+                     h->d_rad, x, lf.C);
   if (ev) LW_HIP(hipEventRecord(ev[1], st));
   for (int n = L; n >= 4; --n) {
     DevLevel& c = h->lv[n];
@@ -642,13 +607,11 @@ int run_matvec(Handle* h, const double2* x, double2* y, double2* y_near, hipEven
   const int rows_per_block = 4;
   dim3 grid(blocks_for(h->ne, rows_per_block)), block(kWave * rows_per_block);
   if (h->fp64_near)
-    hipLaunchKernelGGL(k_receive_near<double2>, grid, block, 0, st, h->ne, lf.nk, h->d_slot_basis, h->d_basis_box,
-                       h->d_rgeo, lf.kap, h->d_ath, h->d_aph, ka, lf.A, h->d_ia, h->d_ja, (const double2*)h->d_z, x,
-                       cc, y, y_near);
+    hipLaunchKernelGGL(k_receive_near<double2>, grid, block, 0, st, h->ne, 2 * lf.nk, h->d_slot_basis,
+                       h->d_basis_box, h->d_rad, lf.A, h->d_ia, h->d_ja, (const double2*)h->d_z, x, cc, y, y_near);
   else
-    hipLaunchKernelGGL(k_receive_near<float2>, grid, block, 0, st, h->ne, lf.nk, h->d_slot_basis, h->d_basis_box,
-                       h->d_rgeo, lf.kap, h->d_ath, h->d_aph, ka, lf.A, h->d_ia, h->d_ja, (const float2*)h->d_z, x,
-                       cc, y, y_near);
+    hipLaunchKernelGGL(k_receive_near<float2>, grid, block, 0, st, h->ne, 2 * lf.nk, h->d_slot_basis,
+                       h->d_basis_box, h->d_rad, lf.A, h->d_ia, h->d_ja, (const float2*)h->d_z, x, cc, y, y_near);
   if (ev) LW_HIP(hipEventRecord(ev[5], st));
   LW_HIP(hipGetLastError());
   return LW_OK;
@@ -1399,9 +1362,7 @@ int lw_matvec_bytes(lw_handle* hh, double* total, double* aggregation, double* r
   Handle* h = &hh->h;
   const lw::Plan& P = h->plan;
   const double c16 = 16.0;
-  // the patterns are regenerated from the per-basis geometry (13 doubles) and
-  // the leaf sample tables (9 doubles per sample) instead of read as funcRAD
-  const double pattern = (double)P.ne * sizeof(lw::RadGeom) + 9.0 * 8.0 * P.lv[P.L].nk;
+  const double pattern = (double)P.lv[P.L].nk * 2 * P.ne * c16;  // one pass over funcRAD
   double campo = 0.0, trans = 0.0;
   for (int n = 3; n <= P.L; ++n) {
     campo += (double)P.lv[n].nk * 2 * P.lv[n].nb * c16;
@@ -1425,7 +1386,7 @@ int lw_matvec_stage_bytes(lw_handle* hh, double* sb) {
   const lw::Plan& P = h->plan;
   const int L = P.L;
   auto field = [&](int n) { return (double)P.lv[n].nk * 2 * P.lv[n].nb * 16.0; };
-  const double pattern = (double)P.ne * sizeof(lw::RadGeom) + 9.0 * 8.0 * P.lv[L].nk;  // geometry, not funcRAD
+  const double pattern = (double)P.lv[L].nk * 2 * P.ne * 16.0;
   const double zb = h->fp64_near ? 16.0 : 8.0;
   sb[0] = pattern + 16.0 * P.ne + field(L) + 8.0 * P.ne;
   sb[1] = 0.0;
