@@ -2,15 +2,16 @@
 // operator.
 //
 //   far field (aggregation, interpolation, translation, anterpolation,
-//   receive) in FP32 complex with FP32 accumulation;
+//   receive) in FP32 complex with FP32 accumulation, or in FP64 (the same
+//   kernels instantiated for double);
 //   near field (complex(4)-stored values, FP64 accumulation), ILU0 solves and
 //   every CGS vector operation and reduction in FP64.
 //
 // Layouts (r = right-hand side, fastest):
 //   batch vectors      V[i*R + r]                  (complex(8))
-//   campo arrays       F[((k*nb + b)*2 + p)*R + r] (complex(4), sample-major so
-//                      one sample's slice of a level is contiguous for the
-//                      translation kernel)
+//   campo arrays       F[((k*nb + b)*2 + p)*R + r] (far-field precision,
+//                      sample-major so one sample's slice of a level is
+//                      contiguous for the translation kernel)
 //   patterns           rad[(s*2 + p)*nk + k]       (leaf slot order)
 // The per-RHS CGS follows CGSold (modIterativo.f90:331-493) column by column:
 // a column that meets its tolerance (or breaks down) is frozen, so every
@@ -285,38 +286,143 @@ __global__ void __launch_bounds__(256) k_b_interp_up(int ncol, int nthc, int nkc
 }
 
 // Translation fused with the downward pass, one wave per (sample k, parent
-// box P, 64-column chunk) at level n:
+// box P, 64*kFarCpl-column chunk) at level n:
 //   A_n[k][c] = sum_{far d} pes[k] T(off(c,d))[k] C_n[k][d]
 //             + sum_{(i,w) in W^T(k)} w conj(ph_{n-1}(oct_c, i)) A_{n-1}[i][P]
 // for the children c of P.  The far boxes of every child of P are the
 // children of P's neighbour parents Q that do not touch it, so a wave loads
-// each source row C_n[k][d] once and applies it to all (up to 8) children of
-// P; the transfer function of a pair depends only on the integer offset
+// each source row C_n[k][d] once and applies it to all children of P; the
+// transfer function of a pair depends only on the integer offset
 // off = 2(Q - P) + oct_d - oct_c in box units, so the k-major table Tk holds
 // one entry per offset code (pes folded in).  Parent-level data come in
 // through the combined weights wp[oct][e] = w_e conj(ph(oct, row_e)).
-// Workgroups that share a sample run on the same XCD (workgroup id mod 8),
-// where the k-slice of C_n and the Tk row stay in L2.
+//
+// Per parent the sources are one flat list of (box d, code part
+// 49 (2ox+3+bx_d) + 7 (2oy+3+by_d) + (2oz+3+bz_d) - 57), padded to a multiple
+// of kSrcU with entries whose codes are near offsets (T = 0).  Everything that
+// selects a transfer value (k, P, the source entry, the child's octant) is
+// uniform over the wave, so the values come in through scalar loads from the
+// Tk row and enter the FMAs as scalar operands; the children of P are
+// compacted (nch = 1..8 accumulators, one code path per count), so no FMA is
+// spent on absent octants and the inner loop has no branches.  Source rows
+// are loaded one group of kSrcU ahead.  Workgroups that share a sample run on
+// the same XCD (workgroup id mod 8), where the k-slice of C_n stays in L2.
 constexpr int kOffCodes = 343;  // (dx, dy, dz) in [-3, 3]^3
+constexpr int kSrcU = 4;        // sources per pipelined group
+constexpr int kFarCpl = 2;      // columns per lane
+// 49*2 + 7*2 + 2: plus any child's octant part 57 - oct_code(o), an offset in
+// [-1, 1]^3, which has no transfer vector (T = 0)
+constexpr int kPadCode = 114;
 
 // Offset-code part of a child octant o (bits x,y,z = 4,2,1): 49 bx + 7 by + bz.
 __host__ __device__ constexpr int oct_code(int o) { return (o >> 2) * 49 + ((o >> 1) & 1) * 7 + (o & 1); }
 
-template <typename S>
-__global__ void __launch_bounds__(256) k_b_far(int nk, int nb, int nP, int ncol, int nchunk, int wpx, int split_p,
-                                               const int* __restrict__ pc_ptr, const int* __restrict__ p_mask,
-                                               const int* __restrict__ p_child, const int* __restrict__ q_ptr,
-                                               const int2* __restrict__ q_ent, const int* __restrict__ bcd,
-                                               const typename Cx<S>::T* __restrict__ Tk,
-                                               const typename Cx<S>::T* __restrict__ C, int nbc, int nwt,
-                                               const int* __restrict__ wt_ptr, const int* __restrict__ wt_row,
-                                               const typename Cx<S>::T* __restrict__ wp,
-                                               const typename Cx<S>::T* __restrict__ Ac,
-                                               typename Cx<S>::T* __restrict__ A) {
+__device__ __forceinline__ float fmadd(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+__device__ __forceinline__ double fmadd(double a, double b, double c) { return __builtin_fma(a, b, c); }
+// acc += t v as four FMAs
+template <typename T2>
+__device__ __forceinline__ void cmac(T2& acc, const T2 t, const T2 v) {
+  acc.x = fmadd(t.x, v.x, acc.x);
+  acc.x = fmadd(-t.y, v.y, acc.x);
+  acc.y = fmadd(t.x, v.y, acc.y);
+  acc.y = fmadd(t.y, v.x, acc.y);
+}
+
+#define LW_FAR_PARAMS                                                                                 \
+  int nb, int ncol, int nbc, int nwt, const int* __restrict__ pc_ptr, const int* __restrict__ bcd,     \
+      const int* __restrict__ boct, const int* __restrict__ ps_ptr, const int2* __restrict__ ps_ent,   \
+      const typename Cx<S>::T* __restrict__ Tk, const typename Cx<S>::T* __restrict__ C,                \
+      const int* __restrict__ wt_ptr, const int* __restrict__ wt_row,                                  \
+      const typename Cx<S>::T* __restrict__ wp, const typename Cx<S>::T* __restrict__ Ac,              \
+      typename Cx<S>::T* __restrict__ A
+#define LW_FAR_ARGS nb, ncol, nbc, nwt, pc_ptr, bcd, boct, ps_ptr, ps_ent, Tk, C, wt_ptr, wt_row, wp, Ac, A
+
+template <typename S, int NCH>
+__device__ __forceinline__ void far_parent(int k, int P, int col, LW_FAR_PARAMS) {
   typedef typename Cx<S>::T T2;
-  __shared__ T2 tks[kOffCodes];  // this sample's transfer functions, read as broadcasts
-  // Two workgroup maps (both: within a workgroup one sample k and one
-  // 64-column chunk, 4 waves = 4 consecutive parents):
+  constexpr int kMaxCq = oct_code(7);  // 57
+  const int c0 = pc_ptr[P];
+  int coff[NCH], wo[NCH];
+#pragma unroll
+  for (int i = 0; i < NCH; ++i) {
+    coff[i] = kMaxCq - bcd[c0 + i];
+    wo[i] = boct[c0 + i] * nwt;
+  }
+  T2 acc[NCH][kFarCpl];
+#pragma unroll
+  for (int i = 0; i < NCH; ++i)
+#pragma unroll
+    for (int c = 0; c < kFarCpl; ++c) acc[i][c] = mk<S>(0, 0);
+  const T2* Tkk = Tk + (size_t)k * kOffCodes;
+  const T2* Ck = C + (size_t)k * nb * ncol + col;
+  const int s0 = ps_ptr[P], s1 = ps_ptr[P + 1];
+  if (s0 < s1) {
+    int2 ea[kSrcU], eb[kSrcU];
+    T2 va[kSrcU][kFarCpl], vb[kSrcU][kFarCpl];
+    auto load = [&](int s, int2(&e)[kSrcU], T2(&v)[kSrcU][kFarCpl]) {
+#pragma unroll
+      for (int u = 0; u < kSrcU; ++u) {
+        e[u] = ps_ent[s + u];
+#pragma unroll
+        for (int c = 0; c < kFarCpl; ++c) v[u][c] = Ck[(size_t)e[u].x * ncol + c * kWave];
+      }
+    };
+    auto apply = [&](const int2(&e)[kSrcU], const T2(&v)[kSrcU][kFarCpl]) {
+#pragma unroll
+      for (int u = 0; u < kSrcU; ++u) {
+        const T2* tb = Tkk + e[u].y;
+#pragma unroll
+        for (int i = 0; i < NCH; ++i) {
+          const T2 t = tb[coff[i]];
+#pragma unroll
+          for (int c = 0; c < kFarCpl; ++c) cmac(acc[i][c], t, v[u][c]);
+        }
+      }
+    };
+    load(s0, ea, va);
+    for (int s = s0;;) {
+      if (s + kSrcU < s1) load(s + kSrcU, eb, vb);
+      apply(ea, va);
+      s += kSrcU;
+      if (s >= s1) break;
+      if (s + kSrcU < s1) load(s + kSrcU, ea, va);
+      apply(eb, vb);
+      s += kSrcU;
+      if (s >= s1) break;
+    }
+  }
+  if (Ac) {
+    const int e0 = wt_ptr[k], e1 = wt_ptr[k + 1];
+    // entries come in groups of 4 (padded with zero weights)
+    for (int e = e0; e < e1; e += 4) {
+      T2 a[4][kFarCpl];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) {
+        const T2* ar = Ac + ((size_t)wt_row[e + u] * nbc + P) * ncol + col;
+#pragma unroll
+        for (int c = 0; c < kFarCpl; ++c) a[u][c] = ar[c * kWave];
+      }
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int i = 0; i < NCH; ++i) {
+          const T2 w = wp[wo[i] + e + u];
+#pragma unroll
+          for (int c = 0; c < kFarCpl; ++c) cmac(acc[i][c], w, a[u][c]);
+        }
+    }
+  }
+  T2* Ak = A + (size_t)k * nb * ncol + col;
+#pragma unroll
+  for (int i = 0; i < NCH; ++i)
+#pragma unroll
+    for (int c = 0; c < kFarCpl; ++c) Ak[(size_t)(c0 + i) * ncol + c * kWave] = acc[i][c];
+}
+
+template <typename S>
+__global__ void __launch_bounds__(256) k_b_far(int nk, int nP, int nchunk, int wpx, int split_p, LW_FAR_PARAMS) {
+  // Two workgroup maps (both: within a workgroup one sample k and one column
+  // chunk, 4 waves = 4 consecutive parents):
   //  split_p: XCD x owns the contiguous parent range [x*nPx, (x+1)*nPx) for
   //           every sample; its L2 then holds the sources of a box range
   //           (~1/8 of a k-slice plus a halo) instead of whole k-slices;
@@ -339,67 +445,20 @@ __global__ void __launch_bounds__(256) k_b_far(int nk, int nb, int nP, int ncol,
     P1 = nP;
   }
   if (k >= nk) return;  // uniform over the workgroup
-  for (int e = threadIdx.x; e < kOffCodes; e += blockDim.x) tks[e] = Tk[(size_t)k * kOffCodes + e];
-  __syncthreads();
   const int P = P0 + __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
   if (P >= P1) return;
-  const int col = chunk * kWave + (threadIdx.x % kWave);
-  // accumulators by child octant: the code offset of an octant is then a
-  // compile-time constant, so each transfer value is one LDS broadcast read
-  // at an immediate offset from the source's code; absent octants are
-  // skipped by the (uniform) mask test and never stored
-  const int mask = p_mask[P];
-  T2 acc[8];
-#pragma unroll
-  for (int o = 0; o < 8; ++o) acc[o] = mk<S>(0, 0);
-  const T2* Ck = C + (size_t)k * nb * ncol + col;
-  constexpr int kMaxCq = oct_code(7);  // 57: code - oct_code(o) >= 0 for every source
-  for (int e = q_ptr[P]; e < q_ptr[P + 1]; ++e) {
-    const int2 qe = q_ent[e];  // (Q, 49 (2ox+3) + 7 (2oy+3) + (2oz+3))
-    const int d0 = pc_ptr[qe.x], nd = pc_ptr[qe.x + 1] - d0;
-    T2 v[8];
-#pragma unroll
-    for (int jd = 0; jd < 8; ++jd) v[jd] = jd < nd ? Ck[(size_t)(d0 + jd) * ncol] : mk<S>(0, 0);
-#pragma unroll
-    for (int jd = 0; jd < 8; ++jd) {
-      if (jd < nd) {
-        const T2* tb = tks + (qe.y + bcd[d0 + jd] - kMaxCq);
-#pragma unroll
-        for (int o = 0; o < 8; ++o) {
-          if (mask & (1 << o)) {
-            const T2 tt = tb[kMaxCq - oct_code(o)];
-            acc[o].x += tt.x * v[jd].x - tt.y * v[jd].y;
-            acc[o].y += tt.x * v[jd].y + tt.y * v[jd].x;
-          }
-        }
-      }
-    }
+  const int col = chunk * kWave * kFarCpl + (threadIdx.x % kWave);
+  switch (pc_ptr[P + 1] - pc_ptr[P]) {
+    case 1: far_parent<S, 1>(k, P, col, LW_FAR_ARGS); break;
+    case 2: far_parent<S, 2>(k, P, col, LW_FAR_ARGS); break;
+    case 3: far_parent<S, 3>(k, P, col, LW_FAR_ARGS); break;
+    case 4: far_parent<S, 4>(k, P, col, LW_FAR_ARGS); break;
+    case 5: far_parent<S, 5>(k, P, col, LW_FAR_ARGS); break;
+    case 6: far_parent<S, 6>(k, P, col, LW_FAR_ARGS); break;
+    case 7: far_parent<S, 7>(k, P, col, LW_FAR_ARGS); break;
+    case 8: far_parent<S, 8>(k, P, col, LW_FAR_ARGS); break;
+    default: break;
   }
-  if (Ac) {
-    const int e0 = wt_ptr[k], e1 = wt_ptr[k + 1];
-    for (int e = e0; e < e1; e += 4) {
-      T2 a[4];
-#pragma unroll
-      for (int u = 0; u < 4; ++u) a[u] = e + u < e1 ? Ac[((size_t)wt_row[e + u] * nbc + P) * ncol + col] : mk<S>(0, 0);
-#pragma unroll
-      for (int u = 0; u < 4; ++u) {
-        if (e + u < e1) {
-#pragma unroll
-          for (int o = 0; o < 8; ++o) {
-            if (mask & (1 << o)) {
-              const T2 w = wp[(size_t)o * nwt + e + u];
-              acc[o].x += w.x * a[u].x - w.y * a[u].y;
-              acc[o].y += w.x * a[u].y + w.y * a[u].x;
-            }
-          }
-        }
-      }
-    }
-  }
-  T2* Ak = A + (size_t)k * nb * ncol + col;
-#pragma unroll
-  for (int o = 0; o < 8; ++o)
-    if (mask & (1 << o)) Ak[(size_t)p_child[8 * P + o] * ncol] = acc[o];
 }
 
 // Tk[k][code] = pes[k] T[tr(code)][k] (zero for codes without a transfer vector)
@@ -915,13 +974,13 @@ struct BatchLevel {
   // fused translation + downward pass at this level (parents at level n-1)
   int nP = 0, nwt = 0;
   int* pc_ptr = nullptr;  // (nP+1) children of each parent
-  int* p_mask = nullptr;  // (nP) child octants present
-  int* p_child = nullptr; // (8, nP) child box of each octant
   int* bcd = nullptr;     // (nb) offset-code part of each box's octant
-  int* q_ptr = nullptr;   // (nP+1) neighbour parents holding far boxes
-  int2* q_ent = nullptr;  // (Q, 49 (2ox+3) + 7 (2oy+3) + (2oz+3)) for parent offset o = Q - P
+  int* boct = nullptr;    // (nb) octant of each box within its parent
+  int* ps_ptr = nullptr;  // (nP+1) flat source list of each parent, a multiple of kSrcU entries
+  int2* ps_ent = nullptr; // (box d, offset-code part of the pair minus 57)
   void* Tk = nullptr;     // (nk, 343) pes-scaled transfer function per offset code
-  void* wp = nullptr;     // (8, nwt) anterpolation weights from level n-1 (n > 3)
+  int *wtp_ptr = nullptr, *wtp_row = nullptr;  // transposed stencil from level n-1, padded (n > 3)
+  void* wp = nullptr;     // (8, nwt) anterpolation weights of the padded entries (n > 3)
   void *C = nullptr, *A = nullptr;
 };
 
@@ -982,6 +1041,8 @@ struct FarBlocks {
   std::vector<int> bo, q_ptr, code_tr;
   std::vector<int2> q_ent;
   std::vector<int> p_mask, p_child, bcd;  // per parent: child octants present, child per octant; per box: code part
+  std::vector<int> ps_ptr;                // per parent: flat padded source list (see k_b_far)
+  std::vector<int2> ps_ent;
 };
 
 // Derives the parent-pair blocks from the plan's per-box far lists and checks
@@ -1079,6 +1140,23 @@ static bool far_blocks(const Plan& P, int n, FarBlocks& fb, std::string& msg) {
   }
   fb.bcd.resize(nb);
   for (int c = 0; c < nb; ++c) fb.bcd[c] = oct_code(fb.bo[c]);
+  // flat source lists, padded with entries whose codes are near offsets
+  for (int o = 0; o < 8; ++o)
+    if (fb.code_tr[kPadCode + oct_code(7) - oct_code(o)] >= 0) {
+      msg = "padding code has a transfer vector";
+      return false;
+    }
+  fb.ps_ptr.assign(nP + 1, 0);
+  fb.ps_ent.clear();
+  for (int Pb = 0; Pb < nP; ++Pb) {
+    for (int e = fb.q_ptr[Pb]; e < fb.q_ptr[Pb + 1]; ++e) {
+      const int Q = fb.q_ent[e].x;
+      for (int d = pp.child_ptr[Q]; d < pp.child_ptr[Q + 1]; ++d)
+        fb.ps_ent.push_back(make_int2(d, fb.q_ent[e].y + fb.bcd[d] - oct_code(7)));
+    }
+    while ((fb.ps_ent.size() - fb.ps_ptr[Pb]) % kSrcU) fb.ps_ent.push_back(make_int2(pp.child_ptr[Pb], kPadCode));
+    fb.ps_ptr[Pb + 1] = (int)fb.ps_ent.size();
+  }
   fb.p_mask.assign(nP, 0);
   fb.p_child.assign(8 * (size_t)nP, 0);
   for (int Pb = 0; Pb < nP; ++Pb)
@@ -1151,12 +1229,12 @@ static int batch_matvec_t(Batch* bt, const double2* X, double2* Y) {
     // workgroups per (XCD, sample) with the parents split over the XCDs, or
     // per sample when there are too few parents to split
     const bool split_p = d.nP >= 64;
-    const int wpx = split_p ? nblk(nblk(d.nP, 8), 4) * nchunk : nblk(d.nP, 4) * nchunk;
+    const int fnchunk = ncol / (kWave * kFarCpl);
+    const int wpx = split_p ? nblk(nblk(d.nP, 8), 4) * fnchunk : nblk(d.nP, 4) * fnchunk;
     const long long grid = split_p ? 8LL * d.nk * wpx : 8LL * nblk(d.nk, 8) * wpx;
-    hipLaunchKernelGGL(k_b_far<S>, dim3((unsigned)grid), dim3(256), 0, st, d.nk, d.nb, d.nP, ncol, nchunk, wpx,
-                       split_p ? 1 : 0, d.pc_ptr, d.p_mask, d.p_child, d.q_ptr, d.q_ent, d.bcd, (const T2*)d.Tk,
-                       (const T2*)d.C, pl ? pl->nb : 0, d.nwt,
-                       pl ? h->lv[n - 1].wt_ptr : nullptr, pl ? h->lv[n - 1].wt_row : nullptr, (const T2*)d.wp,
+    hipLaunchKernelGGL(k_b_far<S>, dim3((unsigned)grid), dim3(256), 0, st, d.nk, d.nP, fnchunk, wpx, split_p ? 1 : 0,
+                       d.nb, ncol, pl ? pl->nb : 0, d.nwt, d.pc_ptr, d.bcd, d.boct, d.ps_ptr, d.ps_ent,
+                       (const T2*)d.Tk, (const T2*)d.C, d.wtp_ptr, d.wtp_row, (const T2*)d.wp,
                        pl ? (const T2*)pl->A : nullptr, (T2*)d.A);
   }
   if (R == 64)
@@ -1396,10 +1474,9 @@ static int setup_far(Batch* bt) {
     }
     b.nP = P.lv[n - 1].nb;
     int* d_code_tr = nullptr;
-    if ((rc = upload_b(bt, &b.pc_ptr, P.lv[n - 1].child_ptr)) || (rc = upload_b(bt, &b.p_mask, fb.p_mask)) ||
-        (rc = upload_b(bt, &b.p_child, fb.p_child)) || (rc = upload_b(bt, &b.bcd, fb.bcd)) ||
-        (rc = upload_b(bt, &b.q_ptr, fb.q_ptr)) || (rc = upload_b(bt, &b.q_ent, fb.q_ent)) ||
-        (rc = upload_b(bt, &d_code_tr, fb.code_tr)))
+    if ((rc = upload_b(bt, &b.pc_ptr, P.lv[n - 1].child_ptr)) || (rc = upload_b(bt, &b.bcd, fb.bcd)) ||
+        (rc = upload_b(bt, &b.boct, fb.bo)) || (rc = upload_b(bt, &b.ps_ptr, fb.ps_ptr)) ||
+        (rc = upload_b(bt, &b.ps_ent, fb.ps_ent)) || (rc = upload_b(bt, &d_code_tr, fb.code_tr)))
       return rc;
     T2* tk = nullptr;
     if ((rc = bmalloc(bt, &tk, (size_t)lp.nk * kOffCodes))) return rc;
@@ -1407,12 +1484,30 @@ static int setup_far(Batch* bt) {
                        d_code_tr, d.T, d.pes, tk);
     b.Tk = tk;
     if (n > 3) {
+      // the transposed stencil with every fine sample's entries padded to a
+      // multiple of 4 (weight 0, row 0)
       const LevelPlan& pp = P.lv[n - 1];
-      b.nwt = (int)pp.wt_w.size();
+      std::vector<int> wptr(lp.nk + 1, 0), wrow;
+      std::vector<double> ww;
+      for (int k = 0; k < lp.nk; ++k) {
+        for (int e = pp.wt_ptr[k]; e < pp.wt_ptr[k + 1]; ++e) {
+          wrow.push_back(pp.wt_row[e]);
+          ww.push_back(pp.wt_w[e]);
+        }
+        while ((wrow.size() - wptr[k]) % 4) {
+          wrow.push_back(0);
+          ww.push_back(0.0);
+        }
+        wptr[k + 1] = (int)wrow.size();
+      }
+      b.nwt = (int)ww.size();
+      double* d_ww = nullptr;
       T2* wp = nullptr;
-      if ((rc = bmalloc(bt, &wp, 8 * (size_t)b.nwt))) return rc;
-      hipLaunchKernelGGL(k_b_wp<S>, dim3(nblk(8LL * b.nwt, 256)), dim3(256), 0, st, pp.nk, b.nwt,
-                         h->lv[n - 1].wt_row, h->lv[n - 1].wt_w, h->lv[n - 1].ph, wp);
+      if ((rc = upload_b(bt, &b.wtp_ptr, wptr)) || (rc = upload_b(bt, &b.wtp_row, wrow)) ||
+          (rc = upload_b(bt, &d_ww, ww)) || (rc = bmalloc(bt, &wp, 8 * (size_t)b.nwt)))
+        return rc;
+      hipLaunchKernelGGL(k_b_wp<S>, dim3(nblk(8LL * b.nwt, 256)), dim3(256), 0, st, pp.nk, b.nwt, b.wtp_row, d_ww,
+                         h->lv[n - 1].ph, wp);
       b.wp = wp;
     }
     const size_t fsz = (size_t)lp.nk * lp.nb * 2 * R;
