@@ -63,8 +63,10 @@ def main():
     ap.add_argument("--batch-prec", choices=["fp64", "fp32"], default="fp64",
                     help="far-field precision of the batched sweep")
     ap.add_argument("--sweep", type=int, default=0, help="angles of the batched sweep (default: chain x columns)")
-    ap.add_argument("--guess", choices=["zero", "warm", "mlp"], default="mlp",
-                    help="initial guess of each later angle of a chain (mlp falls back to warm without weights)")
+    ap.add_argument("--guess", choices=["zero", "warm", "extrap", "mlp"], default="mlp",
+                    help="initial guess of each later angle of a chain: zero, the reference's warm start, the linear "
+                         "extrapolation of two warm starts, or the learned guess (mlp falls back to warm without "
+                         "weights)")
     ap.add_argument("--fp32-probe-iters", type=int, default=300,
                     help="batched CGS iterations of the FP32-far-field probe (0: skip)")
     ap.add_argument("--cpu-iters", type=int, default=3)

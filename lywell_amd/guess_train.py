@@ -14,8 +14,12 @@ by a new draw.  Every
 angle a >= 2 of the sweep becomes one training sample per unknown, with the
 converged currents of angles a-1 and a-2 as inputs and that of angle a as
 the target.  The network (16 -> 256 x 3 -> 4) is trained by the library's
-own forward/backward kernels (lw_guess_train: Adam on random minibatches).
-The weights and a held-out evaluation are written to `out`.
+own forward/backward kernels (lw_guess_train: Adam on random minibatches),
+`steps` steps at `lr` and then `steps2` at `lr2`.  Its output layer starts
+at zero weights with the bias of o2 at `init`: 1 makes the untrained network
+the linear extrapolation 2 xt1 - xt2 of the two phase-corrected neighbours,
+0 the reference's warm start xt1.  The weights and a held-out evaluation are
+written to `out`.
 """
 import argparse
 import json
@@ -74,6 +78,17 @@ def solve_sweep(op, n, pol, errmax, maxit, R=64):
     return phi, dirs, pols, X, its
 
 
+def init_head(net, extrapolation=1.0):
+    """Output layer at zero weights, bias (0, 0, extrapolation, 0): the
+    untrained guess is xt1 + extrapolation (xt1 - xt2)."""
+    p = net.get_params()
+    W, b = net.shapes()[-1]
+    nW = W[0] * W[1]
+    p[-(nW + b[0]):] = 0.0
+    p[-b[0] + 2] = extrapolation
+    net.set_params(p)
+
+
 def samples(net, op, dirs, pols, X):
     """Angles 2..n-1 of a solved sweep as training rows."""
     n = X.shape[1]
@@ -107,8 +122,11 @@ def main(argv=None):
     ap.add_argument("--errmax", type=float, default=1e-5)
     ap.add_argument("--maxit", type=int, default=4000)
     ap.add_argument("--steps", type=int, default=3000)
+    ap.add_argument("--steps2", type=int, default=2000)
     ap.add_argument("--batch-rows", type=int, default=16384)
     ap.add_argument("--lr", type=float, default=1e-3)
+    ap.add_argument("--lr2", type=float, default=2e-4)
+    ap.add_argument("--init", type=float, default=1.0, help="initial bias of o2 (1: extrapolation, 0: warm start)")
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--out", default=DEFAULT_OUT)
     args = ap.parse_args(argv)
@@ -116,6 +134,7 @@ def main(argv=None):
         raise SystemExit("guess_train: no HIP device")
     rng = np.random.default_rng(args.seed)
     net = lw.Mlp(16, 256, 3, 4, seed=args.seed)
+    init_head(net, args.init)
     bodies, skipped = [], 0
     subdivs = [int(v) for v in str(args.subdiv).split(",")]
     t0 = time.time()
@@ -146,7 +165,10 @@ def main(argv=None):
             held = (op, dirs, pols, X, meta)
         i += 1
     loss = net.train(args.steps, batch_rows=args.batch_rows, lr=args.lr, seed=args.seed)
-    log(f"trained {args.steps} steps: loss {loss[:10].mean():.4e} -> {loss[-50:].mean():.4e}")
+    if args.steps2 > 0:
+        loss = np.concatenate([loss, net.train(args.steps2, batch_rows=args.batch_rows, lr=args.lr2,
+                                               seed=args.seed + 1)])
+    log(f"trained {loss.size} steps: loss {loss[:10].mean():.4e} -> {loss[-50:].mean():.4e}")
     op, dirs, pols, X, meta = held
     err = guess_errors(net, op, dirs, pols, X)
     ev = {k: float(np.median(v)) for k, v in err.items()}

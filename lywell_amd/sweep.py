@@ -11,8 +11,9 @@ the previous angle's current times exp(-j k (d - d_prev) . e_centro)
 (main.f90:1017-1022).  Here R chains of consecutive angles run side by side
 in one batched solve: column c of block j solves angle c*nblocks + j, so
 every chain keeps the reference's one-step warm start.  The guess of a block
-is zero (first block), that warm start, or the learned guess (from the two
-previous angles of the chain).
+is zero (first block), that warm start, the linear extrapolation of the two
+previous angles' warm starts, or the learned guess (from the two previous
+angles of the chain).
 """
 import time
 
@@ -50,8 +51,8 @@ class Sweep:
     (lw_batch_restart_column), so no column waits for the block's slowest."""
 
     def __init__(self, op, batch, errmax=1e-4, maxit=20000, guess="warm", mlp=None, pol="VV", log=None):
-        if guess not in ("zero", "warm", "mlp"):
-            raise ValueError("guess: zero, warm or mlp")
+        if guess not in ("zero", "warm", "extrap", "mlp"):
+            raise ValueError("guess: zero, warm, extrap or mlp")
         if guess == "mlp" and mlp is None:
             raise ValueError("the learned guess needs a trained network")
         self.op, self.bt = op, batch
@@ -84,6 +85,9 @@ class Sweep:
                     G = self.mlp.predict_guess(self.op, d, p, d1, x1, d2, x2)
                 else:
                     G = warm_start(x1, self.op.mesh.e_centro, self.k0, d, d1)
+                    if self.guess == "extrap" and j >= 2:
+                        a2, d2, x2 = hist[-2]
+                        G = 2.0 * G - warm_start(x2, self.op.mesh.e_centro, self.k0, d, d2)
             tb = time.time()
             X, it, rr, cv = self.bt.cgs(B, guess=G, errmax=self.errmax, use_prec=False, maxit=self.maxit)
             s = self.bt.rcs(X, p.astype(np.complex128), np.full(R, np.pi / 2), phi[ang])
@@ -110,7 +114,13 @@ class Sweep:
         if self.guess == "mlp" and len(hist) >= 2:
             d2, x2 = hist[-2]
             return self.mlp.predict_guess(self.op, d, p, d1, x1, d2, x2)[:, 0]
-        return warm_start(x1[:, None], self.op.mesh.e_centro, self.k0, d[None, :], d1[None, :])[:, 0]
+        xt1 = warm_start(x1[:, None], self.op.mesh.e_centro, self.k0, d[None, :], d1[None, :])[:, 0]
+        if self.guess == "extrap" and len(hist) >= 2:
+            # linear extrapolation of the two phase-corrected neighbours
+            d2, x2 = hist[-2]
+            xt2 = warm_start(x2[:, None], self.op.mesh.e_centro, self.k0, d[None, :], d2[None, :])[:, 0]
+            return 2.0 * xt1 - xt2
+        return xt1
 
     def run_persistent(self, n, keep_currents=False, check_every=8):
         R = self.bt.R
