@@ -354,6 +354,7 @@ __device__ __forceinline__ void far_parent(int k, int P, int col, LW_FAR_PARAMS)
 #pragma unroll
     for (int c = 0; c < kFarCpl; ++c) acc[i][c] = mk<S>(0, 0);
   const T2* Tkk = Tk + (size_t)k * kOffCodes;
+  // offsets within a level's sample slice fit 32 bits (checked at set-up)
   const T2* Ck = C + (size_t)k * nb * ncol + col;
   const int s0 = ps_ptr[P], s1 = ps_ptr[P + 1];
   if (s0 < s1) {
@@ -364,7 +365,7 @@ __device__ __forceinline__ void far_parent(int k, int P, int col, LW_FAR_PARAMS)
       for (int u = 0; u < kSrcU; ++u) {
         e[u] = ps_ent[s + u];
 #pragma unroll
-        for (int c = 0; c < kFarCpl; ++c) v[u][c] = Ck[(size_t)e[u].x * ncol + c * kWave];
+        for (int c = 0; c < kFarCpl; ++c) v[u][c] = Ck[(unsigned)(e[u].x * ncol + c * kWave)];
       }
     };
     auto apply = [&](const int2(&e)[kSrcU], const T2(&v)[kSrcU][kFarCpl]) {
@@ -379,29 +380,39 @@ __device__ __forceinline__ void far_parent(int k, int P, int col, LW_FAR_PARAMS)
         }
       }
     };
+    // two buffers: the next group (or the last group again at the end, so
+    // that the loads are always in bounds and need no branch) is loaded
+    // before the current group's FMAs
+    // (the scheduling barriers keep the compiler from sinking the loads
+    // back down to their uses)
     load(s0, ea, va);
     for (int s = s0;;) {
-      if (s + kSrcU < s1) load(s + kSrcU, eb, vb);
+      load(min(s + kSrcU, s1 - kSrcU), eb, vb);
+      __builtin_amdgcn_sched_barrier(0);
       apply(ea, va);
       s += kSrcU;
       if (s >= s1) break;
-      if (s + kSrcU < s1) load(s + kSrcU, ea, va);
+      load(min(s + kSrcU, s1 - kSrcU), ea, va);
+      __builtin_amdgcn_sched_barrier(0);
       apply(eb, vb);
       s += kSrcU;
       if (s >= s1) break;
     }
   }
-  if (Ac) {
-    const int e0 = wt_ptr[k], e1 = wt_ptr[k + 1];
-    // entries come in groups of 4 (padded with zero weights)
-    for (int e = e0; e < e1; e += 4) {
-      T2 a[4][kFarCpl];
+  const int e0 = Ac ? wt_ptr[k] : 0, e1 = Ac ? wt_ptr[k + 1] : 0;
+  if (e0 < e1) {
+    // entries come in groups of 4 (padded with zero weights), loaded one
+    // group ahead like the sources
+    T2 aa[4][kFarCpl], ab[4][kFarCpl];
+    auto load = [&](int e, T2(&a)[4][kFarCpl]) {
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
-        const T2* ar = Ac + ((size_t)wt_row[e + u] * nbc + P) * ncol + col;
+        const T2* ar = Ac + (unsigned)((wt_row[e + u] * nbc + P) * ncol + col);
 #pragma unroll
         for (int c = 0; c < kFarCpl; ++c) a[u][c] = ar[c * kWave];
       }
+    };
+    auto apply = [&](int e, const T2(&a)[4][kFarCpl]) {
 #pragma unroll
       for (int u = 0; u < 4; ++u)
 #pragma unroll
@@ -410,13 +421,26 @@ __device__ __forceinline__ void far_parent(int k, int P, int col, LW_FAR_PARAMS)
 #pragma unroll
           for (int c = 0; c < kFarCpl; ++c) cmac(acc[i][c], w, a[u][c]);
         }
+    };
+    load(e0, aa);
+    for (int e = e0;;) {
+      load(min(e + 4, e1 - 4), ab);
+      __builtin_amdgcn_sched_barrier(0);
+      apply(e, aa);
+      e += 4;
+      if (e >= e1) break;
+      load(min(e + 4, e1 - 4), aa);
+      __builtin_amdgcn_sched_barrier(0);
+      apply(e, ab);
+      e += 4;
+      if (e >= e1) break;
     }
   }
   T2* Ak = A + (size_t)k * nb * ncol + col;
 #pragma unroll
   for (int i = 0; i < NCH; ++i)
 #pragma unroll
-    for (int c = 0; c < kFarCpl; ++c) Ak[(size_t)(c0 + i) * ncol + c * kWave] = acc[i][c];
+    for (int c = 0; c < kFarCpl; ++c) Ak[(unsigned)((c0 + i) * ncol + c * kWave)] = acc[i][c];
 }
 
 template <typename S>
@@ -1511,6 +1535,11 @@ static int setup_far(Batch* bt) {
       b.wp = wp;
     }
     const size_t fsz = (size_t)lp.nk * lp.nb * 2 * R;
+    if (fsz >= (1ull << 31)) {
+      // the far kernel indexes a level's campo arrays with 32-bit offsets
+      set_error("lw_batch_create: a level's field array exceeds 2^31 entries");
+      return LW_ERR_ARG;
+    }
     T2 *c = nullptr, *a = nullptr;
     if ((rc = bmalloc(bt, &c, fsz)) || (rc = bmalloc(bt, &a, fsz))) return rc;
     b.C = c;
