@@ -268,9 +268,10 @@ def batched_sweep(lw, op, args, plain_rhs_per_s):
     """The chosen form on the same mesh: a monostatic sweep of args.sweep
     angles (phi in [0, pi), theta = 90 deg, VV) solved as R chains of
     consecutive angles in one persistent batch (FP64 far field by default,
-    FP64 near field and CGS) to the same tolerance; each chain restarts its
-    column with the next angle as soon as one converges, from the learned
-    guess (or the warm start).  Its sigma is checked against FP64 single-RHS
+    FP64 near field and CGS) to the same tolerance; each column restarts with
+    the next angle of its chain as soon as one converges (or, once its chain
+    is done, with the last pending angle of another chain), from the learned
+    guess of two solved neighbours (or the warm start of one).  Its sigma is checked against FP64 single-RHS
     solves of two angles from zero guesses, and its bistatic pattern at the
     first angle on 361 theta points in the phi = 0 cut."""
     from lywell_amd import autotune, guess_train
@@ -311,14 +312,17 @@ def batched_sweep(lw, op, args, plain_rhs_per_s):
     log(f"  bistatic pattern at angle 0 (361 theta, phi = 0): max difference {bistatic:.4f} dB")
     bt.close()
     its = res["iterations"]
-    pos = np.arange(n) % m  # position of each angle in its chain
-    by_pos = {int(q): float(its[pos == q].mean()) for q in range(m)}
-    # steady state of a long sweep: every angle past the second of its chain
-    # starts from the learned guess; time per batched iteration from this run
+    kinds = res["guess_kind"]
+    names = {0: "zero", 1: "one_neighbour_warm_start", 2: f"two_neighbour_{guess}"}
+    by_kind = {names[q]: {"angles": int((kinds == q).sum()), "iterations_mean": float(its[kinds == q].mean())}
+               for q in range(3) if (kinds == q).any()}
+    # steady state of a long sweep: every angle starts from the two-neighbour
+    # guess; time per batched iteration from this run
     s_per_iter = res["seconds"] / max(1.0, float(its.sum()) / R)
-    late = its[pos >= 2] if m > 2 else its
+    late = its[kinds == 2] if (kinds == 2).any() else its
     steady = R / (float(late.mean()) * s_per_iter)
-    log(f"steady state (chain positions >= 2): {late.mean():.0f} iterations per angle, "
+    log(f"iterations by guess: {by_kind}")
+    log(f"steady state (two-neighbour guesses): {late.mean():.0f} iterations per angle, "
         f"{s_per_iter * 1e3:.1f} ms per batched iteration -> {steady:.3f} RHS/s")
     return {"rhs_per_s": round(value, 4), "speedup_vs_plain_sweep": round(value / plain_sweep_rhs_s, 2),
             "steady_state_rhs_per_s": round(steady, 4),
@@ -330,7 +334,7 @@ def batched_sweep(lw, op, args, plain_rhs_per_s):
             "guess": guess, "far_field": "FP64" if fp64 else "FP32", "autotune_ms": times,
             "matvec_ms": round(mv_ms, 3), "matvec_ms_per_rhs": round(mv_ms / R, 4),
             "iterations_mean": float(its.mean()), "iterations_max": int(its.max()),
-            "iterations_mean_by_chain_position": by_pos, "restarts": res["restarts"],
+            "iterations_by_guess": by_kind, "restarts": res["restarts"], "steals": res["steals"],
             "converged": bool(res["converged"].all()),
             "monostatic_rcs_db_err_vs_fp64": [round(e, 5) for e in errs],
             "bistatic_rcs_db_err_vs_fp64_361": round(bistatic, 5),

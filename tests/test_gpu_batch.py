@@ -191,3 +191,36 @@ def test_batch_restart_column():
     for c in (0, 4, 6, 63):
         assert np.array_equal(bt.column(c), X0[:, c])
         assert st2[c, 2] == 1 and st2[c, 3] == st[c, 3]
+
+
+@pytest.mark.parametrize("guess", ["extrap", "mlp"])
+def test_persistent_sweep_with_steals(guess):
+    # every angle of a sweep whose length is not a multiple of the block
+    # width is solved to the tolerance; columns without a chain of their own
+    # take angles from the ends of other chains; sigma matches FP64
+    # single-RHS solves
+    from lywell_amd import guess_train
+    from lywell_amd.sweep import Sweep, polarisations, sweep_directions
+    mesh, op, bt = sphere_batch(True)
+    net = guess_train.load() if guess == "mlp" else None
+    if guess == "mlp" and net is None:
+        pytest.skip("no trained guess network")
+    n = 3 * bt.R + 5
+    sw = Sweep(op, bt, errmax=1e-4, maxit=3000, guess=guess, mlp=net)
+    res = sw.run_persistent(n, keep_currents=True, steal=True)
+    kinds = res["guess_kind"]
+    print(f"[sweep {guess}] {n} angles in {res['seconds']:.1f}s, {res['steals']} steals, iterations by guess kind",
+          {int(q): float(res["iterations"][kinds == q].mean()) for q in range(3) if (kinds == q).any()})
+    assert res["converged"].all() and (res["iterations"] > 0).all()
+    assert res["steals"] > 0
+    # guesses from solved neighbours save iterations over cold starts
+    its = res["iterations"]
+    assert (kinds == 2).any() and its[kinds == 2].mean() < its[kinds == 0].mean()
+    phi, dirs = sweep_directions(n)
+    pols = polarisations(dirs, "VV").astype(np.complex128)
+    for a in (0, n // 2, n - 1):
+        x = res["currents"][:, a]
+        assert rel(op.matvec(x), op.plane_wave(dirs[a], pols[a])) < 3e-4
+        x64, it64, rr64, c64 = op.cgs(op.plane_wave(dirs[a], pols[a]), errmax=1e-4, use_prec=False, maxit=3000)
+        s64 = op.rcs(x64, pols[a], np.pi / 2, phi[a])[0]
+        assert abs(10 * np.log10(res["sigma"][a] / s64)) < 0.01
