@@ -60,6 +60,9 @@ def main():
     ap.add_argument("--batch", type=int, default=64,
                     help="columns of the batched sweep (64 or 128; -1: autotuned; 0: skip the sweep)")
     ap.add_argument("--chain", type=int, default=3, help="angles per column chain of the sweep")
+    ap.add_argument("--no-steal", action="store_true",
+                    help="keep columns whose chain is done idle instead of letting them take the last pending "
+                         "angles of other chains")
     ap.add_argument("--batch-prec", choices=["fp64", "fp32"], default="fp64",
                     help="far-field precision of the batched sweep")
     ap.add_argument("--sweep", type=int, default=0, help="angles of the batched sweep (default: chain x columns)")
@@ -70,6 +73,10 @@ def main():
     ap.add_argument("--fp32-probe-iters", type=int, default=300,
                     help="batched CGS iterations of the FP32-far-field probe (0: skip)")
     ap.add_argument("--cpu-iters", type=int, default=3)
+    ap.add_argument("--ellipsoid-sweep", type=int, default=0,
+                    help="also run the batched monostatic sweep of the 4 x 1 x 1 lambda prolate ellipsoid "
+                         "(30,720 RWG) with this many azimuths per polarisation over 360 degrees, VV and HH "
+                         "(0: skip)")
     ap.add_argument("--matvec-only", action="store_true",
                     help="time the single and batched matvecs only (for kernel profiles), no solves")
     args = ap.parse_args()
@@ -196,6 +203,10 @@ def main():
     if args.fp32_probe_iters > 0:
         probe = precision_probe(lw, op, args)
 
+    ellipsoid = None
+    if args.ellipsoid_sweep > 0:
+        ellipsoid = ellipsoid_sweep(lw, args)
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(mesh, args, use_prec, it_mean)
@@ -238,6 +249,8 @@ def main():
         "batched": batched,
         "fp32_far_field": probe,
     }
+    if ellipsoid is not None:
+        out["ellipsoid_sweep"] = ellipsoid
     print(json.dumps(out))
 
 
@@ -289,7 +302,7 @@ def batched_sweep(lw, op, args, plain_rhs_per_s):
         log("no trained guess network found: using the warm start")
     n = args.sweep if args.sweep > 0 else args.chain * R
     sw = Sweep(op, bt, errmax=args.errmax, maxit=args.maxit, guess=guess, mlp=net, log=log)
-    res = sw.run_persistent(n, keep_currents=True)
+    res = sw.run_persistent(n, keep_currents=True, steal=not args.no_steal)
     value = n / res["seconds"]
     m = (n + R - 1) // R
     plain_sweep_rhs_s, plain_its = plain_sweep(op, args, n, range(0, min(m, n)))
@@ -340,6 +353,86 @@ def batched_sweep(lw, op, args, plain_rhs_per_s):
             "bistatic_rcs_db_err_vs_fp64_361": round(bistatic, 5),
             "precision": f"{'FP64' if fp64 else 'FP32'} far field (aggregation, interpolation, translation, "
                          "receive), FP64 near field and CGS"}
+
+
+def ellipsoid_sweep(lw, args):
+    """The batched monostatic sweep of the PEC prolate ellipsoid with semi-axes
+    4 x 1 x 1 lambda (icosphere subdivision 5 stretched, 30,720 RWG,
+    ladomax 0.25): args.ellipsoid_sweep azimuths over [0, 360) degrees at
+    theta = 90 degrees, VV then HH, each polarisation one persistent sweep in
+    an autotuned block.  The far field is FP32 when a 64-column CGS probe
+    from zero converges in every column, else FP64.  The plain drop-in is
+    timed on the first chain of the VV sweep (the reference's warm start),
+    and sigma is checked against FP64 single-RHS solves."""
+    from lywell_amd import autotune, guess_train
+    from lywell_amd.sweep import Sweep, polarisations, sweep_directions, warm_start
+    n = args.ellipsoid_sweep
+    t0 = time.time()
+    mesh = lw.Mesh.icosphere(5, 4.0, 1.0, 1.0)
+    op = lw.MLFMA(mesh, lam=1.0, ladomax=0.25)
+    log(f"ellipsoid: {mesh.ne} RWG, {op.nlevels} levels, near nnz {op.nnz}, setup {time.time() - t0:.1f}s")
+    span = 2 * np.pi
+    phi, dirs = sweep_directions(n, span)
+    # FP32 far field if CGS converges with it on this mesh
+    probe_it = 4000
+    bt32 = lw.Batch(op, 64, far_fp64=False)
+    a64 = np.arange(64) * n // 64
+    B = bt32.plane_wave(dirs[a64], polarisations(dirs[a64], "VV").astype(np.complex128))
+    _, it32, rr32, cv32 = bt32.cgs(B, errmax=args.errmax, use_prec=False, maxit=probe_it)
+    bt32.close()
+    fp64 = not cv32.all()
+    log(f"ellipsoid FP32-far-field probe: {int(cv32.sum())}/64 columns converged within {probe_it} iterations "
+        f"(mean {it32.mean():.0f}); sweep far field {'FP64' if fp64 else 'FP32'}")
+    best, times = autotune.tune(op, far_fp64=fp64, log=log)
+    bt = lw.Batch(op, best, far_fp64=fp64)
+    net = guess_train.load() if args.guess == "mlp" else None
+    guess = args.guess if (args.guess != "mlp" or net is not None) else "warm"
+    res = {}
+    for pol in ("VV", "HH"):
+        sw = Sweep(op, bt, errmax=args.errmax, maxit=args.maxit, guess=guess, mlp=net, pol=pol, log=log, span=span)
+        res[pol] = sw.run_persistent(n, keep_currents=False)
+    bt.close()
+    seconds = sum(r["seconds"] for r in res.values())
+    rhs_s = 2 * n / seconds
+    # the plain drop-in on the first chain of the VV sweep
+    m = (n + best - 1) // best
+    pols = polarisations(dirs, "VV").astype(np.complex128)
+    k0 = 2 * np.pi / op.lam
+    x_prev, d_prev, plain_its = None, None, []
+    t0 = time.time()
+    for a in range(min(m, n)):
+        b = op.plane_wave(dirs[a], pols[a])
+        g = None if x_prev is None else warm_start(x_prev[:, None], mesh.e_centro, k0, dirs[a][None, :],
+                                                      d_prev[None, :])[:, 0]
+        x, it, rr, cv = op.cgs(b, guess=g, errmax=args.errmax, use_prec=False, maxit=args.maxit)
+        plain_its.append(int(it))
+        x_prev, d_prev = x, dirs[a]
+    plain_s = len(plain_its) / (time.time() - t0)
+    errs = []
+    for pol in ("VV", "HH"):
+        pp = polarisations(dirs, pol).astype(np.complex128)
+        for a in (0, n // 3):
+            x, it, rr, cv = op.cgs(op.plane_wave(dirs[a], pp[a]), errmax=args.errmax, use_prec=False,
+                                   maxit=args.maxit)
+            s64 = op.rcs(x, pp[a], np.pi / 2, phi[a])[0]
+            errs.append(abs(10 * np.log10(res[pol]["sigma"][a] / s64)))
+    kinds = {pol: {int(q): float(r["iterations"][r["guess_kind"] == q].mean())
+                   for q in range(3) if (r["guess_kind"] == q).any()} for pol, r in res.items()}
+    log(f"ellipsoid sweep: {2 * n} RHS in {seconds:.1f}s = {rhs_s:.3f} RHS/s; plain drop-in {plain_s:.3f} RHS/s "
+        f"on angles 0..{len(plain_its) - 1} ({plain_its}); speedup {rhs_s / plain_s:.2f}; "
+        f"iterations by guess kind {kinds}; sigma vs FP64 single solves max {max(errs):.5f} dB")
+    op.close()
+    return {"workload": f"PEC prolate ellipsoid 4 x 1 x 1 lambda, icosphere s=5, {mesh.ne} RWG, "
+                        f"{n} azimuths over 360 deg x VV, HH, CGS to {args.errmax:g}",
+            "rhs": 2 * n, "seconds": round(seconds, 2), "rhs_per_s": round(rhs_s, 4),
+            "plain_rhs_per_s": round(plain_s, 4), "plain_iterations": plain_its,
+            "speedup_vs_plain_sweep": round(rhs_s / plain_s, 2), "nrhs_per_batch": best, "autotune_ms": times,
+            "far_field": "FP64" if fp64 else "FP32",
+            "fp32_probe": {"converged": int(cv32.sum()), "of": 64, "iterations_mean": float(it32.mean()),
+                           "maxit": probe_it},
+            "guess": guess, "iterations_mean": {p: float(r["iterations"].mean()) for p, r in res.items()},
+            "iterations_by_guess_kind": kinds, "converged": all(bool(r["converged"].all()) for r in res.values()),
+            "rcs_db_err_vs_fp64": [round(e, 5) for e in errs]}
 
 
 def precision_probe(lw, op, args):

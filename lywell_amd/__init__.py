@@ -417,10 +417,10 @@ class Mlp:
     depth (tanh) -> out, FP32 master weights, bfloat16 GEMM operands with FP32
     accumulation on the GPU."""
 
-    FEATURES = 16
+    FEATURES = 19
     OUTPUTS = 4
 
-    def __init__(self, in_dim=16, hidden=256, depth=3, out_dim=4, seed=0):
+    def __init__(self, in_dim=19, hidden=256, depth=3, out_dim=4, seed=0):
         L = lib()
         L.lw_mlp_nparams.restype = _C.c_longlong
         L.lw_guess_nsamples.restype = _C.c_longlong

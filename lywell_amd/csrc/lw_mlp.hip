@@ -175,10 +175,13 @@ __global__ void k_adam(long long n, float lr, float b1, float b2, float eps, flo
 
 // ------------------------------------------------------------------ guess head
 
-constexpr int kFeat = 16, kOut = 4;
+constexpr int kFeat = 19, kOut = 4;
 
 // Per row (unknown i, column c; row = i*ncol + c): phase-corrected neighbour
-// currents and the feature vector.
+// currents and the feature vector: the two normalised neighbour currents, the
+// incidence direction and polarisation, the edge centre relative to its leaf
+// box (box units) and to the body (half-sizes), and the unit step d - d1 from
+// the nearest solved angle (which side the neighbours lie on).
 __global__ void k_guess_features(int ne, int ncol, double k0, const double* __restrict__ ec,
                                  const double* __restrict__ boxc, const int* __restrict__ basis_box,
                                  double box_side, const double* __restrict__ body, const double* __restrict__ dir,
@@ -221,6 +224,12 @@ __global__ void k_guess_features(int ne, int ncol, double k0, const double* __re
   f[13] = (float)((e[0] - body[0]) / body[3]);
   f[14] = (float)((e[1] - body[1]) / body[3]);
   f[15] = (float)((e[2] - body[2]) / body[3]);
+  const double st[3] = {d[0] - d1[0], d[1] - d1[1], d[2] - d1[2]};
+  const double sn = sqrt(st[0] * st[0] + st[1] * st[1] + st[2] * st[2]);
+  const double si = sn > 0.0 ? 1.0 / sn : 0.0;
+  f[16] = (float)(st[0] * si);
+  f[17] = (float)(st[1] * si);
+  f[18] = (float)(st[2] * si);
 }
 
 __global__ void k_guess_apply(long long rows, const float* __restrict__ o, const float2* __restrict__ xt1,

@@ -1,19 +1,21 @@
 """Training of the learned initial guess on solutions the solver generates.
 
-    python -m lywell_amd.guess_train [--meshes 12] [--subdiv 3,4] [--angles 64]
+    python -m lywell_amd.guess_train [--meshes 12] [--subdiv 3,4] [--angles 64,192,384]
                                      [--steps 3000] [--out lywell_amd/guess_mlp.npz]
 
 Training meshes are seeded random bodies: ellipsoids with semi-axes uniform
 in [amin, amax] wavelengths and spheres perturbed by real spherical
 harmonics of degree 2-4 (5% total amplitude), icosphere subdivisions cycled
 from `subdiv` (1,920 and 7,680 RWG by default).  For each mesh a monostatic
-sweep of `angles` incidence angles (VV on even meshes, HH on odd ones) is
-solved in one batch with the FP64 far field to `errmax`; a body whose sweep
-does not converge within `maxit` unpreconditioned CGS iterations is replaced
-by a new draw.  Every
-angle a >= 2 of the sweep becomes one training sample per unknown, with the
-converged currents of angles a-1 and a-2 as inputs and that of angle a as
-the target.  The network (16 -> 256 x 3 -> 4) is trained by the library's
+sweep over [0, pi) with the next count of incidence angles from `angles`
+(steps of 2.8, 0.94 and 0.47 degrees by default; VV on even meshes, HH on
+odd ones) is solved in batches with the FP64 far field to `errmax`; a body
+whose sweep does not converge within `maxit` unpreconditioned CGS iterations
+is replaced by a new draw.  Every angle of the sweep becomes training
+samples, one per unknown and side: with the converged currents of angles
+a-1 and a-2 as inputs (a >= 2) and with those of a+1 and a+2 (a <= n-3),
+the converged current of angle a being the target; the step direction
+feature tells the two sides apart.  The network (19 -> 256 x 3 -> 4) is trained by the library's
 own forward/backward kernels (lw_guess_train: Adam on random minibatches),
 `steps` steps at `lr` and then `steps2` at `lr2`.  Its output layer starts
 at zero weights with the bias of o2 at `init`: 1 makes the untrained network
@@ -89,22 +91,31 @@ def init_head(net, extrapolation=1.0):
     net.set_params(p)
 
 
+def _targets(n, reverse):
+    """Target angles and their nearest and next solved neighbours on one side."""
+    a = np.arange(0, n - 2) if reverse else np.arange(2, n)
+    s = -1 if reverse else 1
+    return a, a - s, a - 2 * s
+
+
 def samples(net, op, dirs, pols, X):
-    """Angles 2..n-1 of a solved sweep as training rows."""
+    """Every angle of a solved sweep as training rows, from either side."""
     n = X.shape[1]
-    a = np.arange(2, n)
-    net.add_samples(op, dirs[a], pols[a], dirs[a - 1], X[:, a - 1], dirs[a - 2], X[:, a - 2], X[:, a])
+    for reverse in (False, True):
+        a, a1, a2 = _targets(n, reverse)
+        net.add_samples(op, dirs[a], pols[a], dirs[a1], X[:, a1], dirs[a2], X[:, a2], X[:, a])
 
 
-def guess_errors(net, op, dirs, pols, X):
+def guess_errors(net, op, dirs, pols, X, reverse=False):
     """Relative error of the warm start, the linear extrapolation and the
-    learned guess on angles 2..n-1 of a solved sweep."""
+    learned guess on the angles of a solved sweep, from the two neighbours
+    before each angle (or after it, with reverse)."""
     n = X.shape[1]
-    a = np.arange(2, n)
+    a, a1, a2 = _targets(n, reverse)
     k0 = 2 * np.pi / op.lam
-    xt1 = warm_start(X[:, a - 1], op.mesh.e_centro, k0, dirs[a], dirs[a - 1])
-    xt2 = warm_start(X[:, a - 2], op.mesh.e_centro, k0, dirs[a], dirs[a - 2])
-    g = net.predict_guess(op, dirs[a], pols[a], dirs[a - 1], X[:, a - 1], dirs[a - 2], X[:, a - 2])
+    xt1 = warm_start(X[:, a1], op.mesh.e_centro, k0, dirs[a], dirs[a1])
+    xt2 = warm_start(X[:, a2], op.mesh.e_centro, k0, dirs[a], dirs[a2])
+    g = net.predict_guess(op, dirs[a], pols[a], dirs[a1], X[:, a1], dirs[a2], X[:, a2])
     nrm = np.linalg.norm(X[:, a], axis=0)
     return {"warm": np.linalg.norm(xt1 - X[:, a], axis=0) / nrm,
             "extrapolation": np.linalg.norm(2 * xt1 - xt2 - X[:, a], axis=0) / nrm,
@@ -118,7 +129,7 @@ def main(argv=None):
     ap.add_argument("--amin", type=float, default=0.5)
     ap.add_argument("--amax", type=float, default=2.0)
     ap.add_argument("--ladomax", type=float, default=0.2)
-    ap.add_argument("--angles", type=int, default=64)
+    ap.add_argument("--angles", default="64,192,384", help="sweep lengths over [0, pi), cycled over the meshes")
     ap.add_argument("--errmax", type=float, default=1e-5)
     ap.add_argument("--maxit", type=int, default=4000)
     ap.add_argument("--steps", type=int, default=3000)
@@ -133,10 +144,11 @@ def main(argv=None):
     if lw.device_count() < 1:
         raise SystemExit("guess_train: no HIP device")
     rng = np.random.default_rng(args.seed)
-    net = lw.Mlp(16, 256, 3, 4, seed=args.seed)
+    net = lw.Mlp(19, 256, 3, 4, seed=args.seed)
     init_head(net, args.init)
     bodies, skipped = [], 0
     subdivs = [int(v) for v in str(args.subdiv).split(",")]
+    angles = [int(v) for v in str(args.angles).split(",")]
     t0 = time.time()
     i = 0
     while i <= args.meshes:  # the last mesh is held out
@@ -144,7 +156,8 @@ def main(argv=None):
         mesh, meta = random_body(rng, subdivs[i % len(subdivs)], args.amin, args.amax, kind)
         op = lw.MLFMA(mesh, lam=1.0, ladomax=args.ladomax)
         pol = "VV" if i % 2 == 0 else "HH"
-        sol = solve_sweep(op, args.angles, pol, args.errmax, args.maxit)
+        nang = angles[i % len(angles)]
+        sol = solve_sweep(op, nang, pol, args.errmax, args.maxit)
         if sol is None:
             # unpreconditioned CGS stalls on some bodies: draw another one
             log(f"mesh {i}: {kind} {mesh.ne} RWG, {pol}: sweep not converged in {args.maxit} iterations, redrawn")
@@ -154,11 +167,12 @@ def main(argv=None):
                 raise RuntimeError("too many training sweeps failed to converge")
             continue
         phi, dirs, pols, X, its = sol
-        meta.update({"ne": mesh.ne, "levels": op.nlevels, "pol": pol, "iterations_mean": float(its.mean())})
+        meta.update({"ne": mesh.ne, "levels": op.nlevels, "pol": pol, "angles": nang,
+                     "iterations_mean": float(its.mean())})
         if i < args.meshes:
             samples(net, op, dirs, pols, X)
             bodies.append(meta)
-            log(f"mesh {i}: {kind} {mesh.ne} RWG, {pol}, sweep solved (mean {its.mean():.0f} iterations), "
+            log(f"mesh {i}: {kind} {mesh.ne} RWG, {pol}, {nang} angles solved (mean {its.mean():.0f} iterations), "
                 f"{net.nsamples()} rows, {time.time() - t0:.1f}s")
             op.close()
         else:
@@ -170,10 +184,12 @@ def main(argv=None):
                                                seed=args.seed + 1)])
     log(f"trained {loss.size} steps: loss {loss[:10].mean():.4e} -> {loss[-50:].mean():.4e}")
     op, dirs, pols, X, meta = held
-    err = guess_errors(net, op, dirs, pols, X)
-    ev = {k: float(np.median(v)) for k, v in err.items()}
+    ev = {}
+    for reverse in (False, True):
+        err = guess_errors(net, op, dirs, pols, X, reverse=reverse)
+        ev["reverse" if reverse else "forward"] = {k: float(np.median(v)) for k, v in err.items()}
     log("held-out median relative error of the guess:", json.dumps(ev))
-    np.savez(args.out, params=net.get_params(), arch=np.array([16, 256, 3, 4]),
+    np.savez(args.out, params=net.get_params(), arch=np.array([net.in_dim, net.hidden, net.depth, net.out_dim]),
              loss=loss, meta=json.dumps({"train_meshes": bodies, "held_out": meta, "held_out_error": ev,
                                          "redrawn": skipped, "args": vars(args)}))
     log(f"wrote {args.out}")

@@ -22,8 +22,9 @@ import time
 import numpy as np
 
 
-def sweep_directions(n):
-    phi = np.pi * np.arange(n) / n
+def sweep_directions(n, span=np.pi):
+    """n azimuths phi = span*a/n (the reference's convention is span = pi)."""
+    phi = span * np.arange(n) / n
     return phi, np.stack([-np.cos(phi), -np.sin(phi), np.zeros(n)], axis=1)
 
 
@@ -52,21 +53,22 @@ class Sweep:
     soon as it has converged, while the other columns keep iterating
     (lw_batch_restart_column), so no column waits for the block's slowest."""
 
-    def __init__(self, op, batch, errmax=1e-4, maxit=20000, guess="warm", mlp=None, pol="VV", log=None):
+    def __init__(self, op, batch, errmax=1e-4, maxit=20000, guess="warm", mlp=None, pol="VV", log=None,
+                 span=np.pi):
         if guess not in ("zero", "warm", "extrap", "mlp"):
             raise ValueError("guess: zero, warm, extrap or mlp")
         if guess == "mlp" and mlp is None:
             raise ValueError("the learned guess needs a trained network")
         self.op, self.bt = op, batch
         self.errmax, self.maxit = errmax, maxit
-        self.guess, self.mlp, self.pol = guess, mlp, pol
+        self.guess, self.mlp, self.pol, self.span = guess, mlp, pol, span
         self.k0 = 2 * np.pi / op.lam
         self.log = log or (lambda *a: None)
 
     def run(self, n, keep_currents=False):
         R = self.bt.R
         nblocks = (n + R - 1) // R
-        phi, dirs = sweep_directions(n)
+        phi, dirs = sweep_directions(n, self.span)
         pols = polarisations(dirs, self.pol)
         sigma = np.zeros(n)
         iters = np.zeros(n, np.int64)
@@ -145,21 +147,22 @@ class Sweep:
                 return xt, 1
         return None, 0
 
-    def run_persistent(self, n, keep_currents=False, check_every=8, steal=False):
-        """Column c starts chain c (angles c*m .. c*m+m-1, m = ceil(n/R)) and
+    def run_persistent(self, n, keep_currents=False, check_every=8, steal=True):
+        """Column c starts chain c (angles c*n//R .. (c+1)*n//R - 1) and
         works through it left to right, each angle from the guess its solved
         neighbours allow.  With steal, a column without work takes the last
         pending angle of the longest chain among those whose last pending
         angle already has solved right-hand neighbours for the same kind of
         guess (the next chain's first angles), so that columns do not idle
-        while others still have work.  Stealing is off by default: the shipped
-        network, trained on sweeps in increasing phi, guesses worse from the
-        right-hand side (942 against 798 mean iterations measured on the
-        122,880-RWG sphere), which costs more than the idle tail saves."""
+        while others still have work.  The learned guess is trained from both
+        sides (its step-direction feature tells them apart), so a stolen
+        angle starts from as good a guess as its own chain would give it."""
         R = self.bt.R
         m = (n + R - 1) // R
-        chains = [list(range(c * m, min((c + 1) * m, n))) for c in range(R)]
-        phi, dirs = sweep_directions(n)
+        # chain lengths differ by at most one, so that every column has work
+        bounds = [c * n // R for c in range(R + 1)]
+        chains = [list(range(bounds[c], bounds[c + 1])) for c in range(R)]
+        phi, dirs = sweep_directions(n, self.span)
         pols = polarisations(dirs, self.pol)
         cp = pols.astype(np.complex128)
         sigma = np.zeros(n)
