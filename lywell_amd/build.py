@@ -48,6 +48,8 @@ def build_hip():
         ("lw_device.hip", []),
         ("lw_batch.hip", []),
         ("lw_mlp.hip", []),
+        # bit-exact elimination order and roundings of ilu0C: no contraction
+        ("lw_ilu0.hip", ["-ffp-contract=off"]),
         # bit-exact Legendre sums: no contraction into FMAs in this unit
         ("lw_setup_kernels.hip", ["-ffp-contract=off"]),
         ("lw_plan.cpp", ["-x", "hip"]),

@@ -954,34 +954,25 @@ int lw_near_csr64(lw_handle* hh, int* ia, int* ja, double* z) {
 int lw_ilu0(lw_handle* hh, int* ierr) {
   if (lw::check_handle(hh)) return LW_ERR_ARG;
   Handle* h = &hh->h;
-  const int n = h->ne;
-  int piv;
-  if (h->fp64_near) {
-    std::vector<double> a(2 * (size_t)h->nnz);
-    LW_HIP(hipMemcpyAsync(a.data(), h->d_z, sizeof(double2) * h->nnz, hipMemcpyDeviceToHost, h->stream));
-    LW_HIP(hipStreamSynchronize(h->stream));
-    piv = lw::ilu0_msr<double>(n, h->plan.ia.data(), h->plan.ja.data(), a.data(), h->h_alu64, h->h_jlu, h->h_ju);
-  } else {
-    std::vector<float> a(2 * (size_t)h->nnz);
-    LW_HIP(hipMemcpyAsync(a.data(), h->d_z, sizeof(float2) * h->nnz, hipMemcpyDeviceToHost, h->stream));
-    LW_HIP(hipStreamSynchronize(h->stream));
-    piv = lw::ilu0_msr<float>(n, h->plan.ia.data(), h->plan.ja.data(), a.data(), h->h_alu32, h->h_jlu, h->h_ju);
-  }
+  h->have_ilu = false;
+  int piv = 0;
+  if (int rc = lw::ilu0_device(h, &piv)) return rc;
   if (ierr) *ierr = piv;
   if (piv) {
-    h->have_ilu = false;
     lw::set_error("ilu0: zero pivot at row " + std::to_string(piv));
     return LW_ERR_PIVOT;
   }
-  return lw::upload_factor(h);
+  h->have_ilu = true;
+  return LW_OK;
 }
 
 }  // extern "C"
 
 namespace lw {
-// Device copies of the host MSR factor in h->h_alu*/h_jlu/h_ju (1-based with a
-// pad element so alu(k) is d_alu[k]) and the level sets of both sweeps.
-int upload_factor(Handle* h) {
+// Device arrays of the MSR factor's structure in h->h_jlu/h_ju (1-based with
+// a pad element so alu(k) is d_alu[k]; the value array is allocated here) and
+// the level sets of both sweeps.
+int upload_structure(Handle* h) {
   const int n = h->ne;
   const size_t na = (size_t)h->nnz + 2;
   if (!h->d_alu) {
@@ -998,10 +989,6 @@ int upload_factor(Handle* h) {
     std::vector<int> ju1((size_t)n + 1, 0);
     if (int rc = lw::dmalloc(h, &h->d_ju, ju1.size())) return rc;
   }
-  if (h->fp64_near)
-    LW_HIP(hipMemcpyAsync(h->d_alu, h->h_alu64.data(), sizeof(double2) * na, hipMemcpyHostToDevice, h->stream));
-  else
-    LW_HIP(hipMemcpyAsync(h->d_alu, h->h_alu32.data(), sizeof(float2) * na, hipMemcpyHostToDevice, h->stream));
   LW_HIP(hipMemcpyAsync(h->d_jlu, h->h_jlu.data(), sizeof(int) * na, hipMemcpyHostToDevice, h->stream));
   std::vector<int> ju1((size_t)n + 1, 0);
   for (int i = 0; i < n; ++i) ju1[i + 1] = h->h_ju[i];
@@ -1015,6 +1002,19 @@ int upload_factor(Handle* h) {
   if (int rc = lw::upload(h, &h->d_lrows, lrows)) return rc;
   if (int rc = lw::upload(h, &h->d_uptr, uptr)) return rc;
   if (int rc = lw::upload(h, &h->d_urows, urows)) return rc;
+  LW_HIP(hipStreamSynchronize(h->stream));
+  return LW_OK;
+}
+
+// Device copies of a complete host MSR factor (structure and values).
+int upload_factor(Handle* h) {
+  h->have_ilu = false;
+  if (int rc = upload_structure(h)) return rc;
+  const size_t na = (size_t)h->nnz + 2;
+  if (h->fp64_near)
+    LW_HIP(hipMemcpyAsync(h->d_alu, h->h_alu64.data(), sizeof(double2) * na, hipMemcpyHostToDevice, h->stream));
+  else
+    LW_HIP(hipMemcpyAsync(h->d_alu, h->h_alu32.data(), sizeof(float2) * na, hipMemcpyHostToDevice, h->stream));
   LW_HIP(hipStreamSynchronize(h->stream));
   h->have_ilu = true;
   return LW_OK;

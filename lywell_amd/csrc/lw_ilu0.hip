@@ -1,0 +1,272 @@
+// ILU(0) factorisation of the near-field matrix on the GPU: ilu0C
+// (modIterativo.f90:1805-1976) as crear_precond calls it (main.f90:884-903),
+// with the rows level-scheduled.
+//
+// The factor is the reference's MSR layout (1-based, a pad element first):
+// alu(ii) holds the inverted diagonal of row ii, alu(jlu(ii) .. ju(ii)-1) its
+// L part (ascending columns < ii) and alu(ju(ii) .. jlu(ii+1)-1) its U part.
+// Row ii is eliminated against the rows jrow of its L part in ascending
+// order, exactly as the reference does, so it can start once those rows are
+// final: the rows of one level of the lower-triangular dependency graph are
+// independent.  One workgroup walks the levels; each wave factors one row at
+// a time with the row staged in LDS: per elimination step the wave's lanes
+// take the entries of jrow's U part, find their columns in row ii by binary
+// search over the row's sorted columns, and update them.  Every entry of row
+// ii is updated in the same order and with the same arithmetic as the
+// sequential reference (products in the storage precision, updates in
+// double, contraction off), so the factor is bit-identical to it.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <climits>
+#include <string>
+#include <vector>
+
+#include "lw_internal.h"
+
+namespace lw {
+namespace {
+
+constexpr int kWaveSize = 64;
+constexpr int kIluMaxWaves = 16;
+
+// Orders the wave's LDS accesses: every lane's writes are complete before any
+// lane reads on.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+// CSR (natural order, 0-based) values into the MSR positions: one wave per
+// row; the diagonal goes to alu(i), the other entries keep their CSR order.
+template <typename T2>
+__global__ void k_ilu0_scatter(int n, const int* __restrict__ ia, const int* __restrict__ ja,
+                               const T2* __restrict__ z, const int* __restrict__ jlu, T2* __restrict__ alu) {
+  const int i = blockIdx.x * (blockDim.x / kWaveSize) + threadIdx.x / kWaveSize;
+  const int lane = threadIdx.x % kWaveSize;
+  if (i >= n) return;
+  const int r0 = ia[i], r1 = ia[i + 1], p0 = jlu[i + 1];
+  // CSR position of the diagonal (every row has one; checked on the host)
+  int jd = INT_MAX;
+  for (int j = r0 + lane; j < r1; j += kWaveSize)
+    if (ja[j] == i) jd = j;
+  for (int off = kWaveSize / 2; off > 0; off /= 2) jd = min(jd, __shfl_xor(jd, off));
+  for (int j = r0 + lane; j < r1; j += kWaveSize) {
+    if (j == jd)
+      alu[i + 1] = z[j];
+    else
+      alu[p0 + (j - r0) - (j > jd ? 1 : 0)] = z[j];
+  }
+}
+
+template <typename S>
+struct IluCx;
+template <>
+struct IluCx<float> {
+  typedef float2 T;
+};
+template <>
+struct IluCx<double> {
+  typedef double2 T;
+};
+
+template <typename S>
+__global__ void __launch_bounds__(1024) k_ilu0_factor(int nlev, const int* __restrict__ lptr,
+                                                      const int* __restrict__ lrows, const int* __restrict__ jlu,
+                                                      const int* __restrict__ ju, typename IluCx<S>::T* alu,
+                                                      int maxrow, int* __restrict__ piv) {
+#pragma clang fp contract(off)
+  typedef typename IluCx<S>::T T2;
+  extern __shared__ char smem[];
+  const int nw = blockDim.x / kWaveSize;
+  const int wave = threadIdx.x / kWaveSize, lane = threadIdx.x % kWaveSize;
+  T2* dslot = reinterpret_cast<T2*>(smem) + wave;                        // the row's diagonal
+  T2* vals = reinterpret_cast<T2*>(smem) + nw + (size_t)wave * maxrow;  // its other entries
+  int* cols = reinterpret_cast<int*>(reinterpret_cast<T2*>(smem) + nw + (size_t)nw * maxrow) +
+              (size_t)wave * maxrow;
+  for (int l = 0; l < nlev; ++l) {
+    for (int r = lptr[l] + wave; r < lptr[l + 1]; r += nw) {
+      const int ii = lrows[r] + 1;
+      const int p0 = jlu[ii], len = jlu[ii + 1] - p0, pu = ju[ii];
+      for (int e = lane; e < len; e += kWaveSize) {
+        vals[e] = alu[p0 + e];
+        cols[e] = jlu[p0 + e];
+      }
+      if (lane == 0) *dslot = alu[ii];
+      wave_lds_sync();
+      for (int j = p0; j < pu; ++j) {
+        // tl = alu(j) * alu(jrow), in the storage precision
+        const int jrow = cols[j - p0];
+        const T2 a = vals[j - p0], b = alu[jrow];
+        T2 t;
+        t.x = a.x * b.x - a.y * b.y;
+        t.y = a.x * b.y + a.y * b.x;
+        wave_lds_sync();
+        if (lane == 0) vals[j - p0] = t;
+        const double tr = (double)t.x, ti = (double)t.y;
+        const int q1 = jlu[jrow + 1];
+        for (int jj = ju[jrow] + lane; jj < q1; jj += kWaveSize) {
+          const int c = jlu[jj];
+          T2* w = nullptr;
+          if (c == ii) {
+            w = dslot;
+          } else {
+            int lo = 0, hi = len - 1;
+            while (lo <= hi) {
+              const int mid = (lo + hi) >> 1, cm = cols[mid];
+              if (cm == c) {
+                w = vals + mid;
+                break;
+              }
+              if (cm < c)
+                lo = mid + 1;
+              else
+                hi = mid - 1;
+            }
+          }
+          if (w) {
+            const T2 u = alu[jj], wv = *w;
+            T2 o;
+            o.x = (S)((double)wv.x - (tr * (double)u.x - ti * (double)u.y));
+            o.y = (S)((double)wv.y - (tr * (double)u.y + ti * (double)u.x));
+            *w = o;
+          }
+        }
+        wave_lds_sync();
+      }
+      if (lane == 0) {
+        // 1/d with Smith's scaling, in double
+        const T2 d = *dslot;
+        const double dr = (double)d.x, di = (double)d.y;
+        if (dr == 0.0 && di == 0.0) atomicMin(piv, ii);
+        double qr, qi;
+        if (fabs(dr) >= fabs(di)) {
+          const double q = di / dr, den = dr + di * q;
+          qr = 1.0 / den;
+          qi = -q / den;
+        } else {
+          const double q = dr / di, den = dr * q + di;
+          qr = q / den;
+          qi = -1.0 / den;
+        }
+        T2 inv;
+        inv.x = (S)qr;
+        inv.y = (S)qi;
+        alu[ii] = inv;
+      }
+      for (int e = lane; e < len; e += kWaveSize) alu[p0 + e] = vals[e];
+      wave_lds_sync();
+    }
+    __syncthreads();  // the level's rows are final before the next level reads them
+  }
+}
+
+}  // namespace
+
+// MSR structure of the factor (the index arrays ilu0C builds, without values):
+// jlu(1) = n+2, row ii's off-diagonal entries at jlu(ii) .. jlu(ii+1)-1 in CSR
+// order, ju(ii) = the first entry with a column above ii.  Requires every row
+// to hold its diagonal and ascending columns.
+int ilu0_structure(int n, const std::vector<int>& ia, const std::vector<int>& ja, std::vector<int>& jlu,
+                   std::vector<int>& ju, int& maxrow) {
+  const long long nnz = ia[n];
+  jlu.assign((size_t)nnz + 2, 0);
+  ju.assign(n, 0);
+  maxrow = 0;
+  int p = n + 2;
+  jlu[1] = p;
+  for (int i = 0; i < n; ++i) {
+    bool diag = false;
+    for (int j = ia[i]; j < ia[i + 1]; ++j) {
+      if (j > ia[i] && ja[j] <= ja[j - 1]) {
+        set_error("ilu0: near-field columns of a row are not ascending");
+        return LW_ERR_STATE;
+      }
+      if (ja[j] == i) {
+        diag = true;
+        ju[i] = p;
+      } else {
+        jlu[p++] = ja[j] + 1;
+      }
+    }
+    if (!diag) {
+      set_error("ilu0: row " + std::to_string(i + 1) + " has no diagonal entry");
+      return LW_ERR_STATE;
+    }
+    jlu[i + 2] = p;
+    maxrow = std::max(maxrow, ia[i + 1] - ia[i] - 1);
+  }
+  return LW_OK;
+}
+
+// The factor of the near field on the device into h->d_alu (structure, levels
+// and device index arrays from upload_factor's layout).  *piv = 0 or the
+// first row with a zero pivot.
+int ilu0_device(Handle* h, int* piv) {
+  const int n = h->ne;
+  int maxrow = 0;
+  if (int rc = ilu0_structure(n, h->plan.ia, h->plan.ja, h->h_jlu, h->h_ju, maxrow)) return rc;
+  if (int rc = upload_structure(h)) return rc;
+  const size_t na = (size_t)h->nnz + 2;
+  const size_t esz = h->fp64_near ? sizeof(double2) : sizeof(float2);
+  // waves that fit the device's LDS per workgroup, each with one row
+  int dev = 0, lds = 0;
+  LW_HIP(hipGetDevice(&dev));
+  LW_HIP(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+  const size_t per_wave = (size_t)std::max(maxrow, 1) * (esz + sizeof(int)) + esz;
+  const int nw = (int)std::min<size_t>(kIluMaxWaves, lds / per_wave);
+  if (nw < 1) {
+    set_error("ilu0: a near-field row of " + std::to_string(maxrow) + " entries does not fit in LDS");
+    return LW_ERR_ARG;
+  }
+  const size_t shm = (size_t)nw * per_wave;
+  int* d_piv = nullptr;
+  LW_HIP(hipMalloc(&d_piv, sizeof(int)));
+  const int none = INT_MAX;
+  hipStream_t st = h->stream;
+  int rc = LW_OK;
+  auto fail = [&](hipError_t e) {
+    set_error(std::string("ilu0: ") + hipGetErrorString(e));
+    rc = LW_ERR_HIP;
+  };
+  hipError_t e = hipMemcpyAsync(d_piv, &none, sizeof(int), hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = hipMemsetAsync(h->d_alu, 0, esz * na, st);
+  if (e == hipSuccess) {
+    const int rows_per_block = 4;
+    if (h->fp64_near) {
+      hipLaunchKernelGGL(k_ilu0_scatter<double2>, dim3((n + rows_per_block - 1) / rows_per_block),
+                         dim3(rows_per_block * kWaveSize), 0, st, n, h->d_ia, h->d_ja, (const double2*)h->d_z,
+                         h->d_jlu, (double2*)h->d_alu);
+      hipLaunchKernelGGL(k_ilu0_factor<double>, dim3(1), dim3(nw * kWaveSize), shm, st, h->nlev_l, h->d_lptr,
+                         h->d_lrows, h->d_jlu, h->d_ju, (double2*)h->d_alu, maxrow, d_piv);
+    } else {
+      hipLaunchKernelGGL(k_ilu0_scatter<float2>, dim3((n + rows_per_block - 1) / rows_per_block),
+                         dim3(rows_per_block * kWaveSize), 0, st, n, h->d_ia, h->d_ja, (const float2*)h->d_z,
+                         h->d_jlu, (float2*)h->d_alu);
+      hipLaunchKernelGGL(k_ilu0_factor<float>, dim3(1), dim3(nw * kWaveSize), shm, st, h->nlev_l, h->d_lptr,
+                         h->d_lrows, h->d_jlu, h->d_ju, (float2*)h->d_alu, maxrow, d_piv);
+    }
+    e = hipGetLastError();
+  }
+  int p = none;
+  if (e == hipSuccess) e = hipMemcpyAsync(&p, d_piv, sizeof(int), hipMemcpyDeviceToHost, st);
+  // the host copy of the factor (lw_ilu0_get and the Fortran module arrays)
+  if (e == hipSuccess) {
+    if (h->fp64_near) {
+      h->h_alu64.assign(2 * na, 0.0);
+      e = hipMemcpyAsync(h->h_alu64.data(), h->d_alu, esz * na, hipMemcpyDeviceToHost, st);
+    } else {
+      h->h_alu32.assign(2 * na, 0.0f);
+      e = hipMemcpyAsync(h->h_alu32.data(), h->d_alu, esz * na, hipMemcpyDeviceToHost, st);
+    }
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  if (e != hipSuccess) fail(e);
+  (void)hipFree(d_piv);
+  if (rc) return rc;
+  *piv = p == none ? 0 : p;
+  return LW_OK;
+}
+
+}  // namespace lw

@@ -174,6 +174,13 @@ int launch_setup_transfer(Handle* h, int level, const double* d_binom2, const do
                           const int* d_tL, const double* d_th, int hstride);
 int launch_setup_near(Handle* h);
 
+// The MSR factor's structure (and value array) on the device, and the level
+// sets of both triangular sweeps (lw_device.hip).
+int upload_structure(Handle* h);
+// ILU(0) of the near field on the device (lw_ilu0.hip); *piv = 0 or the first
+// row with a zero pivot.
+int ilu0_device(Handle* h, int* piv);
+
 // One single-RHS matvec (MatxVec_MLFMA) on device vectors, on the handle's
 // stream (lw_device.hip).
 int run_matvec(Handle* h, const double2* x, double2* y, double2* y_near, hipEvent_t* ev);

@@ -545,101 +545,6 @@ int build_plan(const Geometry& g, double lambda, double ladomax, Plan& plan, std
 
 // ---------------------------------------------------------------- ILU0
 
-namespace {
-template <typename T>
-struct Cx {
-  T re, im;
-};
-}  // namespace
-
-template <typename T>
-int ilu0_msr(int n, const int* ia, const int* ja, const T* a, std::vector<T>& alu,
-             std::vector<int>& jlu, std::vector<int>& ju) {
-  // Saad's ILU(0) in the reference's form (modIterativo.f90:1901-1976):
-  // row ii is copied into MSR, then eliminated against earlier rows' U parts.
-  // With T = float the products alu(j)*alu(jrow) are formed in single
-  // precision and the updates in double, rounded back to single on store.
-  const long long nnz = ia[n];
-  alu.assign(2 * (size_t)(nnz + 2), T(0));
-  jlu.assign((size_t)nnz + 2, 0);
-  ju.assign(n, 0);
-  std::vector<int> iw((size_t)n + 1, 0);
-  auto get = [&](long long k, double& re, double& im) {
-    re = (double)alu[2 * k];
-    im = (double)alu[2 * k + 1];
-  };
-  auto set = [&](long long k, double re, double im) {
-    alu[2 * k] = (T)re;
-    alu[2 * k + 1] = (T)im;
-  };
-  int ju0 = n + 2;
-  jlu[1] = ju0;  // jlu(1)
-  for (int ii = 1; ii <= n; ++ii) {
-    const int js = ju0;
-    for (int j = ia[ii - 1]; j < ia[ii]; ++j) {
-      const int jcol = ja[j] + 1;
-      if (jcol == ii) {
-        alu[2 * ii] = a[2 * (size_t)j];
-        alu[2 * ii + 1] = a[2 * (size_t)j + 1];
-        iw[jcol] = ii;
-        ju[ii - 1] = ju0;
-      } else {
-        alu[2 * (size_t)ju0] = a[2 * (size_t)j];
-        alu[2 * (size_t)ju0 + 1] = a[2 * (size_t)j + 1];
-        jlu[ju0] = jcol;
-        iw[jcol] = ju0;
-        ++ju0;
-      }
-    }
-    jlu[ii + 1] = ju0;
-    const int jf = ju0 - 1, jm = ju[ii - 1] - 1;
-    for (int j = js; j <= jm; ++j) {
-      const int jrow = jlu[j];
-      double tr, ti;
-      {
-        const T ar = alu[2 * (size_t)j], ai = alu[2 * (size_t)j + 1];
-        const T br = alu[2 * (size_t)jrow], bi = alu[2 * (size_t)jrow + 1];
-        const T pr = ar * br - ai * bi, pi_ = ar * bi + ai * br;
-        tr = (double)pr;
-        ti = (double)pi_;
-      }
-      set(j, tr, ti);
-      for (int jj = ju[jrow - 1]; jj <= jlu[jrow + 1] - 1; ++jj) {
-        const int jw = iw[jlu[jj]];
-        if (jw != 0) {
-          double wr, wi, ur, ui;
-          get(jw, wr, wi);
-          get(jj, ur, ui);
-          set(jw, wr - (tr * ur - ti * ui), wi - (tr * ui + ti * ur));
-        }
-      }
-    }
-    double dr, di;
-    get(ii, dr, di);
-    if (dr == 0.0 && di == 0.0) return ii;
-    // 1/(dr + i di) with Smith's scaling
-    double qr, qi;
-    if (std::fabs(dr) >= std::fabs(di)) {
-      const double r = di / dr, den = dr + di * r;
-      qr = 1.0 / den;
-      qi = -r / den;
-    } else {
-      const double r = dr / di, den = dr * r + di;
-      qr = r / den;
-      qi = -1.0 / den;
-    }
-    set(ii, qr, qi);
-    iw[ii] = 0;
-    for (int i = js; i <= jf; ++i) iw[jlu[i]] = 0;
-  }
-  return 0;
-}
-
-template int ilu0_msr<float>(int, const int*, const int*, const float*, std::vector<float>&,
-                             std::vector<int>&, std::vector<int>&);
-template int ilu0_msr<double>(int, const int*, const int*, const double*, std::vector<double>&,
-                              std::vector<int>&, std::vector<int>&);
-
 void tri_levels(int n, const std::vector<int>& jlu, const std::vector<int>& ju, bool upper,
                 std::vector<int>& lev_ptr, std::vector<int>& lev_rows) {
   // jlu/ju are the 1-based MSR arrays stored with a leading pad (jlu[k] is

@@ -79,13 +79,6 @@ int build_plan(const Geometry& g, double lambda, double ladomax, Plan& plan, std
 // flang NORM2 of a 3-vector (one-pass scaled algorithm).
 double norm2_3(const double* v);
 
-// ILU0 (ilu0C, modIterativo.f90:1805-1976) in MSR form.  T is float (the
-// reference's complex(4) factor) or double.  a: CSR values (interleaved
-// complex, same precision as T).  Returns 0 or the 1-based zero-pivot row.
-template <typename T>
-int ilu0_msr(int n, const int* ia, const int* ja, const T* a, std::vector<T>& alu,
-             std::vector<int>& jlu, std::vector<int>& ju);
-
 // Level sets of the forward (L) and backward (U) sweeps of an MSR factor.
 void tri_levels(int n, const std::vector<int>& jlu, const std::vector<int>& ju, bool upper,
                 std::vector<int>& lev_ptr, std::vector<int>& lev_rows);

@@ -134,6 +134,32 @@ def test_ilu0_and_lusol(name):
 
 
 @pytest.mark.parametrize("name", list(CASES))
+def test_ilu0_factor_on_the_device_matches_the_sequential_oracle(name):
+    # the level-scheduled GPU factorisation against ilu0C's sequential order
+    # (the oracle) on the same complex(4) near-field values: identical
+    # structure; the values differ at most where the inverted diagonals'
+    # complex divisions (Smith's here, the C library's in the oracle) round
+    # differently
+    from _oracle import lib, _P
+    mesh, gpu, ora = setup_case(name)
+    assert gpu.ilu0() == 0
+    alu, jlu, ju = gpu.ilu0_factor()
+    ia, ja, z = gpu.near_csr()
+    nnz = len(ja)
+    alu_o = np.zeros(nnz + 1, np.complex128)
+    jlu_o = np.zeros(nnz + 1, np.int32)
+    ju_o = np.zeros(mesh.ne, np.int32)
+    ierr = lib().or_ilu0(mesh.ne, _P(ia), _P(ja), _P(np.ascontiguousarray(z)), 0, _P(alu_o), _P(jlu_o), _P(ju_o))
+    assert ierr == 0
+    assert np.array_equal(jlu, jlu_o) and np.array_equal(ju, ju_o)
+    same = np.mean(alu == alu_o)
+    d = np.abs(alu - alu_o) / np.maximum(np.abs(alu_o), 1e-30)
+    print(f"[ilu0 {name}] bit-identical entries {same:.6f}, max relative difference {d.max():.2e}")
+    assert same > 0.999
+    assert d.max() < 1e-5
+
+
+@pytest.mark.parametrize("name", list(CASES))
 @pytest.mark.parametrize("use_prec", [False, True])
 def test_cgs_first_iterations(name, use_prec):
     mesh, gpu, ora = setup_case(name)
