@@ -10,8 +10,8 @@
 // final: the rows of one level of the lower-triangular dependency graph are
 // independent.  One workgroup walks the levels; each wave factors one row at
 // a time with the row staged in LDS: per elimination step the wave's lanes
-// take the entries of jrow's U part, find their columns in row ii by binary
-// search over the row's sorted columns, and update them.  Every entry of row
+// take consecutive runs of jrow's U part, find their columns in row ii by
+// interleaved binary searches over the row's sorted columns, and update them.  Every entry of row
 // ii is updated in the same order and with the same arithmetic as the
 // sequential reference (products in the storage precision, updates in
 // double, contraction off), so the factor is bit-identical to it.
@@ -19,6 +19,7 @@
 
 #include <algorithm>
 #include <climits>
+#include <cstdio>
 #include <string>
 #include <vector>
 
@@ -29,6 +30,7 @@ namespace {
 
 constexpr int kWaveSize = 64;
 constexpr int kIluMaxWaves = 16;
+constexpr int kIluChunk = 4;  // consecutive U-part entries per lane and block
 
 // Orders the wave's LDS accesses: every lane's writes are complete before any
 // lane reads on.
@@ -36,6 +38,16 @@ __device__ __forceinline__ void wave_lds_sync() {
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+// The same for LDS only, leaving vector-memory loads in flight: the wave
+// waits for its LDS operations (lgkmcnt 0; vmcnt and expcnt at their maxima)
+// and the compiler moves no memory access across.
+__device__ __forceinline__ void wave_lds_order() {
+  asm volatile("" ::: "memory");
+  __builtin_amdgcn_s_waitcnt(0xC07F);
+  __builtin_amdgcn_wave_barrier();
+  asm volatile("" ::: "memory");
 }
 
 // CSR (natural order, 0-based) values into the MSR positions: one wave per
@@ -95,45 +107,100 @@ __global__ void __launch_bounds__(1024) k_ilu0_factor(int nlev, const int* __res
       }
       if (lane == 0) *dslot = alu[ii];
       wave_lds_sync();
+      // Per elimination step the lanes need jrow's U-part bounds, its
+      // inverted diagonal and their run of its U part: all final, so they are
+      // fetched ahead (bounds two steps, runs one step), with clamped
+      // addresses so that the loads are unconditional and stay in flight
+      // while the current step is applied.
+      int qa0 = 0, qa1 = 0, qb0 = 0, qb1 = 0;
+      T2 ba = {}, bb = {};
+      int cca[kIluChunk], ccb[kIluChunk];
+      T2 uua[kIluChunk], uub[kIluChunk];
+      auto meta = [&](int j, int& q0, int& q1, T2& b) {
+        const int jr = cols[min(j, pu - 1) - p0];
+        q0 = ju[jr];
+        q1 = jlu[jr + 1];
+        b = alu[jr];
+      };
+      auto load = [&](int b0, int q1, int(&cc)[kIluChunk], T2(&uu)[kIluChunk]) {
+#pragma unroll
+        for (int u = 0; u < kIluChunk; ++u) {
+          const int jj = b0 + lane * kIluChunk + u;
+          const int jc = min(jj, max(q1 - 1, 0));
+          const int c = jlu[jc];
+          uu[u] = alu[jc];
+          cc[u] = jj < q1 ? c : INT_MAX;
+        }
+      };
+      if (p0 < pu) {
+        meta(p0, qa0, qa1, ba);
+        load(qa0, qa1, cca, uua);
+        meta(p0 + 1, qb0, qb1, bb);
+      }
       for (int j = p0; j < pu; ++j) {
+        int qc0, qc1;
+        T2 bc;
+        load(qb0, qb1, ccb, uub);
+        meta(j + 2, qc0, qc1, bc);
         // tl = alu(j) * alu(jrow), in the storage precision
-        const int jrow = cols[j - p0];
-        const T2 a = vals[j - p0], b = alu[jrow];
+        const T2 a = vals[j - p0];
         T2 t;
-        t.x = a.x * b.x - a.y * b.y;
-        t.y = a.x * b.y + a.y * b.x;
-        wave_lds_sync();
+        t.x = a.x * ba.x - a.y * ba.y;
+        t.y = a.x * ba.y + a.y * ba.x;
+        wave_lds_order();
         if (lane == 0) vals[j - p0] = t;
         const double tr = (double)t.x, ti = (double)t.y;
-        const int q1 = jlu[jrow + 1];
-        for (int jj = ju[jrow] + lane; jj < q1; jj += kWaveSize) {
-          const int c = jlu[jj];
-          T2* w = nullptr;
-          if (c == ii) {
-            w = dslot;
-          } else {
-            int lo = 0, hi = len - 1;
-            while (lo <= hi) {
-              const int mid = (lo + hi) >> 1, cm = cols[mid];
-              if (cm == c) {
-                w = vals + mid;
-                break;
+        // row ii's columns above jrow start after position j - p0
+        const int lo0 = j - p0 + 1;
+        for (int b0 = qa0; b0 < qa1; b0 += kWaveSize * kIluChunk) {
+          if (b0 > qa0) load(b0, qa1, cca, uua);  // rows with more than 64*kIluChunk U entries
+          if (cca[0] != INT_MAX) {
+            // position of each of the lane's columns in row ii: interleaved
+            // branch-free lower-bound searches over the row's columns above
+            // jrow (a uniform number of halvings, so no lane waits on
+            // another's longer walk)
+            const int n0 = len - lo0;
+            int base[kIluChunk];
+#pragma unroll
+            for (int u = 0; u < kIluChunk; ++u) base[u] = lo0;
+            for (int nn = n0; nn > 1;) {
+              const int half = nn >> 1;
+#pragma unroll
+              for (int u = 0; u < kIluChunk; ++u) base[u] = cols[base[u] + half] < cca[u] ? base[u] + half : base[u];
+              nn -= half;
+            }
+#pragma unroll
+            for (int u = 0; u < kIluChunk; ++u) {
+              const int c = cca[u];
+              T2* w = nullptr;
+              if (c == ii) {
+                w = dslot;
+              } else if (c != INT_MAX && n0 > 0) {
+                const int pos = base[u] + (cols[base[u]] < c ? 1 : 0);
+                if (pos < len && cols[pos] == c) w = vals + pos;
               }
-              if (cm < c)
-                lo = mid + 1;
-              else
-                hi = mid - 1;
+              if (w) {
+                const T2 wv = *w;
+                T2 o;
+                o.x = (S)((double)wv.x - (tr * (double)uua[u].x - ti * (double)uua[u].y));
+                o.y = (S)((double)wv.y - (tr * (double)uua[u].y + ti * (double)uua[u].x));
+                *w = o;
+              }
             }
           }
-          if (w) {
-            const T2 u = alu[jj], wv = *w;
-            T2 o;
-            o.x = (S)((double)wv.x - (tr * (double)u.x - ti * (double)u.y));
-            o.y = (S)((double)wv.y - (tr * (double)u.y + ti * (double)u.x));
-            *w = o;
-          }
         }
-        wave_lds_sync();
+        wave_lds_order();
+        qa0 = qb0;
+        qa1 = qb1;
+        ba = bb;
+        qb0 = qc0;
+        qb1 = qc1;
+        bb = bc;
+#pragma unroll
+        for (int u = 0; u < kIluChunk; ++u) {
+          cca[u] = ccb[u];
+          uua[u] = uub[u];
+        }
       }
       if (lane == 0) {
         // 1/d with Smith's scaling, in double
@@ -211,9 +278,11 @@ int ilu0_device(Handle* h, int* piv) {
   const size_t na = (size_t)h->nnz + 2;
   const size_t esz = h->fp64_near ? sizeof(double2) : sizeof(float2);
   // waves that fit the device's LDS per workgroup, each with one row
-  int dev = 0, lds = 0;
+  int dev = 0, lds = 0, lds_optin = 0;
   LW_HIP(hipGetDevice(&dev));
   LW_HIP(hipDeviceGetAttribute(&lds, hipDeviceAttributeMaxSharedMemoryPerBlock, dev));
+  if (hipDeviceGetAttribute(&lds_optin, hipDeviceAttributeSharedMemPerBlockOptin, dev) == hipSuccess)
+    lds = std::max(lds, lds_optin);
   const size_t per_wave = (size_t)std::max(maxrow, 1) * (esz + sizeof(int)) + esz;
   const int nw = (int)std::min<size_t>(kIluMaxWaves, lds / per_wave);
   if (nw < 1) {
@@ -221,6 +290,14 @@ int ilu0_device(Handle* h, int* piv) {
     return LW_ERR_ARG;
   }
   const size_t shm = (size_t)nw * per_wave;
+  if (h->fp64_near)
+    LW_HIP(hipFuncSetAttribute((const void*)k_ilu0_factor<double>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)shm));
+  else
+    LW_HIP(hipFuncSetAttribute((const void*)k_ilu0_factor<float>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               (int)shm));
+  if (progress_every() > 0)
+    fprintf(stderr, "[lw_ilu0] %d levels, %d waves, %zu bytes of LDS\n", h->nlev_l, nw, shm);
   int* d_piv = nullptr;
   LW_HIP(hipMalloc(&d_piv, sizeof(int)));
   const int none = INT_MAX;

@@ -572,6 +572,16 @@ void tri_levels(int n, const std::vector<int>& jlu, const std::vector<int>& ju, 
   lev_rows.resize(n);
   std::vector<int> fill(lev_ptr.begin(), lev_ptr.end() - 1);
   for (int i = 0; i < n; ++i) lev_rows[fill[lev[i]]++] = i;
+  // within a level, rows with more entries on the swept side first (the
+  // rows of a level are independent; the factorisation deals them to its
+  // waves in this order)
+  for (int l = 0; l <= maxl; ++l) {
+    auto side = [&](int i) {
+      return upper ? jlu[i + 2] - ju[i] : ju[i] - jlu[i + 1];
+    };
+    std::stable_sort(lev_rows.begin() + lev_ptr[l], lev_rows.begin() + lev_ptr[l + 1],
+                     [&](int a, int b) { return side(a) > side(b); });
+  }
 }
 
 }  // namespace lw
