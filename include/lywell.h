@@ -111,7 +111,9 @@ int lw_near_csr64(lw_handle *h, int *ia, int *ja, double *z);
 /* ------------------------------------------------------------------ modIterativo */
 
 /* crear_precond ILU0 branch (main.f90:884-903) -> ilu0C (modIterativo.f90:1805)
- * on the handle's near CSR; the MSR factor stays on the device.
+ * on the handle's near CSR, factorised on the device (level-scheduled, in the
+ * reference's elimination order and arithmetic); the MSR factor stays on the
+ * device and a host copy is kept for lw_ilu0_get.
  * *ierr = 0 or the zero-pivot row (then LW_ERR_PIVOT is returned). */
 int lw_ilu0(lw_handle *h, int *ierr);
 /* Host copy of the MSR factor: alu(nnz+1) complex(4) (complex(8) with

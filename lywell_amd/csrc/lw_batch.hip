@@ -152,16 +152,23 @@ __global__ void __launch_bounds__(256) k_b_aggregate(int nk, int nb, const int* 
       const int nc = min(KC, nf - c0), nc4 = (nc + 3) & ~3;
       for (int ks = 0; ks < nc4; ks += 4) {
         const int sl = ks + lk;
-        const T2 a = (sl < nc && m < nk2) ? rad[(size_t)(s0 + c0 + sl) * nk2 + m] : mk<S>(0, 0);
+        // clamped, unconditional pattern load; out-of-range entries weigh 0
+        const T2 av = rad[(size_t)(s0 + c0 + min(sl, nc - 1)) * nk2 + min(m, nk2 - 1)];
+        const T2 a = (sl < nc && m < nk2) ? av : mk<S>(0, 0);
         const T2* xr = xs + sl * RS + li;
+        T2 x[NT];
 #pragma unroll
-        for (int jt = 0; jt < NT; ++jt) {
-          const T2 x = xr[jt * 16];
-          cr[jt] = mma(a.x, x.x, cr[jt]);
-          cr[jt] = mma(-a.y, x.y, cr[jt]);
-          ci[jt] = mma(a.x, x.y, ci[jt]);
-          ci[jt] = mma(a.y, x.x, ci[jt]);
-        }
+        for (int jt = 0; jt < NT; ++jt) x[jt] = xr[jt * 16];
+        // the 2*NT accumulators in turn, so that no MFMA waits on the one
+        // before it (each accumulator sees its terms in the same order)
+#pragma unroll
+        for (int jt = 0; jt < NT; ++jt) cr[jt] = mma(a.x, x[jt].x, cr[jt]);
+#pragma unroll
+        for (int jt = 0; jt < NT; ++jt) ci[jt] = mma(a.x, x[jt].y, ci[jt]);
+#pragma unroll
+        for (int jt = 0; jt < NT; ++jt) cr[jt] = mma(-a.y, x[jt].y, cr[jt]);
+#pragma unroll
+        for (int jt = 0; jt < NT; ++jt) ci[jt] = mma(a.y, x[jt].x, ci[jt]);
       }
     }
     if (tile < ntile) {
@@ -517,10 +524,10 @@ __global__ void k_b_wp(int nkc, int nwt, const int* __restrict__ wt_row, const d
 // Receive on MFMA, one workgroup per leaf box:
 //   Yf[basis(s)][r] = sum_m conj(rad(s; m)) A_L[m][b][r],  m = p*nk + k,
 // a complex GEMM with the 2nk items as the contraction.  A wave owns 16 of
-// the box's bases and 64 columns.  Each lane loads 4 consecutive items of one
-// pattern row at once (coalesced pieces), so within a block of 16 items, MFMA
-// step q pairs lane group g with item 4g + q; the A_L operand is fetched with
-// the same assignment.
+// the box's bases and 64 columns.  Each lane loads QB consecutive items of
+// one pattern row (coalesced pieces), so within a block of 4*QB items, MFMA
+// step q pairs lane group g with item QB*g + q; the A_L operand is fetched
+// with the same assignment.
 template <typename S, int R>
 __global__ void __launch_bounds__(128) k_b_receive(int nk, int nb, const int* __restrict__ box_slot,
                                                    const int* __restrict__ slot_basis,
@@ -547,28 +554,53 @@ __global__ void __launch_bounds__(128) k_b_receive(int nk, int nb, const int* __
       dr[jt] = Acc{0, 0, 0, 0};
       di[jt] = Acc{0, 0, 0, 0};
     }
-    for (int mb = 0; mb < nk2; mb += 16) {
-      T2 a4[4];
-      const int m0 = mb + 4 * lk;
+    // one block of 4*QB items (lane group g takes items QB*g .. QB*g+QB-1):
+    // the pattern pieces and the A_L operands, with clamped unconditional
+    // loads (out-of-range items weigh 0), fetched a block ahead of the MFMAs
+    // that use them
+    constexpr int QB = 2;
+    T2 aa[QB], ab[QB], xa[QB][NT], xb[QB][NT];
+    auto load = [&](int mb, T2(&a4)[QB], T2(&x)[QB][NT]) {
+      const int m0 = mb + QB * lk;
 #pragma unroll
-      for (int q = 0; q < 4; ++q) a4[q] = (row_ok && m0 + q < nk2) ? rrow[m0 + q] : mk<S>(0, 0);
+      for (int q = 0; q < QB; ++q) {
+        const int m = min(m0 + q, nk2 - 1);
+        const T2 av = rrow[m];
+        a4[q] = (row_ok && m0 + q < nk2) ? av : mk<S>(0, 0);
+        const int p = m / nk, k = m % nk;
+        const T2* ap = AL + (((size_t)k * nb + b) * 2 + p) * R + c0 + li;
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int m = m0 + q;  // item of this lane group in step q
-        const T2* ap = nullptr;
-        if (m < nk2) {
-          const int p = m / nk, k = m % nk;
-          ap = AL + (((size_t)k * nb + b) * 2 + p) * R + c0 + li;
-        }
-#pragma unroll
-        for (int jt = 0; jt < NT; ++jt) {
-          const T2 x = ap ? ap[jt * 16] : mk<S>(0, 0);
-          dr[jt] = mma(a4[q].x, x.x, dr[jt]);
-          dr[jt] = mma(a4[q].y, x.y, dr[jt]);
-          di[jt] = mma(a4[q].x, x.y, di[jt]);
-          di[jt] = mma(-a4[q].y, x.x, di[jt]);
-        }
+        for (int jt = 0; jt < NT; ++jt) x[q][jt] = ap[jt * 16];
       }
+    };
+    auto apply = [&](const T2(&a4)[QB], const T2(&x)[QB][NT]) {
+      // the 2*NT accumulators in turn, so that no MFMA waits on the one
+      // before it (each accumulator sees its terms in the same order)
+#pragma unroll
+      for (int q = 0; q < QB; ++q) {
+#pragma unroll
+        for (int jt = 0; jt < NT; ++jt) dr[jt] = mma(a4[q].x, x[q][jt].x, dr[jt]);
+#pragma unroll
+        for (int jt = 0; jt < NT; ++jt) di[jt] = mma(a4[q].x, x[q][jt].y, di[jt]);
+#pragma unroll
+        for (int jt = 0; jt < NT; ++jt) dr[jt] = mma(a4[q].y, x[q][jt].y, dr[jt]);
+#pragma unroll
+        for (int jt = 0; jt < NT; ++jt) di[jt] = mma(-a4[q].y, x[q][jt].x, di[jt]);
+      }
+    };
+    constexpr int MB = 4 * QB;
+    load(0, aa, xa);
+    for (int mb = 0;;) {
+      load(min(mb + MB, nk2 - 1), ab, xb);
+      __builtin_amdgcn_sched_barrier(0);
+      apply(aa, xa);
+      mb += MB;
+      if (mb >= nk2) break;
+      load(min(mb + MB, nk2 - 1), aa, xa);
+      __builtin_amdgcn_sched_barrier(0);
+      apply(ab, xb);
+      mb += MB;
+      if (mb >= nk2) break;
     }
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
