@@ -60,9 +60,9 @@ def main():
     ap.add_argument("--batch", type=int, default=64,
                     help="columns of the batched sweep (64 or 128; -1: autotuned; 0: skip the sweep)")
     ap.add_argument("--chain", type=int, default=3, help="angles per column chain of the sweep")
-    ap.add_argument("--no-steal", action="store_true",
-                    help="keep columns whose chain is done idle instead of letting them take the last pending "
-                         "angles of other chains")
+    ap.add_argument("--no-preempt", action="store_true",
+                    help="let every cold start of the sweep run to convergence instead of giving way to angles "
+                         "that have a guided guess")
     ap.add_argument("--batch-prec", choices=["fp64", "fp32"], default="fp64",
                     help="far-field precision of the batched sweep")
     ap.add_argument("--sweep", type=int, default=0, help="angles of the batched sweep (default: chain x columns)")
@@ -281,10 +281,10 @@ def batched_sweep(lw, op, args, plain_rhs_per_s):
     """The chosen form on the same mesh: a monostatic sweep of args.sweep
     angles (phi in [0, pi), theta = 90 deg, VV) solved as R chains of
     consecutive angles in one persistent batch (FP64 far field by default,
-    FP64 near field and CGS) to the same tolerance; each column restarts with
-    the next angle of its chain as soon as one converges (or, once its chain
-    is done, with the last pending angle of another chain), from the learned
-    guess of two solved neighbours (or the warm start of one).  Its sigma is checked against FP64 single-RHS
+    FP64 near field and CGS) to the same tolerance; each column restarts as
+    soon as it converges with the pending angle that has the best guess (the
+    learned guess of two solved neighbours, the warm start of one, or zero),
+    and slow cold starts give way to guided angles.  Its sigma is checked against FP64 single-RHS
     solves of two angles from zero guesses, and its bistatic pattern at the
     first angle on 361 theta points in the phi = 0 cut."""
     from lywell_amd import autotune, guess_train
@@ -302,7 +302,7 @@ def batched_sweep(lw, op, args, plain_rhs_per_s):
         log("no trained guess network found: using the warm start")
     n = args.sweep if args.sweep > 0 else args.chain * R
     sw = Sweep(op, bt, errmax=args.errmax, maxit=args.maxit, guess=guess, mlp=net, log=log)
-    res = sw.run_persistent(n, keep_currents=True, steal=not args.no_steal)
+    res = sw.run_persistent(n, keep_currents=True, preempt=not args.no_preempt)
     value = n / res["seconds"]
     m = (n + R - 1) // R
     plain_sweep_rhs_s, plain_its = plain_sweep(op, args, n, range(0, min(m, n)))
@@ -347,7 +347,7 @@ def batched_sweep(lw, op, args, plain_rhs_per_s):
             "guess": guess, "far_field": "FP64" if fp64 else "FP32", "autotune_ms": times,
             "matvec_ms": round(mv_ms, 3), "matvec_ms_per_rhs": round(mv_ms / R, 4),
             "iterations_mean": float(its.mean()), "iterations_max": int(its.max()),
-            "iterations_by_guess": by_kind, "restarts": res["restarts"], "steals": res["steals"],
+            "iterations_by_guess": by_kind, "restarts": res["restarts"], "preempted": res["preempted"],
             "converged": bool(res["converged"].all()),
             "monostatic_rcs_db_err_vs_fp64": [round(e, 5) for e in errs],
             "bistatic_rcs_db_err_vs_fp64_361": round(bistatic, 5),
@@ -390,7 +390,7 @@ def ellipsoid_sweep(lw, args):
     res = {}
     for pol in ("VV", "HH"):
         sw = Sweep(op, bt, errmax=args.errmax, maxit=args.maxit, guess=guess, mlp=net, pol=pol, log=log, span=span)
-        res[pol] = sw.run_persistent(n, keep_currents=False)
+        res[pol] = sw.run_persistent(n, keep_currents=False, preempt=not args.no_preempt)
     bt.close()
     seconds = sum(r["seconds"] for r in res.values())
     rhs_s = 2 * n / seconds
