@@ -60,9 +60,9 @@ def main():
     ap.add_argument("--batch", type=int, default=64,
                     help="columns of the batched sweep (64 or 128; -1: autotuned; 0: skip the sweep)")
     ap.add_argument("--chain", type=int, default=3, help="angles per column chain of the sweep")
-    ap.add_argument("--no-preempt", action="store_true",
-                    help="let every cold start of the sweep run to convergence instead of giving way to angles "
-                         "that have a guided guess")
+    ap.add_argument("--no-steal", action="store_true",
+                    help="keep columns whose chain is done idle instead of letting them take the last pending "
+                         "angles of other chains")
     ap.add_argument("--batch-prec", choices=["fp64", "fp32"], default="fp64",
                     help="far-field precision of the batched sweep")
     ap.add_argument("--sweep", type=int, default=0, help="angles of the batched sweep (default: chain x columns)")
@@ -77,6 +77,9 @@ def main():
                     help="also run the batched monostatic sweep of the 4 x 1 x 1 lambda prolate ellipsoid "
                          "(30,720 RWG) with this many azimuths per polarisation over 360 degrees, VV and HH "
                          "(0: skip)")
+    ap.add_argument("--uneven", type=int, default=0,
+                    help="also solve an uneven batch of this many seeded random bodies (1,920-30,720 RWG), "
+                         "concurrently and one after another (0: skip)")
     ap.add_argument("--matvec-only", action="store_true",
                     help="time the single and batched matvecs only (for kernel profiles), no solves")
     args = ap.parse_args()
@@ -206,6 +209,9 @@ def main():
     ellipsoid = None
     if args.ellipsoid_sweep > 0:
         ellipsoid = ellipsoid_sweep(lw, args)
+    uneven = None
+    if args.uneven > 0:
+        uneven = uneven_batch(lw, args)
 
     cpu = None
     if not args.no_cpu:
@@ -251,6 +257,8 @@ def main():
     }
     if ellipsoid is not None:
         out["ellipsoid_sweep"] = ellipsoid
+    if uneven is not None:
+        out["uneven_batch"] = uneven
     print(json.dumps(out))
 
 
@@ -281,10 +289,10 @@ def batched_sweep(lw, op, args, plain_rhs_per_s):
     """The chosen form on the same mesh: a monostatic sweep of args.sweep
     angles (phi in [0, pi), theta = 90 deg, VV) solved as R chains of
     consecutive angles in one persistent batch (FP64 far field by default,
-    FP64 near field and CGS) to the same tolerance; each column restarts as
-    soon as it converges with the pending angle that has the best guess (the
-    learned guess of two solved neighbours, the warm start of one, or zero),
-    and slow cold starts give way to guided angles.  Its sigma is checked against FP64 single-RHS
+    FP64 near field and CGS) to the same tolerance; each column restarts with
+    the next angle of its chain as soon as one converges (or, once its chain
+    is done, with the last pending angle of another chain), from the learned
+    guess of two solved neighbours (or the warm start of one).  Its sigma is checked against FP64 single-RHS
     solves of two angles from zero guesses, and its bistatic pattern at the
     first angle on 361 theta points in the phi = 0 cut."""
     from lywell_amd import autotune, guess_train
@@ -302,7 +310,7 @@ def batched_sweep(lw, op, args, plain_rhs_per_s):
         log("no trained guess network found: using the warm start")
     n = args.sweep if args.sweep > 0 else args.chain * R
     sw = Sweep(op, bt, errmax=args.errmax, maxit=args.maxit, guess=guess, mlp=net, log=log)
-    res = sw.run_persistent(n, keep_currents=True, preempt=not args.no_preempt)
+    res = sw.run_persistent(n, keep_currents=True, steal=not args.no_steal)
     value = n / res["seconds"]
     m = (n + R - 1) // R
     plain_sweep_rhs_s, plain_its = plain_sweep(op, args, n, range(0, min(m, n)))
@@ -347,7 +355,7 @@ def batched_sweep(lw, op, args, plain_rhs_per_s):
             "guess": guess, "far_field": "FP64" if fp64 else "FP32", "autotune_ms": times,
             "matvec_ms": round(mv_ms, 3), "matvec_ms_per_rhs": round(mv_ms / R, 4),
             "iterations_mean": float(its.mean()), "iterations_max": int(its.max()),
-            "iterations_by_guess": by_kind, "restarts": res["restarts"], "preempted": res["preempted"],
+            "iterations_by_guess": by_kind, "restarts": res["restarts"], "steals": res["steals"],
             "converged": bool(res["converged"].all()),
             "monostatic_rcs_db_err_vs_fp64": [round(e, 5) for e in errs],
             "bistatic_rcs_db_err_vs_fp64_361": round(bistatic, 5),
@@ -383,14 +391,19 @@ def ellipsoid_sweep(lw, args):
     fp64 = not cv32.all()
     log(f"ellipsoid FP32-far-field probe: {int(cv32.sum())}/64 columns converged within {probe_it} iterations "
         f"(mean {it32.mean():.0f}); sweep far field {'FP64' if fp64 else 'FP32'}")
+    # one chain per column starts cold, and cold starts dominate on this body:
+    # the sweep takes --batch columns (64 by default) and autotunes only with
+    # --batch -1
     best, times = autotune.tune(op, far_fp64=fp64, log=log)
+    if args.batch in (64, 128):
+        best = args.batch
     bt = lw.Batch(op, best, far_fp64=fp64)
     net = guess_train.load() if args.guess == "mlp" else None
     guess = args.guess if (args.guess != "mlp" or net is not None) else "warm"
     res = {}
     for pol in ("VV", "HH"):
         sw = Sweep(op, bt, errmax=args.errmax, maxit=args.maxit, guess=guess, mlp=net, pol=pol, log=log, span=span)
-        res[pol] = sw.run_persistent(n, keep_currents=False, preempt=not args.no_preempt)
+        res[pol] = sw.run_persistent(n, keep_currents=False, steal=not args.no_steal)
     bt.close()
     seconds = sum(r["seconds"] for r in res.values())
     rhs_s = 2 * n / seconds
@@ -433,6 +446,37 @@ def ellipsoid_sweep(lw, args):
             "guess": guess, "iterations_mean": {p: float(r["iterations"].mean()) for p, r in res.items()},
             "iterations_by_guess_kind": kinds, "converged": all(bool(r["converged"].all()) for r in res.values()),
             "rcs_db_err_vs_fp64": [round(e, 5) for e in errs]}
+
+
+def uneven_batch(lw, args):
+    """args.uneven seeded random PEC bodies (icosphere subdivision from
+    {3, 4, 5}, radius in [0.5, 2.5] lambda, 5% spherical-harmonic radial
+    perturbation), one random plane wave each, solved by CGSold without a
+    preconditioner to args.errmax: all bodies concurrently (16 host threads,
+    one stream per body), then one after another."""
+    from lywell_amd import uneven
+    t0 = time.time()
+    batch = uneven.bodies(args.uneven, seed=7)
+    ne = np.array([m["ne"] for _, _, m in batch])
+    log(f"uneven batch: {len(batch)} bodies, {ne.min()}-{ne.max()} RWG ({ne.sum()} in all), "
+        f"setup {time.time() - t0:.1f}s")
+    maxit = min(args.maxit, 20000)
+    t_con, res_con = uneven.solve(batch, errmax=args.errmax, maxit=maxit, workers=16)
+    t_seq, res_seq = uneven.solve(batch, errmax=args.errmax, maxit=maxit, workers=1)
+    its = np.array([r[0] for r in res_con])
+    conv = np.array([r[2] for r in res_con])
+    same = all(a[0] == b[0] for a, b in zip(res_con, res_seq))
+    log(f"uneven batch: concurrent {t_con:.1f}s, one after another {t_seq:.1f}s ({t_seq / t_con:.2f}x); "
+        f"{int(conv.sum())}/{len(batch)} converged within {maxit} iterations; iterations "
+        f"{int(its.min())}-{int(its.max())}; identical iteration counts in both runs: {same}")
+    for op, _, _ in batch:
+        op.close()
+    return {"bodies": len(batch), "unknowns_min": int(ne.min()), "unknowns_max": int(ne.max()),
+            "unknowns_total": int(ne.sum()), "seconds_concurrent": round(t_con, 2),
+            "seconds_one_after_another": round(t_seq, 2), "rhs_per_s_concurrent": round(len(batch) / t_con, 4),
+            "rhs_per_s_one_after_another": round(len(batch) / t_seq, 4), "speedup": round(t_seq / t_con, 2),
+            "converged": int(conv.sum()), "maxit": maxit, "iterations_mean": float(its.mean()),
+            "iterations_max": int(its.max()), "identical_iterations": bool(same)}
 
 
 def precision_probe(lw, op, args):

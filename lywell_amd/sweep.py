@@ -14,8 +14,8 @@ every chain keeps the reference's one-step warm start.  The guess of a block
 is zero (first block), that warm start, the linear extrapolation of the two
 previous angles' warm starts, or the learned guess (from the two previous
 angles of the chain).  The persistent form restarts each column as soon as it
-converges, with the pending angle that has the best guess from its solved
-neighbours, and can preempt slow cold starts.
+converges and lets columns that run out of work take angles from the end of
+other chains.
 """
 import time
 
@@ -48,10 +48,10 @@ class Sweep:
 
     run(): blocks of R angles, each block solved to completion before the
     next starts (the slowest column of a block sets its length).
-    run_persistent(): every column is restarted with a new angle as soon as
-    it has converged, while the other columns keep iterating
-    (lw_batch_restart_column), so no column waits for the block's slowest;
-    the angles are taken in the order that gives them the best guesses."""
+    run_persistent(): every column works through its own chain of
+    consecutive angles and is restarted with the next angle of its chain as
+    soon as it has converged, while the other columns keep iterating
+    (lw_batch_restart_column), so no column waits for the block's slowest."""
 
     def __init__(self, op, batch, errmax=1e-4, maxit=20000, guess="warm", mlp=None, pol="VV", log=None,
                  span=np.pi):
@@ -139,8 +139,8 @@ class Sweep:
         return nb
 
     def _neighbour_guess(self, a, solved, d, p, nb):
-        """Guess for angle a from the solved angles around it: the two on one
-        side (first those before it, the order the reference sweeps in) for
+        """Guess for angle a from the solved angles around it: the two before
+        it (the order the reference sweeps in) or else the two after it for
         the learned guess and the extrapolation, one neighbour's warm start,
         or zero.  Returns (guess, kind) with kind 0 zero, 1 one neighbour,
         2 two neighbours."""
@@ -160,32 +160,22 @@ class Sweep:
                 return xt, 1
         return None, 0
 
-    def _quality(self, a, solved, nb):
-        """The kind of guess angle a would start from now (see _neighbour_guess)."""
-        if self.guess == "zero":
-            return 0
-        if self.guess in ("mlp", "extrap"):
-            for s in (-1, 1):
-                if nb(a, s) in solved and nb(a, 2 * s) in solved:
-                    return 2
-        return 1 if (nb(a, -1) in solved or nb(a, 1) in solved) else 0
-
-    def run_persistent(self, n, keep_currents=False, check_every=8, preempt=True):
-        """All n angles through the R columns of one persistent batch.
-
-        The R columns start from zero guesses at evenly spread angles.  Each
-        time a column converges it takes the pending angle with the best
-        guess its solved neighbours allow (two neighbours on one side, or
-        one); with none available it waits (a cold start again only when no
-        column runs, at the angle farthest from the solved ones).  With preempt, a column still on a cold
-        start that has run more than twice the mean iterations of the guided
-        solves so far (and is not within 10x of the tolerance) gives way to a
-        pending angle that now has a guided guess; its own angle goes back to
-        the pending set.  On bodies whose cold starts take many times longer
-        than guided solves, the sweep then grows from the quickest cold starts
-        instead of waiting for the slowest."""
+    def run_persistent(self, n, keep_currents=False, check_every=8, steal=True):
+        """Column c starts chain c (angles c*n//R .. (c+1)*n//R - 1) and
+        works through it left to right, each angle from the guess its solved
+        neighbours allow.  With steal, a column without work takes the last
+        pending angle of the longest chain among those whose last pending
+        angle already has solved right-hand neighbours for the same kind of
+        guess (the next chain's first angles), so that columns do not idle
+        while others still have work.  The learned guess is trained from both
+        sides (its step-direction feature tells them apart), so a stolen
+        angle starts from as good a guess as its own chain would give it."""
         R = self.bt.R
         nb = self._neighbours(n)
+        m = (n + R - 1) // R
+        # chain lengths differ by at most one, so that every column has work
+        bounds = [c * n // R for c in range(R + 1)]
+        chains = [list(range(bounds[c], bounds[c + 1])) for c in range(R)]
         phi, dirs = sweep_directions(n, self.span)
         pols = polarisations(dirs, self.pol)
         cp = pols.astype(np.complex128)
@@ -195,19 +185,34 @@ class Sweep:
         kind = np.zeros(n, np.int8)
         currents = np.zeros((self.op.ne, n), np.complex128) if keep_currents else None
         solved = {}  # angle -> (direction, current)
-        pending = set(range(n))
         work = [-1] * R  # angle each column is solving
-        wkind = [0] * R  # kind of guess it started from
-        guided_its = []
-        preempted = 0
-        seeds = sorted({c * n // R for c in range(min(R, n))})
-        for c, a in enumerate(seeds):
-            work[c] = a
-            pending.discard(a)
-        first = np.array([a if a >= 0 else seeds[0] for a in work])  # idle columns repeat an angle
+        steals = 0
+        need = {"zero": 0, "warm": 1}.get(self.guess, 2)  # solved neighbours the guess uses
+
+        def steal_angle():
+            # the last pending angle of the longest chain among those whose
+            # last pending angle already has the guess its own chain would
+            # give it, from its right-hand neighbours
+            nonlocal steals
+            best = -1
+            for j in range(R):
+                if chains[j]:
+                    a = chains[j][-1]
+                    if all(nb(a, q) in solved for q in range(1, need + 1)) and (
+                            best < 0 or len(chains[j]) > len(chains[best])):
+                        best = j
+            if best < 0:
+                return -1
+            steals += 1
+            return chains[best].pop()
+
+        for c in range(R):
+            work[c] = chains[c].pop(0) if chains[c] else -1
+        first = np.array([a if a >= 0 else n - 1 for a in work])  # idle columns repeat an angle
         t0 = time.time()
         B = self.bt.plane_wave(dirs[first], cp[first])
         self.bt.cgs_start(B, errmax=self.errmax, maxit=self.maxit)
+        nsolved = 0
         restarts = 0
         next_log = R
 
@@ -216,23 +221,9 @@ class Sweep:
             b = self.op.plane_wave(dirs[a], cp[a])
             g, kind[a] = self._neighbour_guess(a, solved, dirs[a], pols[a], nb)
             self.bt.restart(c, b, g)
-            work[c], wkind[c] = a, int(kind[a])
-            pending.discard(a)
             restarts += 1
 
-        def coldest():
-            # the pending angle farthest (around the sweep) from every solved
-            # or running angle
-            taken = np.array(sorted(set(solved) | {a for a in work if a >= 0}))
-            cand = np.array(sorted(pending))
-            if taken.size == 0:
-                return int(cand[0])
-            dist = np.abs(cand[:, None] - taken[None, :])
-            if abs(self.span - 2 * np.pi) < 1e-9:
-                dist = np.minimum(dist, n - dist)
-            return int(cand[np.argmax(dist.min(axis=1))])
-
-        while len(solved) < n:
+        while any(a >= 0 for a in work):
             st, rr = self.bt.advance(check_every)
             for c in range(R):
                 a = work[c]
@@ -245,45 +236,24 @@ class Sweep:
                 if keep_currents:
                     currents[:, a] = x
                 solved[a] = (dirs[a], x)
-                if wkind[c] > 0:
-                    guided_its.append(int(st[c, 3]))
-                work[c] = -1
-            if len(solved) == n:
-                break
-            # pending angles by the guess they would get now
-            # (the better guess first; among equals the angle whose solved
-            # neighbour precedes it, so that each column tends to walk on in
-            # the sweep's direction and its next angle gets two neighbours)
-            q = {a: self._quality(a, solved, nb) for a in pending}
-            guided = sorted((a for a in pending if q[a] > 0),
-                            key=lambda a: (-q[a], 0 if nb(a, -1) in solved else 1, a))
-            for c in range(R):
-                if work[c] >= 0 or not pending:
-                    continue
-                if guided:
-                    start(c, guided.pop(0))
-                elif all(w < 0 for w in work):
-                    # only when nothing runs: otherwise a free column waits
-                    # for a guided angle rather than start another cold one
-                    start(c, coldest())
-            if preempt and guided and guided_its:
-                limit = 2.0 * float(np.mean(guided_its))
-                cold = sorted((c for c in range(R) if work[c] >= 0 and wkind[c] == 0 and st[c, 0]
-                               and st[c, 3] > limit and rr[c] > 10 * self.errmax), key=lambda c: -st[c, 3])
-                for c in cold:
-                    if not guided:
-                        break
-                    pending.add(work[c])
-                    start(c, guided.pop(0))
-                    preempted += 1
-            if len(solved) >= next_log:
-                self.log(f"sweep: {len(solved)}/{n} angles solved, {preempted} cold starts preempted, "
-                         f"{time.time() - t0:.1f}s")
+                nsolved += 1
+                work[c] = chains[c].pop(0) if chains[c] else -1
+                if work[c] >= 0:
+                    start(c, work[c])
+            if steal:
+                for c in range(R):
+                    if work[c] < 0:
+                        work[c] = steal_angle()
+                        if work[c] < 0:
+                            break
+                        start(c, work[c])
+            if nsolved >= next_log:
+                self.log(f"sweep: {nsolved}/{n} angles solved, {time.time() - t0:.1f}s")
                 next_log += R
         wall = time.time() - t0
-        self.log(f"sweep ({self.guess} guess, persistent, {preempted} cold starts preempted): {n} angles in "
-                 f"{wall:.2f}s, mean {iters.mean():.0f} iterations per angle")
+        self.log(f"sweep ({self.guess} guess, persistent, {steals} steals): {n} angles in {wall:.2f}s, "
+                 f"mean {iters.mean():.0f} iterations per angle")
         return {"phi": phi, "sigma": sigma, "iterations": iters, "converged": conv, "seconds": wall,
-                "restarts": restarts, "preempted": preempted, "guess_kind": kind, "currents": currents,
-                "blocks": [{"columns": R}]}
+                "restarts": restarts, "steals": steals, "guess_kind": kind, "currents": currents,
+                "blocks": [{"chain_length": m, "columns": R}]}
 

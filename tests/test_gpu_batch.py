@@ -194,10 +194,11 @@ def test_batch_restart_column():
 
 
 @pytest.mark.parametrize("guess", ["extrap", "mlp"])
-def test_persistent_sweep(guess):
+def test_persistent_sweep_with_steals(guess):
     # every angle of a sweep whose length is not a multiple of the block
-    # width is solved to the tolerance, each column taking the angle with the
-    # best guess when it frees up; sigma matches FP64 single-RHS solves
+    # width is solved to the tolerance; columns whose chain is done take
+    # angles from the ends of other chains; sigma matches FP64 single-RHS
+    # solves
     from lywell_amd import guess_train
     from lywell_amd.sweep import Sweep, polarisations, sweep_directions
     mesh, op, bt = sphere_batch(True)
@@ -206,17 +207,15 @@ def test_persistent_sweep(guess):
         pytest.skip("no trained guess network")
     n = 3 * bt.R + 5
     sw = Sweep(op, bt, errmax=1e-4, maxit=3000, guess=guess, mlp=net)
-    res = sw.run_persistent(n, keep_currents=True)
+    res = sw.run_persistent(n, keep_currents=True, steal=True)
     kinds = res["guess_kind"]
     its = res["iterations"]
-    print(f"[sweep {guess}] {n} angles in {res['seconds']:.1f}s, {res['preempted']} preempted, iterations by "
-          f"guess kind", {int(q): float(its[kinds == q].mean()) for q in range(3) if (kinds == q).any()})
+    print(f"[sweep {guess}] {n} angles in {res['seconds']:.1f}s, {res['steals']} steals, iterations by guess kind",
+          {int(q): float(its[kinds == q].mean()) for q in range(3) if (kinds == q).any()})
     assert res["converged"].all() and (its > 0).all()
-    # the R columns start cold at spread angles; every other angle starts
-    # from its solved neighbours, most of them from two
-    assert (kinds == 0).sum() <= bt.R
-    assert (kinds == 2).sum() > n // 4
-    assert its[kinds > 0].mean() < its[kinds == 0].mean()
+    assert res["steals"] > 0
+    # guesses from solved neighbours save iterations over cold starts
+    assert (kinds == 2).any() and its[kinds == 2].mean() < its[kinds == 0].mean()
     phi, dirs = sweep_directions(n)
     pols = polarisations(dirs, "VV").astype(np.complex128)
     for a in (0, n // 2, n - 1):
