@@ -192,6 +192,7 @@ def main():
                 "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
                 "algorithmic_bytes_per_launch": alg / nl, "mean_launch_ms": round(dom_ms / nl, 4),
                 "launches_per_matvec": nl}
+    roofline["traffic"] = pmc_traffic(kernel_of[dom], args)
     copy_ms, copy_gbs = op.bench_copy(1 << 30, reps=5)
     roofline["measured_copy_GBps"] = round(copy_gbs, 1)
     roofline["frac_of_measured_copy"] = round(achieved / copy_gbs, 4)
@@ -260,6 +261,25 @@ def main():
     if uneven is not None:
         out["uneven_batch"] = uneven
     print(json.dumps(out))
+
+
+def pmc_traffic(kernel, args):
+    """HBM bytes per launch of `kernel` on the default workload, from the
+    committed rocprofv3 PMC summary (FETCH_SIZE and WRITE_SIZE per dispatch,
+    calibrated on the library's copy kernel), or None for another workload
+    or kernel."""
+    if (args.subdiv, args.radius, args.ladomax) != (6, 4.0, 0.25):
+        return None
+    path = os.path.join(ROOT, "profiles", "hbm_traffic_matvecs_sphere122880.md")
+    try:
+        with open(path) as f:
+            for line in f:
+                cells = [c.strip() for c in line.strip().strip("|").split("|")]
+                if len(cells) == 6 and cells[0].strip("`").split("<")[0] == kernel:
+                    return round((float(cells[4]) + float(cells[5])) * 1e6)
+    except (OSError, ValueError):
+        return None
+    return None
 
 
 def plain_sweep(op, args, n, chain):
