@@ -77,6 +77,8 @@ def main():
                     help="also run the batched monostatic sweep of the 4 x 1 x 1 lambda prolate ellipsoid "
                          "(30,720 RWG) with this many azimuths per polarisation over 360 degrees, VV and HH "
                          "(0: skip)")
+    ap.add_argument("--ellipsoid-batch", type=int, default=64,
+                    help="columns of the ellipsoid sweep (64 or 128; -1: autotuned)")
     ap.add_argument("--uneven", type=int, default=0,
                     help="also solve an uneven batch of this many seeded random bodies (1,920-30,720 RWG), "
                          "concurrently and one after another (0: skip)")
@@ -412,11 +414,11 @@ def ellipsoid_sweep(lw, args):
     log(f"ellipsoid FP32-far-field probe: {int(cv32.sum())}/64 columns converged within {probe_it} iterations "
         f"(mean {it32.mean():.0f}); sweep far field {'FP64' if fp64 else 'FP32'}")
     # one chain per column starts cold, and cold starts dominate on this body:
-    # the sweep takes --batch columns (64 by default) and autotunes only with
-    # --batch -1
+    # the sweep takes --ellipsoid-batch columns (64 by default) and autotunes
+    # only with -1
     best, times = autotune.tune(op, far_fp64=fp64, log=log)
-    if args.batch in (64, 128):
-        best = args.batch
+    if args.ellipsoid_batch in (64, 128):
+        best = args.ellipsoid_batch
     bt = lw.Batch(op, best, far_fp64=fp64)
     net = guess_train.load() if args.guess == "mlp" else None
     guess = args.guess if (args.guess != "mlp" or net is not None) else "warm"

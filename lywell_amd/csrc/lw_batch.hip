@@ -618,10 +618,10 @@ __global__ void __launch_bounds__(128) k_b_receive(int nk, int nb, const int* __
 // row of a box has the same near columns (the bases of the box's near boxes,
 // ascending), so the box's block of the near matrix is a dense (nf x ncol)
 // matrix applied to ncol rows of X: a complex GEMM on FP64 MFMA.  A wave owns
-// 16 of the box's rows and 64 right-hand sides; each lane loads 4
-// consecutive matrix entries of its row at once, so within a block of 16
-// columns MFMA step q pairs lane group g with column 4g + q, and the X rows
-// are fetched with the same assignment.  The stored values (complex(4) or
+// 16 of the box's rows and 64 right-hand sides; each lane loads QB
+// consecutive matrix entries of its row, so within a block of 4*QB columns
+// MFMA step q pairs lane group g with column QB*g + q, and the X rows are
+// fetched with the same assignment.  The stored values (complex(4) or
 // complex(8)) are widened to FP64; accumulation is FP64.
 template <typename V, typename F>
 __global__ void __launch_bounds__(128) k_b_near_box(int R, const int* __restrict__ box_slot,
@@ -649,29 +649,49 @@ __global__ void __launch_bounds__(128) k_b_near_box(int R, const int* __restrict
       dr[jt] = f64x4v{0, 0, 0, 0};
       di[jt] = f64x4v{0, 0, 0, 0};
     }
-    for (int jb = 0; jb < ncol; jb += 16) {
-      const int jl = jb + 4 * lk;
-      double2 a4[4];
-      const double2* xp[4];
+    // blocks of 4*QB columns (lane group g takes columns QB*g .. QB*g+QB-1):
+    // the matrix entries and the X rows, with clamped unconditional loads
+    // (out-of-range entries weigh 0), fetched a block ahead of the MFMAs
+    constexpr int QB = 2, CB = 4 * QB;
+    double2 aa[QB], ab[QB], xa[QB][NT], xb[QB][NT];
+    auto load = [&](int jb, double2(&a4)[QB], double2(&x)[QB][NT]) {
+      const int jl = jb + QB * lk;
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const bool ok = jl + q < ncol;
-        a4[q] = (row_ok && ok) ? to_d2(zrow[jl + q]) : make_double2(0.0, 0.0);
-        xp[q] = ok ? X + (size_t)cb[jl + q] * R + c0 + li : nullptr;
+      for (int q = 0; q < QB; ++q) {
+        const int j = min(jl + q, ncol - 1);
+        const double2 av = to_d2(zrow[j]);
+        a4[q] = (row_ok && jl + q < ncol) ? av : make_double2(0.0, 0.0);
+        const double2* xp = X + (size_t)cb[j] * R + c0 + li;
+#pragma unroll
+        for (int jt = 0; jt < NT; ++jt) x[q][jt] = xp[jt * 16];
       }
+    };
+    auto apply = [&](const double2(&a4)[QB], const double2(&x)[QB][NT]) {
+      // the 2*NT accumulators in turn (each sees its terms in the same order)
 #pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        double2 x[NT];
+      for (int q = 0; q < QB; ++q) {
 #pragma unroll
-        for (int jt = 0; jt < NT; ++jt) x[jt] = xp[q] ? xp[q][jt * 16] : make_double2(0.0, 0.0);
+        for (int jt = 0; jt < NT; ++jt) dr[jt] = mma(a4[q].x, x[q][jt].x, dr[jt]);
 #pragma unroll
-        for (int jt = 0; jt < NT; ++jt) {
-          dr[jt] = mma(a4[q].x, x[jt].x, dr[jt]);
-          dr[jt] = mma(-a4[q].y, x[jt].y, dr[jt]);
-          di[jt] = mma(a4[q].x, x[jt].y, di[jt]);
-          di[jt] = mma(a4[q].y, x[jt].x, di[jt]);
-        }
+        for (int jt = 0; jt < NT; ++jt) di[jt] = mma(a4[q].x, x[q][jt].y, di[jt]);
+#pragma unroll
+        for (int jt = 0; jt < NT; ++jt) dr[jt] = mma(-a4[q].y, x[q][jt].y, dr[jt]);
+#pragma unroll
+        for (int jt = 0; jt < NT; ++jt) di[jt] = mma(a4[q].y, x[q][jt].x, di[jt]);
       }
+    };
+    load(0, aa, xa);
+    for (int jb = 0;;) {
+      load(min(jb + CB, ncol - 1), ab, xb);
+      __builtin_amdgcn_sched_barrier(0);
+      apply(aa, xa);
+      jb += CB;
+      if (jb >= ncol) break;
+      load(min(jb + CB, ncol - 1), aa, xa);
+      __builtin_amdgcn_sched_barrier(0);
+      apply(ab, xb);
+      jb += CB;
+      if (jb >= ncol) break;
     }
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
