@@ -246,6 +246,7 @@ def main():
         "true_relres": true_res,
         "ms_per_iteration": round(per_iter_ms, 3),
         "matvec_ms": round(mv_ms, 4),
+        "matvecs_per_s": round(1e3 / mv_ms, 1),
         "matvec_stage_ms": dict(zip(stage_names, [round(float(s), 4) for s in stages])),
         "matvec_GBps_algorithmic": round(matvec_gbs, 1),
         "stage_GBps_algorithmic": stage_gbs,
@@ -348,6 +349,11 @@ def batched_sweep(lw, op, args, plain_rhs_per_s):
         ref[a] = x
         log(f"  angle {a}: FP64 single solve {it64} iterations, batched {int(res['iterations'][a])}; "
             f"monostatic sigma difference {errs[-1]:.4f} dB")
+    # true final residual of the batched solutions with the FP64 operator
+    true_rr = []
+    for a in (0, n // 2):
+        b = op.plane_wave(dirs[a], pols[a])
+        true_rr.append(float(np.linalg.norm(op.matvec(res["currents"][:, a]) - b) / np.linalg.norm(b)))
     theta = np.linspace(0.0, np.pi, 361)
     sb = op.rcs(res["currents"][:, 0], pols[0], theta, 0.0)
     s1 = op.rcs(ref[0], pols[0], theta, 0.0)
@@ -380,6 +386,8 @@ def batched_sweep(lw, op, args, plain_rhs_per_s):
             "iterations_by_guess": by_kind, "restarts": res["restarts"], "steals": res["steals"],
             "converged": bool(res["converged"].all()),
             "monostatic_rcs_db_err_vs_fp64": [round(e, 5) for e in errs],
+            "true_relres_angles_0_and_half": true_rr,
+            "rhs_matvecs_per_s": round(R / (mv_ms * 1e-3), 1),
             "bistatic_rcs_db_err_vs_fp64_361": round(bistatic, 5),
             "precision": f"{'FP64' if fp64 else 'FP32'} far field (aggregation, interpolation, translation, "
                          "receive), FP64 near field and CGS"}
