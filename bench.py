@@ -59,7 +59,7 @@ def main():
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
     ap.add_argument("--batch", type=int, default=64,
                     help="columns of the batched sweep (64 or 128; -1: autotuned; 0: skip the sweep)")
-    ap.add_argument("--chain", type=int, default=3, help="angles per column chain of the sweep")
+    ap.add_argument("--chain", type=int, default=6, help="angles per column chain of the sweep")
     ap.add_argument("--no-steal", action="store_true",
                     help="keep columns whose chain is done idle instead of letting them take the last pending "
                          "angles of other chains")
