@@ -1116,7 +1116,7 @@ static int upload_b(Batch* b, T** p, const std::vector<T>& v) {
 struct FarBlocks {
   std::vector<int> bo, q_ptr, code_tr;
   std::vector<int2> q_ent;
-  std::vector<int> p_mask, p_child, bcd;  // per parent: child octants present, child per octant; per box: code part
+  std::vector<int> bcd;                   // per box: offset-code part of its octant
   std::vector<int> ps_ptr;                // per parent: flat padded source list (see k_b_far)
   std::vector<int2> ps_ent;
 };
@@ -1233,17 +1233,16 @@ static bool far_blocks(const Plan& P, int n, FarBlocks& fb, std::string& msg) {
     while ((fb.ps_ent.size() - fb.ps_ptr[Pb]) % kSrcU) fb.ps_ent.push_back(make_int2(pp.child_ptr[Pb], kPadCode));
     fb.ps_ptr[Pb + 1] = (int)fb.ps_ent.size();
   }
-  fb.p_mask.assign(nP, 0);
-  fb.p_child.assign(8 * (size_t)nP, 0);
-  for (int Pb = 0; Pb < nP; ++Pb)
+  for (int Pb = 0; Pb < nP; ++Pb) {
+    int mask = 0;
     for (int c = pp.child_ptr[Pb]; c < pp.child_ptr[Pb + 1]; ++c) {
-      if (fb.p_mask[Pb] & (1 << fb.bo[c])) {
+      if (mask & (1 << fb.bo[c])) {
         msg = "two children of one parent in the same octant";
         return false;
       }
-      fb.p_mask[Pb] |= 1 << fb.bo[c];
-      fb.p_child[8 * (size_t)Pb + fb.bo[c]] = c;
+      mask |= 1 << fb.bo[c];
     }
+  }
   return true;
 }
 
