@@ -66,7 +66,7 @@ def main():
     ap.add_argument("--batch-prec", choices=["fp64", "fp32"], default="fp64",
                     help="far-field precision of the batched sweep")
     ap.add_argument("--sweep", type=int, default=0, help="angles of the batched sweep (default: chain x columns)")
-    ap.add_argument("--guess", choices=["zero", "warm", "extrap", "mlp"], default="mlp",
+    ap.add_argument("--guess", choices=["zero", "warm", "extrap", "quad", "mlp"], default="mlp",
                     help="initial guess of each later angle of a chain: zero, the reference's warm start, the linear "
                          "extrapolation of two warm starts, or the learned guess (mlp falls back to warm without "
                          "weights)")
@@ -362,16 +362,16 @@ def batched_sweep(lw, op, args, plain_rhs_per_s):
     bt.close()
     its = res["iterations"]
     kinds = res["guess_kind"]
-    names = {0: "zero", 1: "one_neighbour_warm_start", 2: f"two_neighbour_{guess}"}
+    names = {0: "zero", 1: "one_neighbour_warm_start", 2: f"two_neighbour_{guess}", 3: "three_neighbour_quad"}
     by_kind = {names[q]: {"angles": int((kinds == q).sum()), "iterations_mean": float(its[kinds == q].mean())}
-               for q in range(3) if (kinds == q).any()}
+               for q in range(4) if (kinds == q).any()}
     # steady state of a long sweep: every angle starts from the two-neighbour
     # guess; time per batched iteration from this run
     s_per_iter = res["seconds"] / max(1.0, float(its.sum()) / R)
-    late = its[kinds == 2] if (kinds == 2).any() else its
+    late = its[kinds >= 2] if (kinds >= 2).any() else its
     steady = R / (float(late.mean()) * s_per_iter)
     log(f"iterations by guess: {by_kind}")
-    log(f"steady state (two-neighbour guesses): {late.mean():.0f} iterations per angle, "
+    log(f"steady state (guesses from two or more neighbours): {late.mean():.0f} iterations per angle, "
         f"{s_per_iter * 1e3:.1f} ms per batched iteration -> {steady:.3f} RHS/s")
     return {"rhs_per_s": round(value, 4), "speedup_vs_plain_sweep": round(value / plain_sweep_rhs_s, 2),
             "steady_state_rhs_per_s": round(steady, 4),

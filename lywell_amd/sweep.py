@@ -55,8 +55,8 @@ class Sweep:
 
     def __init__(self, op, batch, errmax=1e-4, maxit=20000, guess="warm", mlp=None, pol="VV", log=None,
                  span=np.pi):
-        if guess not in ("zero", "warm", "extrap", "mlp"):
-            raise ValueError("guess: zero, warm, extrap or mlp")
+        if guess not in ("zero", "warm", "extrap", "quad", "mlp"):
+            raise ValueError("guess: zero, warm, extrap, quad or mlp")
         if guess == "mlp" and mlp is None:
             raise ValueError("the learned guess needs a trained network")
         self.op, self.bt = op, batch
@@ -141,17 +141,30 @@ class Sweep:
     def _neighbour_guess(self, a, solved, d, p, nb):
         """Guess for angle a from the solved angles around it: the two before
         it (the order the reference sweeps in) or else the two after it for
-        the learned guess and the extrapolation, one neighbour's warm start,
-        or zero.  Returns (guess, kind) with kind 0 zero, 1 one neighbour,
-        2 two neighbours."""
+        the learned guess and the extrapolation (three for the quadratic
+        extrapolation), one neighbour's warm start, or zero.  Returns (guess,
+        kind) with kind 0 zero, 1 one neighbour, 2 two, 3 three neighbours."""
         if self.guess == "zero":
             return None, 0
-        if self.guess in ("mlp", "extrap"):
+        if self.guess == "quad":
+            # quadratic extrapolation of three phase-corrected neighbours
+            for s in (-1, 1):
+                aa = [nb(a, q * s) for q in (1, 2, 3)]
+                if all(x in solved for x in aa):
+                    xt = [warm_start(solved[x][1][:, None], self.op.mesh.e_centro, self.k0, d[None, :],
+                                     solved[x][0][None, :])[:, 0] for x in aa]
+                    return 3.0 * xt[0] - 3.0 * xt[1] + xt[2], 3
+        if self.guess in ("mlp", "extrap", "quad"):
             for s in (-1, 1):
                 a1, a2 = nb(a, s), nb(a, 2 * s)
                 if a1 in solved and a2 in solved:
                     (d1, x1), (d2, x2) = solved[a1], solved[a2]
-                    return self._guess([(d2, x2), (d1, x1)], d, p), 2
+                    g = self._guess([(d2, x2), (d1, x1)], d, p) if self.guess != "quad" else None
+                    if g is None:
+                        xt1 = warm_start(x1[:, None], self.op.mesh.e_centro, self.k0, d[None, :], d1[None, :])[:, 0]
+                        xt2 = warm_start(x2[:, None], self.op.mesh.e_centro, self.k0, d[None, :], d2[None, :])[:, 0]
+                        g = 2.0 * xt1 - xt2
+                    return g, 2
         for s in (-1, 1):
             a1 = nb(a, s)
             if a1 in solved:
