@@ -174,7 +174,7 @@ def main():
     dom = max(cand, key=cand.get)
     nnz = op.nnz
     sbytes = op.matvec_stage_bytes()
-    kernel_of = {"aggregation": "k_aggregate", "interp_up": "k_interp_up", "translation": "k_translate",
+    kernel_of = {"aggregation": "k_aggregate", "interp_up": "k_interp_up", "translation": "k_translate_parent",
                  "interp_down": "k_interp_down", "receive_near": "k_receive_near", "lusol": "k_lusol"}
     launches = {"aggregation": 1, "interp_up": op.nlevels - 3, "translation": op.nlevels - 2,
                 "interp_down": op.nlevels - 3, "receive_near": 1, "lusol": 1}

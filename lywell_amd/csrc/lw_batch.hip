@@ -451,30 +451,20 @@ __device__ __forceinline__ void far_parent(int k, int P, int col, LW_FAR_PARAMS)
 }
 
 template <typename S>
-__global__ void __launch_bounds__(256) k_b_far(int nk, int nP, int nchunk, int wpx, int split_p, LW_FAR_PARAMS) {
-  // Two workgroup maps (both: within a workgroup one sample k and one column
-  // chunk, 4 waves = 4 consecutive parents):
-  //  split_p: XCD x owns the contiguous parent range [x*nPx, (x+1)*nPx) for
-  //           every sample; its L2 then holds the sources of a box range
-  //           (~1/8 of a k-slice plus a halo) instead of whole k-slices;
-  //  else:    (few parents) XCD x owns the samples k = x mod 8.
+__global__ void __launch_bounds__(256) k_b_far(int nk, int nP, int nchunk, int wpx, int nsplit, LW_FAR_PARAMS) {
+  // Workgroup map: within a workgroup one sample k and one column chunk,
+  // 4 waves = 4 consecutive parents.  The parents are cut into nsplit
+  // contiguous ranges (nsplit divides 8) and XCD x owns range x % nsplit for
+  // the samples k = x / nsplit mod 8 / nsplit, so that its L2 holds the
+  // sources of a box range (1/nsplit of a k-slice plus a halo) for the few
+  // samples it has in flight.  nsplit = 1 gives each XCD whole k-slices,
+  // nsplit = 8 a range of every k-slice.
   const int g = blockIdx.x, x = g % 8, j = g / 8;
-  int k, P0, P1, pb, chunk;
-  if (split_p) {
-    const int nPx = (nP + 7) / 8, nPBx = wpx / nchunk;
-    k = j / wpx;
-    chunk = (j % wpx) / nPBx;
-    pb = (j % wpx) % nPBx;
-    P0 = x * nPx + pb * 4;
-    P1 = min((x + 1) * nPx, nP);
-  } else {
-    k = (j / wpx) * 8 + x;
-    const int nPB = wpx / nchunk;
-    chunk = (j % wpx) / nPB;
-    pb = (j % wpx) % nPB;
-    P0 = pb * 4;
-    P1 = nP;
-  }
+  const int kstep = 8 / nsplit, pr = x % nsplit;
+  const int nPx = (nP + nsplit - 1) / nsplit, nPBx = wpx / nchunk;
+  const int k = (j / wpx) * kstep + x / nsplit;
+  const int chunk = (j % wpx) / nPBx, pb = (j % wpx) % nPBx;
+  const int P0 = pr * nPx + pb * 4, P1 = min((pr + 1) * nPx, nP);
   if (k >= nk) return;  // uniform over the workgroup
   const int P = P0 + __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
   if (P >= P1) return;
@@ -490,6 +480,120 @@ __global__ void __launch_bounds__(256) k_b_far(int nk, int nP, int nchunk, int w
     case 8: far_parent<S, 8>(k, P, col, LW_FAR_ARGS); break;
     default: break;
   }
+}
+
+// Single right-hand side translation by parent blocks (the sum of
+// modMLFMA.f90:1837-1853 regrouped as in k_b_far): one lane per sample k; a
+// workgroup takes 64 consecutive samples of one parent P, its wave w the
+// children 2w and 2w+1 of P.  Each row of P's flat source list (both
+// polarisations) is applied to the wave's children with the pair's transfer
+// value Tc[code][k] (pes folded in, zero for near offsets and pads); the
+// waves of a workgroup read the same source rows, which they share through
+// the CU's cache.  Lanes past nk load sample nk-1 and store nothing.
+template <int NCH>
+__device__ __forceinline__ void translate_parent(int nk, int k, int c0, int s0, int s1,
+                                                 const int* __restrict__ bcd, const int2* __restrict__ ps_ent,
+                                                 const double2* __restrict__ Tc, const double2* __restrict__ C,
+                                                 double2* __restrict__ B) {
+  constexpr int kMaxCq = oct_code(7);
+  const int kl = min(k, nk - 1);
+  int coff[NCH];
+#pragma unroll
+  for (int i = 0; i < NCH; ++i) coff[i] = kMaxCq - bcd[c0 + i];
+  double2 a0[NCH], a1[NCH];
+#pragma unroll
+  for (int i = 0; i < NCH; ++i) {
+    a0[i] = make_double2(0.0, 0.0);
+    a1[i] = make_double2(0.0, 0.0);
+  }
+  if (s0 < s1) {
+    // software pipeline over single sources: the source row and the NCH
+    // transfer values of source s+1 are loaded before the FMAs of source s
+    // (two alternating buffers); the list entries come a group of kSrcU
+    // ahead through scalar loads
+    int2 ec[kSrcU], en[kSrcU];
+    double2 va[2], vb[2], ta[NCH], tb[NCH];
+    auto entries = [&](int s, int2(&e)[kSrcU]) {
+#pragma unroll
+      for (int u = 0; u < kSrcU; ++u) e[u] = ps_ent[s + u];
+    };
+    auto load = [&](const int2 e, double2(&v)[2], double2(&t)[NCH]) {
+      const double2* cr = C + (size_t)e.x * 2 * nk + kl;
+      v[0] = cr[0];
+      v[1] = cr[nk];
+#pragma unroll
+      for (int i = 0; i < NCH; ++i) t[i] = Tc[(size_t)(e.y + coff[i]) * nk + kl];
+    };
+    auto apply = [&](const double2(&v)[2], const double2(&t)[NCH]) {
+#pragma unroll
+      for (int i = 0; i < NCH; ++i) {
+        cmac(a0[i], t[i], v[0]);
+        cmac(a1[i], t[i], v[1]);
+      }
+    };
+    static_assert(kSrcU % 2 == 0, "sources alternate between two buffers");
+    entries(s0, ec);
+    load(ec[0], va, ta);
+    for (int g = s0; g < s1; g += kSrcU) {
+      // the next group's entries (the last group again at the end, so that
+      // every load stays in bounds)
+      entries(min(g + kSrcU, s1 - kSrcU), en);
+#pragma unroll
+      for (int u = 0; u < kSrcU; u += 2) {
+        load(ec[u + 1], vb, tb);
+        __builtin_amdgcn_sched_barrier(0);
+        apply(va, ta);
+        load(u + 2 < kSrcU ? ec[u + 2] : en[0], va, ta);
+        __builtin_amdgcn_sched_barrier(0);
+        apply(vb, tb);
+      }
+#pragma unroll
+      for (int u = 0; u < kSrcU; ++u) ec[u] = en[u];
+    }
+  }
+  if (k < nk) {
+#pragma unroll
+    for (int i = 0; i < NCH; ++i) {
+      double2* o = B + (size_t)(c0 + i) * 2 * nk + k;
+      o[0] = a0[i];
+      o[nk] = a1[i];
+    }
+  }
+}
+
+__global__ void __launch_bounds__(256) k_translate_parent(int nk, int nchk, const int* __restrict__ pc_ptr,
+                                                          const int* __restrict__ bcd,
+                                                          const int* __restrict__ ps_ptr,
+                                                          const int2* __restrict__ ps_ent,
+                                                          const double2* __restrict__ Tc,
+                                                          const double2* __restrict__ C, double2* __restrict__ B) {
+  // grid: (parent, chunk of 64 samples), XCD-contiguous
+  const int wb = xcd_block(blockIdx.x, gridDim.x);
+  const int P = wb / nchk;
+  const int k = (wb % nchk) * kWave + threadIdx.x % kWave;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+  const int c0 = pc_ptr[P], nw = min(2, pc_ptr[P + 1] - c0 - 2 * w);
+  const int s0 = ps_ptr[P], s1 = ps_ptr[P + 1];
+  if (nw == 2)
+    translate_parent<2>(nk, k, c0 + 2 * w, s0, s1, bcd, ps_ent, Tc, C, B);
+  else if (nw == 1)
+    translate_parent<1>(nk, k, c0 + 2 * w, s0, s1, bcd, ps_ent, Tc, C, B);
+}
+
+// Tc[code][k] = pes[k] T[tr(code)][k] (zero for codes without a transfer
+// vector): the code-major table of the single-RHS translation
+__global__ void k_tc(int nk, const int* __restrict__ code_tr, const double2* __restrict__ T,
+                     const double* __restrict__ pes, double2* __restrict__ Tc) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= (long long)nk * kOffCodes) return;
+  const int code = (int)(t / nk), k = (int)(t % nk);
+  const int tr = code_tr[code];
+  if (tr < 0) {
+    Tc[t] = make_double2(0.0, 0.0);
+    return;
+  }
+  const double2 v = T[(size_t)tr * nk + k];
+  Tc[t] = make_double2(v.x * pes[k], v.y * pes[k]);
 }
 
 // Tk[k][code] = pes[k] T[tr(code)][k] (zero for codes without a transfer vector)
@@ -1068,6 +1172,8 @@ struct Batch {
   int *ncol_ptr = nullptr, *ncols = nullptr;  // near columns of each leaf box (shared by its rows)
   void* rad = nullptr;                        // leaf patterns in the far-field precision
   std::vector<BatchLevel> lv;
+  // parent ranges of the far kernel per level (1, 2, 4 or 8; see k_b_far)
+  int far_split[16] = {};
   double2 *X = nullptr, *Y = nullptr, *w[12] = {};
   void* Yf = nullptr;
   double* part = nullptr;
@@ -1301,13 +1407,10 @@ static int batch_matvec_t(Batch* bt, const double2* X, double2* Y) {
   for (int n = 3; n <= L; ++n) {
     BatchLevel& d = bt->lv[n];
     const BatchLevel* pl = n > 3 ? &bt->lv[n - 1] : nullptr;
-    // workgroups per (XCD, sample) with the parents split over the XCDs, or
-    // per sample when there are too few parents to split
-    const bool split_p = d.nP >= 64;
-    const int fnchunk = ncol / (kWave * kFarCpl);
-    const int wpx = split_p ? nblk(nblk(d.nP, 8), 4) * fnchunk : nblk(d.nP, 4) * fnchunk;
-    const long long grid = split_p ? 8LL * d.nk * wpx : 8LL * nblk(d.nk, 8) * wpx;
-    hipLaunchKernelGGL(k_b_far<S>, dim3((unsigned)grid), dim3(256), 0, st, d.nk, d.nP, fnchunk, wpx, split_p ? 1 : 0,
+    const int ns = bt->far_split[n], fnchunk = ncol / (kWave * kFarCpl);
+    const int wpx = nblk(nblk(d.nP, ns), 4) * fnchunk;
+    const long long grid = 8LL * nblk(d.nk, 8 / ns) * wpx;
+    hipLaunchKernelGGL(k_b_far<S>, dim3((unsigned)grid), dim3(256), 0, st, d.nk, d.nP, fnchunk, wpx, ns,
                        d.nb, ncol, pl ? pl->nb : 0, d.nwt, d.pc_ptr, d.bcd, d.boct, d.ps_ptr, d.ps_ent,
                        (const T2*)d.Tk, (const T2*)d.C, d.wtp_ptr, d.wtp_row, (const T2*)d.wp,
                        pl ? (const T2*)pl->A : nullptr, (T2*)d.A);
@@ -1548,6 +1651,7 @@ static int setup_far(Batch* bt) {
       return LW_ERR_STATE;
     }
     b.nP = P.lv[n - 1].nb;
+    bt->far_split[n] = b.nP >= 64 ? 8 : 1;
     int* d_code_tr = nullptr;
     if ((rc = upload_b(bt, &b.pc_ptr, P.lv[n - 1].child_ptr)) || (rc = upload_b(bt, &b.bcd, fb.bcd)) ||
         (rc = upload_b(bt, &b.boct, fb.bo)) || (rc = upload_b(bt, &b.ps_ptr, fb.ps_ptr)) ||
@@ -1597,6 +1701,43 @@ static int setup_far(Batch* bt) {
     b.A = a;
   }
   return LW_OK;
+}
+
+int setup_far_parent(Handle* h) {
+  const Plan& P = h->plan;
+  const int L = h->L;
+  std::vector<FarBlocks> fbs(L + 1);
+  for (int n = 3; n <= L; ++n) {
+    std::string msg;
+    if (!far_blocks(P, n, fbs[n], msg)) {
+      set_error("lw_mlfma_init: " + msg);
+      return LW_ERR_STATE;
+    }
+  }
+  int rc;
+  for (int n = 3; n <= L; ++n) {
+    DevLevel& d = h->lv[n];
+    const FarBlocks& fb = fbs[n];
+    d.nps = (long long)fb.ps_ent.size();
+    int* d_code_tr = nullptr;
+    if ((rc = upload(h, &d.pc_ptr, P.lv[n - 1].child_ptr)) || (rc = upload(h, &d.bcd, fb.bcd)) ||
+        (rc = upload(h, &d.ps_ptr, fb.ps_ptr)) ||
+        (rc = upload(h, &d.ps_ent, fb.ps_ent)) || (rc = upload(h, &d_code_tr, fb.code_tr)) ||
+        (rc = dmalloc(h, &d.Tc, (size_t)d.nk * kOffCodes)))
+      return rc;
+    hipLaunchKernelGGL(k_tc, dim3(nblk((long long)d.nk * kOffCodes, 256)), dim3(256), 0, h->stream, d.nk, d_code_tr,
+                       d.T, d.pes, d.Tc);
+    LW_HIP(hipGetLastError());
+  }
+  LW_HIP(hipStreamSynchronize(h->stream));
+  return LW_OK;
+}
+
+void launch_translate_parent(Handle* h, int n) {
+  DevLevel& d = h->lv[n];
+  const int nP = h->plan.lv[n - 1].nb, nchk = nblk(d.nk, kWave);
+  hipLaunchKernelGGL(k_translate_parent, dim3(nP * nchk), dim3(256), 0, h->stream, d.nk, nchk,
+                     d.pc_ptr, d.bcd, d.ps_ptr, d.ps_ent, d.Tc, d.C, d.B);
 }
 
 }  // namespace lw

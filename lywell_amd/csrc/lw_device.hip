@@ -146,29 +146,6 @@ __global__ void __launch_bounds__(256) k_interp_up(int nkc, int nkf, const int* 
   }
 }
 
-// Translation (modMLFMA.f90:1837-1853): campoB = pesos * sum_far campoC(far) *
-// T(transfer index).  Grid: (box, chunk of 256 samples).
-__global__ void __launch_bounds__(256) k_translate(int nk, const int* __restrict__ far_ptr,
-                                                   const int* __restrict__ far_box, const int* __restrict__ far_tr,
-                                                   const double2* __restrict__ T, const double* __restrict__ pes,
-                                                   const double2* __restrict__ C, double2* __restrict__ B) {
-  const int nch = (nk + blockDim.x - 1) / blockDim.x;
-  const int wb = xcd_block(blockIdx.x, gridDim.x);
-  const int b = wb / nch;
-  const int k = (wb % nch) * blockDim.x + threadIdx.x;
-  if (k >= nk) return;
-  double2 a0 = make_double2(0.0, 0.0), a1 = make_double2(0.0, 0.0);
-  for (int m = far_ptr[b]; m < far_ptr[b + 1]; ++m) {
-    const int fb = far_box[m];
-    const double2 t = T[(size_t)far_tr[m] * nk + k];
-    a0 = cadd(a0, cmul(C[((size_t)fb * 2 + 0) * nk + k], t));
-    a1 = cadd(a1, cmul(C[((size_t)fb * 2 + 1) * nk + k], t));
-  }
-  const double p = pes[k];
-  B[((size_t)b * 2 + 0) * nk + k] = make_double2(a0.x * p, a0.y * p);
-  B[((size_t)b * 2 + 1) * nk + k] = make_double2(a1.x * p, a1.y * p);
-}
-
 // Downward pass (modMLFMA.f90:1857-1897): shift the parent's field to the
 // child centre, anterpolate (the transpose of Interpolar, :1370-1390, in
 // gather form) and add the child's own translated field.  One workgroup per
@@ -583,12 +560,7 @@ int run_matvec(Handle* h, const double2* x, double2* y, double2* y_near, hipEven
                        p.ph, p.wint, p.iint, c.C, p.C);
   }
   if (ev) LW_HIP(hipEventRecord(ev[2], st));
-  for (int n = 3; n <= L; ++n) {
-    DevLevel& d = h->lv[n];
-    dim3 grid(d.nb * blocks_for(d.nk, 256));
-    hipLaunchKernelGGL(k_translate, grid, dim3(256), 0, st, d.nk, d.far_ptr, d.far_box, d.far_tr, d.T, d.pes,
-                       d.C, d.B);
-  }
+  for (int n = 3; n <= L; ++n) launch_translate_parent(h, n);
   if (ev) LW_HIP(hipEventRecord(ev[3], st));
   for (int n = 4; n <= L; ++n) {
     DevLevel& c = h->lv[n];
@@ -698,9 +670,6 @@ static int init_device_state(Handle* h, const Geometry& g) {
     std::vector<int> par(lp.parent.begin(), lp.parent.end());
     TRY(upload(h, &d.parent, par));
     if (n < L) TRY(upload(h, &d.child_ptr, lp.child_ptr));
-    TRY(upload(h, &d.far_ptr, lp.far_ptr));
-    TRY(upload(h, &d.far_box, lp.far_box));
-    TRY(upload(h, &d.far_tr, lp.far_tr));
     TRY(dmalloc(h, &d.T, (size_t)lp.nk * std::max(lp.nT, 1)));
     const size_t fsz = (size_t)lp.nk * 2 * lp.nb;
     TRY(dmalloc(h, &d.C, fsz));
@@ -739,6 +708,8 @@ static int init_device_state(Handle* h, const Geometry& g) {
     TRY(upload(h, &d_tL, lp.trans_L));
     TRY(launch_setup_transfer(h, n, d_binom2, d_tu, d_tL, d_th, lp.hstride));
   }
+  // the translation's parent blocks
+  TRY(setup_far_parent(h));
   // level 3's upward output and level 2 have no parents: nothing else to upload
   // leaf ordering and patterns
   TRY(upload(h, &h->d_slot_basis, P.slot_basis));
@@ -1392,8 +1363,11 @@ int lw_matvec_stage_bytes(lw_handle* hh, double* sb) {
   sb[1] = 0.0;
   for (int n = L; n >= 4; --n) sb[1] += field(n) + field(n - 1) + 12.0 * 9 * P.lv[n - 1].nk + 16.0 * 8 * P.lv[n - 1].nk;
   sb[2] = 0.0;
+  // parent blocks: C read and B written once, the (code, k) transfer table,
+  // the flat source lists and the per-box and per-parent index arrays
   for (int n = 3; n <= L; ++n)
-    sb[2] += 2 * field(n) + 16.0 * P.lv[n].nk * P.lv[n].nT + 8.0 * (double)P.lv[n].far_box.size() + 16.0 * P.lv[n].nk;
+    sb[2] += 2 * field(n) + 16.0 * P.lv[n].nk * 343 + 8.0 * (double)h->lv[n].nps + 4.0 * P.lv[n].nb +
+             8.0 * (P.lv[n - 1].nb + 1);
   sb[3] = 0.0;
   for (int n = 4; n <= L; ++n) sb[3] += field(n - 1) + 2 * field(n) + 12.0 * 9 * P.lv[n - 1].nk;
   sb[4] = pattern + field(L) + (zb + 4.0) * (double)P.nnz + 4.0 * (P.ne + 1) + 2.0 * 16.0 * P.ne + 8.0 * P.ne;

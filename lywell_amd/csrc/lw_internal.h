@@ -77,9 +77,6 @@ struct DevLevel {
   double* centre = nullptr;   // (3, nb)
   int* parent = nullptr;      // (nb) into level-1
   int* child_ptr = nullptr;   // (nb+1) into level+1 (non-leaf)
-  int* far_ptr = nullptr;     // (nb+1)
-  int* far_box = nullptr;     // (npairs)
-  int* far_tr = nullptr;      // (npairs)
   double2* T = nullptr;       // (nk, nT), transfer function of vector t at T[t*nk + k]
   double2* C = nullptr;       // campoC (nk, 2, nb): C[(b*2+p)*nk + k]
   double2* B = nullptr;       // campoB
@@ -96,6 +93,17 @@ struct DevLevel {
   double2* ph = nullptr;      // (8, nk)
   int* oct = nullptr;         // octant of each box within its parent (n > 3)
   long long npairs = 0;
+  // translation by parent blocks: per parent at level n-1 a flat source
+  // list padded to a multiple of 4 entries (box, offset-code part of the
+  // pair), the offset-code part of each box's octant, and
+  // Tc[code*nk + k] = pes[k] T[tr(code)][k] per integer offset code (zero for
+  // offsets without a transfer vector)
+  int* pc_ptr = nullptr;  // (nb of level n-1, + 1) children of each parent
+  int* bcd = nullptr;
+  int* ps_ptr = nullptr;
+  int2* ps_ent = nullptr;
+  long long nps = 0;  // entries of ps_ent
+  double2* Tc = nullptr;
 };
 
 struct Handle {
@@ -180,6 +188,13 @@ int upload_structure(Handle* h);
 // ILU(0) of the near field on the device (lw_ilu0.hip); *piv = 0 or the first
 // row with a zero pivot.
 int ilu0_device(Handle* h, int* piv);
+
+// Parent-block tables of the single-RHS translation (lw_batch.hip). The far
+// lists of every level must regroup into parent-neighbour blocks (checked
+// against the plan's per-box lists); LW_ERR_STATE otherwise.
+int setup_far_parent(Handle* h);
+// campoB of level n by parent blocks on the handle's stream (lw_batch.hip).
+void launch_translate_parent(Handle* h, int n);
 
 // One single-RHS matvec (MatxVec_MLFMA) on device vectors, on the handle's
 // stream (lw_device.hip).
