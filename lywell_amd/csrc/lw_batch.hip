@@ -485,79 +485,42 @@ __global__ void __launch_bounds__(256) k_b_far(int nk, int nP, int nchunk, int w
 // Single right-hand side translation by parent blocks (the sum of
 // modMLFMA.f90:1837-1853 regrouped as in k_b_far): one lane per sample k; a
 // workgroup takes 64 consecutive samples of one parent P, its wave w the
-// children 2w and 2w+1 of P.  Each row of P's flat source list (both
-// polarisations) is applied to the wave's children with the pair's transfer
-// value Tc[code][k] (pes folded in, zero for near offsets and pads); the
-// waves of a workgroup read the same source rows, which they share through
-// the CU's cache.  Lanes past nk load sample nk-1 and store nothing.
-template <int NCH>
-__device__ __forceinline__ void translate_parent(int nk, int k, int c0, int s0, int s1,
-                                                 const int* __restrict__ bcd, const int2* __restrict__ ps_ent,
-                                                 const double2* __restrict__ Tc, const double2* __restrict__ C,
-                                                 double2* __restrict__ B) {
-  constexpr int kMaxCq = oct_code(7);
-  const int kl = min(k, nk - 1);
-  int coff[NCH];
+// children 2w and 2w+1 of P.  P's flat source list is walked in groups of
+// kSrcU = 4 rows: wave u fetches row u of the next group (both
+// polarisations) into registers while the current group is applied from LDS,
+// then stores it to the other LDS buffer before the group barrier, so every
+// source row is read from memory once per workgroup.  Each row is applied to
+// the wave's children with the pair's transfer value Tc[code][k] (pes folded
+// in, zero for near offsets and pads), loaded one source ahead.  Lanes past
+// nk load sample nk-1 and store nothing.
+constexpr int kTrWaves = 4;
+static_assert(kTrWaves == kSrcU, "one source row per wave and group");
+
+template <int NW>
+__device__ __forceinline__ void translate_group(const double2 (*sv)[2][kWave], int lane, int nk, int kl,
+                                                const int2 (&e)[kSrcU], const int (&coff)[2],
+                                                const double2* __restrict__ Tc, double2 (&a0)[2],
+                                                double2 (&a1)[2]) {
+  double2 ta[NW], tb[NW];
+  auto tload = [&](int u, double2(&t)[NW]) {
 #pragma unroll
-  for (int i = 0; i < NCH; ++i) coff[i] = kMaxCq - bcd[c0 + i];
-  double2 a0[NCH], a1[NCH];
+    for (int i = 0; i < NW; ++i) t[i] = Tc[(size_t)(e[u].y + coff[i]) * nk + kl];
+  };
+  auto apply = [&](int u, const double2(&t)[NW]) {
+    const double2 v0 = sv[u][0][lane], v1 = sv[u][1][lane];
 #pragma unroll
-  for (int i = 0; i < NCH; ++i) {
-    a0[i] = make_double2(0.0, 0.0);
-    a1[i] = make_double2(0.0, 0.0);
-  }
-  if (s0 < s1) {
-    // software pipeline over single sources: the source row and the NCH
-    // transfer values of source s+1 are loaded before the FMAs of source s
-    // (two alternating buffers); the list entries come a group of kSrcU
-    // ahead through scalar loads
-    int2 ec[kSrcU], en[kSrcU];
-    double2 va[2], vb[2], ta[NCH], tb[NCH];
-    auto entries = [&](int s, int2(&e)[kSrcU]) {
-#pragma unroll
-      for (int u = 0; u < kSrcU; ++u) e[u] = ps_ent[s + u];
-    };
-    auto load = [&](const int2 e, double2(&v)[2], double2(&t)[NCH]) {
-      const double2* cr = C + (size_t)e.x * 2 * nk + kl;
-      v[0] = cr[0];
-      v[1] = cr[nk];
-#pragma unroll
-      for (int i = 0; i < NCH; ++i) t[i] = Tc[(size_t)(e.y + coff[i]) * nk + kl];
-    };
-    auto apply = [&](const double2(&v)[2], const double2(&t)[NCH]) {
-#pragma unroll
-      for (int i = 0; i < NCH; ++i) {
-        cmac(a0[i], t[i], v[0]);
-        cmac(a1[i], t[i], v[1]);
-      }
-    };
-    static_assert(kSrcU % 2 == 0, "sources alternate between two buffers");
-    entries(s0, ec);
-    load(ec[0], va, ta);
-    for (int g = s0; g < s1; g += kSrcU) {
-      // the next group's entries (the last group again at the end, so that
-      // every load stays in bounds)
-      entries(min(g + kSrcU, s1 - kSrcU), en);
-#pragma unroll
-      for (int u = 0; u < kSrcU; u += 2) {
-        load(ec[u + 1], vb, tb);
-        __builtin_amdgcn_sched_barrier(0);
-        apply(va, ta);
-        load(u + 2 < kSrcU ? ec[u + 2] : en[0], va, ta);
-        __builtin_amdgcn_sched_barrier(0);
-        apply(vb, tb);
-      }
-#pragma unroll
-      for (int u = 0; u < kSrcU; ++u) ec[u] = en[u];
+    for (int i = 0; i < NW; ++i) {
+      cmac(a0[i], t[i], v0);
+      cmac(a1[i], t[i], v1);
     }
-  }
-  if (k < nk) {
+  };
+  tload(0, ta);
 #pragma unroll
-    for (int i = 0; i < NCH; ++i) {
-      double2* o = B + (size_t)(c0 + i) * 2 * nk + k;
-      o[0] = a0[i];
-      o[nk] = a1[i];
-    }
+  for (int u = 0; u < kSrcU; u += 2) {
+    tload(u + 1, tb);
+    apply(u, ta);
+    if (u + 2 < kSrcU) tload(u + 2, ta);
+    apply(u + 1, tb);
   }
 }
 
@@ -567,17 +530,62 @@ __global__ void __launch_bounds__(256) k_translate_parent(int nk, int nchk, cons
                                                           const int2* __restrict__ ps_ent,
                                                           const double2* __restrict__ Tc,
                                                           const double2* __restrict__ C, double2* __restrict__ B) {
+  // source rows of two groups: [buffer][row][polarisation][lane]
+  __shared__ double2 sv[2][kSrcU][2][kWave];
+  constexpr int kMaxCq = oct_code(7);
   // grid: (parent, chunk of 64 samples), XCD-contiguous
   const int wb = xcd_block(blockIdx.x, gridDim.x);
   const int P = wb / nchk;
-  const int k = (wb % nchk) * kWave + threadIdx.x % kWave;
+  const int lane = threadIdx.x % kWave;
+  const int k = (wb % nchk) * kWave + lane, kl = min(k, nk - 1);
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
-  const int c0 = pc_ptr[P], nw = min(2, pc_ptr[P + 1] - c0 - 2 * w);
+  const int c0 = pc_ptr[P] + 2 * w, nw = max(0, min(2, pc_ptr[P + 1] - c0));
   const int s0 = ps_ptr[P], s1 = ps_ptr[P + 1];
-  if (nw == 2)
-    translate_parent<2>(nk, k, c0 + 2 * w, s0, s1, bcd, ps_ent, Tc, C, B);
-  else if (nw == 1)
-    translate_parent<1>(nk, k, c0 + 2 * w, s0, s1, bcd, ps_ent, Tc, C, B);
+  int coff[2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i) coff[i] = kMaxCq - bcd[min(c0 + i, pc_ptr[P + 1] - 1)];
+  double2 a0[2], a1[2];
+#pragma unroll
+  for (int i = 0; i < 2; ++i) {
+    a0[i] = make_double2(0.0, 0.0);
+    a1[i] = make_double2(0.0, 0.0);
+  }
+  if (s0 < s1) {
+    // row w of the first group
+    {
+      const double2* cr = C + (size_t)ps_ent[s0 + w].x * 2 * nk + kl;
+      sv[0][w][0][lane] = cr[0];
+      sv[0][w][1][lane] = cr[nk];
+    }
+    __syncthreads();
+    int cur = 0;
+    for (int g = s0; g < s1; g += kSrcU) {
+      int2 e[kSrcU];
+#pragma unroll
+      for (int u = 0; u < kSrcU; ++u) e[u] = ps_ent[g + u];
+      // row w of the next group (the last group again at the end, so that
+      // the load stays in bounds)
+      const double2* cr = C + (size_t)ps_ent[min(g + kSrcU, s1 - kSrcU) + w].x * 2 * nk + kl;
+      const double2 n0 = cr[0], n1 = cr[nk];
+      if (nw == 2)
+        translate_group<2>(sv[cur], lane, nk, kl, e, coff, Tc, a0, a1);
+      else if (nw == 1)
+        translate_group<1>(sv[cur], lane, nk, kl, e, coff, Tc, a0, a1);
+      sv[cur ^ 1][w][0][lane] = n0;
+      sv[cur ^ 1][w][1][lane] = n1;
+      __syncthreads();
+      cur ^= 1;
+    }
+  }
+  if (k < nk) {
+#pragma unroll
+    for (int i = 0; i < 2; ++i)
+      if (i < nw) {
+        double2* o = B + (size_t)(c0 + i) * 2 * nk + k;
+        o[0] = a0[i];
+        o[nk] = a1[i];
+      }
+  }
 }
 
 // Tc[code][k] = pes[k] T[tr(code)][k] (zero for codes without a transfer
