@@ -82,19 +82,36 @@ __global__ void __launch_bounds__(256) k_aggregate(int nk2, const int* __restric
                                                    const double2* __restrict__ x, double2* __restrict__ C) {
   const int b = xcd_block(blockIdx.x, gridDim.x);
   const int s0 = box_slot[b], s1 = box_slot[b + 1];
-  // two samples per thread per pass: independent accumulators keep more
-  // pattern loads in flight
+  // two samples per thread per pass (the second one clamped, so that the
+  // loop has no branch), and the bases in groups of kAggU whose pattern
+  // loads are all issued before their FMAs; each sample accumulates its
+  // bases in box order
+  constexpr int kAggU = 8;
   for (int e = threadIdx.x; e < nk2; e += 2 * blockDim.x) {
-    const int e2 = e + blockDim.x;
-    const bool two = e2 < nk2;
+    const int e2 = e + blockDim.x, e2l = min(e2, nk2 - 1);
     double2 acc = make_double2(0.0, 0.0), acc2 = make_double2(0.0, 0.0);
-    for (int s = s0; s < s1; ++s) {
+    int s = s0;
+    for (; s + kAggU <= s1; s += kAggU) {
+      double2 xv[kAggU], r1[kAggU], r2[kAggU];
+#pragma unroll
+      for (int u = 0; u < kAggU; ++u) {
+        xv[u] = x[slot_basis[s + u]];
+        r1[u] = rad[(size_t)(s + u) * nk2 + e];
+        r2[u] = rad[(size_t)(s + u) * nk2 + e2l];
+      }
+#pragma unroll
+      for (int u = 0; u < kAggU; ++u) {
+        acc = cadd(acc, cmul(r1[u], xv[u]));
+        acc2 = cadd(acc2, cmul(r2[u], xv[u]));
+      }
+    }
+    for (; s < s1; ++s) {
       const double2 xv = x[slot_basis[s]];
       acc = cadd(acc, cmul(rad[(size_t)s * nk2 + e], xv));
-      if (two) acc2 = cadd(acc2, cmul(rad[(size_t)s * nk2 + e2], xv));
+      acc2 = cadd(acc2, cmul(rad[(size_t)s * nk2 + e2l], xv));
     }
     C[(size_t)b * nk2 + e] = acc;
-    if (two) C[(size_t)b * nk2 + e2] = acc2;
+    if (e2 < nk2) C[(size_t)b * nk2 + e2] = acc2;
   }
 }
 
@@ -165,13 +182,15 @@ __global__ void __launch_bounds__(256) k_interp_down(int nkc, int nkf, const int
   const double2* src = Ap + (size_t)P * 2 * nkc;
   const double2* e = ph + (size_t)oct[c] * nkc;  // exp(+j k khat . r); conjugated below
   double2 a0 = make_double2(0.0, 0.0), a1 = make_double2(0.0, 0.0);
-  const int q0 = wt_ptr[s], q1 = wt_ptr[s + 1];
-  int q = q0;
+  // this sample's entries, a wave's block stored entry-major
+  const int q0 = wt_ptr[s / kWave], cnt = (wt_ptr[s / kWave + 1] - q0) / kWave;
+  const WtEntry* tq = wt + q0 + s % kWave;
+  int q = 0;
   // four stencil entries per step: their loads are issued together
-  for (; q + 4 <= q1; q += 4) {
+  for (; q + 4 <= cnt; q += 4) {
     WtEntry t[4];
 #pragma unroll
-    for (int u = 0; u < 4; ++u) t[u] = wt[q + u];
+    for (int u = 0; u < 4; ++u) t[u] = tq[(q + u) * kWave];
 #pragma unroll
     for (int u = 0; u < 4; ++u) {
       const int r = t[u].r;
@@ -183,8 +202,8 @@ __global__ void __launch_bounds__(256) k_interp_down(int nkc, int nkf, const int
       a1.y += t[u].w * v1.y;
     }
   }
-  for (; q < q1; ++q) {
-    const WtEntry t = wt[q];
+  for (; q < cnt; ++q) {
+    const WtEntry t = tq[q * kWave];
     const double2 pr = e[t.r];
     const double2 v0 = cmulc(pr, src[t.r]), v1 = cmulc(pr, src[nkc + t.r]);
     a0.x += t.w * v0.x;
@@ -219,12 +238,13 @@ __global__ void __launch_bounds__(512) k_interp_down_lds(int nkc, int nkf, const
   __syncthreads();
   for (int s = threadIdx.x; s < nkf; s += blockDim.x) {
     double2 a0 = make_double2(0.0, 0.0), a1 = make_double2(0.0, 0.0);
-    const int q0 = wt_ptr[s], q1 = wt_ptr[s + 1];
-    int q = q0;
-    for (; q + 4 <= q1; q += 4) {
+    const int q0 = wt_ptr[s / kWave], cnt = (wt_ptr[s / kWave + 1] - q0) / kWave;
+    const WtEntry* tq = wt + q0 + s % kWave;
+    int q = 0;
+    for (; q + 4 <= cnt; q += 4) {
       WtEntry t[4];
 #pragma unroll
-      for (int u = 0; u < 4; ++u) t[u] = wt[q + u];
+      for (int u = 0; u < 4; ++u) t[u] = tq[(q + u) * kWave];
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
         const double2 v0 = lds[t[u].r], v1 = lds[nkc + t[u].r];
@@ -234,8 +254,8 @@ __global__ void __launch_bounds__(512) k_interp_down_lds(int nkc, int nkf, const
         a1.y += t[u].w * v1.y;
       }
     }
-    for (; q < q1; ++q) {
-      const WtEntry t = wt[q];
+    for (; q < cnt; ++q) {
+      const WtEntry t = tq[q * kWave];
       a0.x += t.w * lds[t.r].x;
       a0.y += t.w * lds[t.r].y;
       a1.x += t.w * lds[nkc + t.r].x;
@@ -678,12 +698,24 @@ static int init_device_state(Handle* h, const Geometry& g) {
     if (n < L) {
       TRY(upload(h, &d.wint, lp.wint));
       TRY(upload(h, &d.iint, lp.iint));
-      TRY(upload(h, &d.wt_ptr, lp.wt_ptr));
-      TRY(upload(h, &d.wt_row, lp.wt_row));
-      TRY(upload(h, &d.wt_w, lp.wt_w));
       {
-        std::vector<WtEntry> wt(lp.wt_w.size());
-        for (size_t q = 0; q < wt.size(); ++q) wt[q] = WtEntry{lp.wt_w[q], lp.wt_row[q], 0};
+        // the transposed stencil in blocks of kWave fine samples, entry-major
+        // within a block (entry u of sample s at wt_ptr[s / kWave] +
+        // u * kWave + s % kWave), each block padded with zero weights (row
+        // 0) to its longest run, so that a wave's entry loads are contiguous
+        const int nkf = (int)lp.wt_ptr.size() - 1, nbk = (nkf + kWave - 1) / kWave;
+        std::vector<int> off(nbk + 1, 0);
+        for (int bk = 0; bk < nbk; ++bk) {
+          int m = 0;
+          for (int q = bk * kWave; q < std::min(nkf, (bk + 1) * kWave); ++q)
+            m = std::max(m, lp.wt_ptr[q + 1] - lp.wt_ptr[q]);
+          off[bk + 1] = off[bk] + m * kWave;
+        }
+        std::vector<WtEntry> wt(off[nbk], WtEntry{0.0, 0, 0});
+        for (int q = 0; q < nkf; ++q)
+          for (int e = lp.wt_ptr[q]; e < lp.wt_ptr[q + 1]; ++e)
+            wt[off[q / kWave] + (e - lp.wt_ptr[q]) * kWave + q % kWave] = WtEntry{lp.wt_w[e], lp.wt_row[e], 0};
+        TRY(upload(h, &d.wt_ptr, off));
         TRY(upload(h, &d.wt, wt));
       }
       TRY(dmalloc(h, &d.ph, 8 * (size_t)lp.nk));

@@ -84,10 +84,11 @@ struct DevLevel {
   // interpolation from level+1 to this level (3 <= n < L)
   double* wint = nullptr;     // (9, nk)
   int* iint = nullptr;        // (9, nk) fine sample, 0-based
-  int* wt_ptr = nullptr;      // (nk_fine + 1)
-  int* wt_row = nullptr;      // coarse sample of each transposed entry
-  double* wt_w = nullptr;     // weight of each transposed entry
-  WtEntry* wt = nullptr;      // (weight, coarse row) pairs of the transpose
+  // (weight, coarse row) pairs of the transpose for each fine sample, in
+  // blocks of 64 samples stored entry-major (entry u of sample s at
+  // wt[wt_ptr[s / 64] + 64 u + s % 64]), each block padded with zero weights
+  int* wt_ptr = nullptr;      // (ceil(nk_fine / 64) + 1) block offsets
+  WtEntry* wt = nullptr;
   // exp(+j k khat_i . r_o) for the 8 child offsets r_o = (+-q,+-q,+-q),
   // q = lado/4, octant bits (x,y,z) = (4,2,1), set = positive side (n < L)
   double2* ph = nullptr;      // (8, nk)
