@@ -490,9 +490,13 @@ __global__ void __launch_bounds__(256) k_b_far(int nk, int nP, int nchunk, int w
 // polarisations) into registers while the current group is applied from LDS,
 // then stores it to the other LDS buffer before the group barrier, so every
 // source row is read from memory once per workgroup.  Each row is applied to
-// the wave's children with the pair's transfer value Tc[code][k] (pes folded
-// in, zero for near offsets and pads), loaded one source ahead.  Lanes past
-// nk load sample nk-1 and store nothing.
+// the wave's children with the pair's transfer value Tc[code][k] (zero for
+// near offsets and pads), loaded one source ahead.  Each child's far boxes
+// come in ascending box order, the order of the plan's per-box far lists,
+// and each term is added as the reference adds it (campoB += campoC * T, then
+// campoB * pesos), so the sums are those of the per-box form: the near and
+// pad entries add exact zeros.  Lanes past nk load sample nk-1 and store
+// nothing.
 constexpr int kTrWaves = 4;
 static_assert(kTrWaves == kSrcU, "one source row per wave and group");
 
@@ -510,8 +514,8 @@ __device__ __forceinline__ void translate_group(const double2 (*sv)[2][kWave], i
     const double2 v0 = sv[u][0][lane], v1 = sv[u][1][lane];
 #pragma unroll
     for (int i = 0; i < NW; ++i) {
-      cmac(a0[i], t[i], v0);
-      cmac(a1[i], t[i], v1);
+      a0[i] = dadd(a0[i], dmul(v0, t[i]));
+      a1[i] = dadd(a1[i], dmul(v1, t[i]));
     }
   };
   tload(0, ta);
@@ -529,6 +533,7 @@ __global__ void __launch_bounds__(256) k_translate_parent(int nk, int nchk, cons
                                                           const int* __restrict__ ps_ptr,
                                                           const int2* __restrict__ ps_ent,
                                                           const double2* __restrict__ Tc,
+                                                          const double* __restrict__ pes,
                                                           const double2* __restrict__ C, double2* __restrict__ B) {
   // source rows of two groups: [buffer][row][polarisation][lane]
   __shared__ double2 sv[2][kSrcU][2][kWave];
@@ -578,20 +583,21 @@ __global__ void __launch_bounds__(256) k_translate_parent(int nk, int nchk, cons
     }
   }
   if (k < nk) {
+    const double p = pes[k];
 #pragma unroll
     for (int i = 0; i < 2; ++i)
       if (i < nw) {
         double2* o = B + (size_t)(c0 + i) * 2 * nk + k;
-        o[0] = a0[i];
-        o[nk] = a1[i];
+        o[0] = make_double2(a0[i].x * p, a0[i].y * p);
+        o[nk] = make_double2(a1[i].x * p, a1[i].y * p);
       }
   }
 }
 
-// Tc[code][k] = pes[k] T[tr(code)][k] (zero for codes without a transfer
-// vector): the code-major table of the single-RHS translation
+// Tc[code][k] = T[tr(code)][k] (zero for codes without a transfer vector):
+// the code-major table of the single-RHS translation
 __global__ void k_tc(int nk, const int* __restrict__ code_tr, const double2* __restrict__ T,
-                     const double* __restrict__ pes, double2* __restrict__ Tc) {
+                     double2* __restrict__ Tc) {
   const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= (long long)nk * kOffCodes) return;
   const int code = (int)(t / nk), k = (int)(t % nk);
@@ -600,8 +606,7 @@ __global__ void k_tc(int nk, const int* __restrict__ code_tr, const double2* __r
     Tc[t] = make_double2(0.0, 0.0);
     return;
   }
-  const double2 v = T[(size_t)tr * nk + k];
-  Tc[t] = make_double2(v.x * pes[k], v.y * pes[k]);
+  Tc[t] = T[(size_t)tr * nk + k];
 }
 
 // Tk[k][code] = pes[k] T[tr(code)][k] (zero for codes without a transfer vector)
@@ -1734,7 +1739,7 @@ int setup_far_parent(Handle* h) {
         (rc = dmalloc(h, &d.Tc, (size_t)d.nk * kOffCodes)))
       return rc;
     hipLaunchKernelGGL(k_tc, dim3(nblk((long long)d.nk * kOffCodes, 256)), dim3(256), 0, h->stream, d.nk, d_code_tr,
-                       d.T, d.pes, d.Tc);
+                       d.T, d.Tc);
     LW_HIP(hipGetLastError());
   }
   LW_HIP(hipStreamSynchronize(h->stream));
@@ -1745,7 +1750,7 @@ void launch_translate_parent(Handle* h, int n) {
   DevLevel& d = h->lv[n];
   const int nP = h->plan.lv[n - 1].nb, nchk = nblk(d.nk, kWave);
   hipLaunchKernelGGL(k_translate_parent, dim3(nP * nchk), dim3(256), 0, h->stream, d.nk, nchk,
-                     d.pc_ptr, d.bcd, d.ps_ptr, d.ps_ent, d.Tc, d.C, d.B);
+                     d.pc_ptr, d.bcd, d.ps_ptr, d.ps_ent, d.Tc, d.pes, d.C, d.B);
 }
 
 }  // namespace lw

@@ -97,7 +97,7 @@ struct DevLevel {
   // translation by parent blocks: per parent at level n-1 a flat source
   // list padded to a multiple of 4 entries (box, offset-code part of the
   // pair), the offset-code part of each box's octant, and
-  // Tc[code*nk + k] = pes[k] T[tr(code)][k] per integer offset code (zero for
+  // Tc[code*nk + k] = T[tr(code)][k] per integer offset code (zero for
   // offsets without a transfer vector)
   int* pc_ptr = nullptr;  // (nb of level n-1, + 1) children of each parent
   int* bcd = nullptr;
