@@ -78,7 +78,10 @@ int lw_tri_geometry(int np, const double *p_coord, int nt, const int *t_p, doubl
 /* inicializar_MLFMA (modMLFMA.f90:113-275).  Arguments mirror the
  * reference's dummy arguments plus lambda and conf%criterio_ladomax, which
  * the reference reads from module globals (modGlobalParam.f90:20-43).
- * Builds the octree and operators and uploads them to the current device. */
+ * Builds the octree and operators and uploads them to the current device.
+ * LW_ERR_STATE if a level's far lists do not regroup into parent-neighbour
+ * blocks (the translation runs per parent block; the octree built here
+ * always regroups, the check guards the plan). */
 int lw_mlfma_init(double lambda, double ladomax, int flags, int num_e, const double *e_centro,
                   const int *e_t, const int *e_po, const double *e_long, int num_t,
                   const double *t_area, const double *t_baric, const double *t_baric_sub,

@@ -312,8 +312,9 @@ __global__ void __launch_bounds__(256) k_b_interp_up(int ncol, int nthc, int nkc
 // Tk row and enter the FMAs as scalar operands; the children of P are
 // compacted (nch = 1..8 accumulators, one code path per count), so no FMA is
 // spent on absent octants and the inner loop has no branches.  Source rows
-// are loaded one group of kSrcU ahead.  Workgroups that share a sample run on
-// the same XCD (workgroup id mod 8), where the k-slice of C_n stays in L2.
+// are loaded one group of kSrcU ahead.  The parents are cut into 1, 2, 4 or 8
+// contiguous ranges per level, each range on the XCDs (workgroup id mod 8)
+// that take its samples (see k_b_far).
 constexpr int kOffCodes = 343;  // (dx, dy, dz) in [-3, 3]^3
 constexpr int kSrcU = 4;        // sources per pipelined group
 constexpr int kFarCpl = 2;      // columns per lane
