@@ -53,7 +53,7 @@ def main():
     ap.add_argument("--errmax", type=float, default=1e-4)
     ap.add_argument("--maxit", type=int, default=20000)
     ap.add_argument("--prec", choices=["none", "ilu0"], default="none",
-                    help="preconditioner of the timed solves (ILU0 diverges on this mesh; see DESIGN.md)")
+                    help="preconditioner of the timed solves (ILU0-preconditioned CGS diverges on this mesh)")
     ap.add_argument("--ilu0-probe", type=int, default=20,
                     help="ILU0-CGS iterations run beside the timed solves to report the ILU0 path (0: skip)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU-baseline leg")
@@ -87,7 +87,7 @@ def main():
     args = ap.parse_args()
 
     if args.gpus != 1:
-        print(json.dumps({"error": "this build runs on one GPU; multi-GPU runs are not part of it",
+        print(json.dumps({"error": "bench.py runs on one GPU; multi-GPU runs are not supported",
                           "n_gpus": args.gpus}))
         sys.exit(2)
 
