@@ -1138,6 +1138,20 @@ void or_box_centres(const or_mlfma *h, int level, double *c, int *parent) {
   }
 }
 
+long long or_far_lists(const or_mlfma *h, int level, int *ptr, int *boxes) {
+  const Level *lv = &h->lv[level];
+  long long n = 0;
+  if (ptr) ptr[0] = 0;
+  for (int b = 0; b < lv->nb; ++b) {
+    for (int m = 0; m < lv->b[b].nfar; ++m) {
+      if (boxes) boxes[n] = lv->b[b].far_box[m] - 1;
+      ++n;
+    }
+    if (ptr) ptr[b + 1] = (int)n;
+  }
+  return n;
+}
+
 /* ---------------------------------------------------------------- ILU0 */
 
 int or_ilu0(int n, const int *ia, const int *ja, const double *a, int fp64,

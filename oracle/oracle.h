@@ -58,6 +58,10 @@ long long or_level_table(const or_mlfma *h, int level, int which, double *out);
 /* Leaf box of each basis (1-based), box centres and parents per level. */
 void or_leaf_of_basis(const or_mlfma *h, int *leaf);
 void or_box_centres(const or_mlfma *h, int level, double *c, int *parent);
+/* Far list of every box at a level, in list order (cubLej_indFuncTransf(1,:),
+ * modMLFMA.f90:1840-1841): ptr (nboxes + 1) and 0-based far boxes. Returns
+ * the total count; with ptr == NULL only the count. */
+long long or_far_lists(const or_mlfma *h, int level, int *ptr, int *boxes);
 
 /* MatxVec_MLFMA (modMLFMA.f90:1901-1946). x, y: interleaved complex[ne]. */
 void or_matvec(or_mlfma *h, const double *x, double *y);

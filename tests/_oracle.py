@@ -31,6 +31,7 @@ def lib():
         L.or_mlfma_init.restype = C.c_void_p
         L.or_near_nnz.restype = C.c_longlong
         L.or_level_table.restype = C.c_longlong
+        L.or_far_lists.restype = C.c_longlong
         L.or_legendre.restype = C.c_double
         L.or_norm2.restype = C.c_double
         L.or_sigma_rcs.restype = C.c_double
@@ -105,6 +106,23 @@ class Oracle:
         lib().or_level_info(self.h, level, *[C.byref(x) for x in v], C.byref(npairs))
         nb, nm, nth, nph, nt = (x.value for x in v)
         return dict(nboxes=nb, nmult=nm, nth=nth, nph=nph, nk=nth * nph, ntransf=nt, npairs=npairs.value)
+
+    def far_lists(self, level):
+        """(ptr, boxes): each box's far list at `level`, 0-based, in list order."""
+        n = lib().or_far_lists(self.h, level, None, None)
+        nb = self.level_info(level)["nboxes"]
+        ptr = np.zeros(nb + 1, np.int32)
+        boxes = np.zeros(max(n, 1), np.int32)
+        lib().or_far_lists(self.h, level, _P(ptr), _P(boxes))
+        return ptr, boxes[:n]
+
+    def boxes(self, level):
+        """(centres (nb, 3), 0-based parent of each box) at `level`."""
+        nb = self.level_info(level)["nboxes"]
+        c = np.zeros((nb, 3))
+        par = np.zeros(nb, np.int32)
+        lib().or_box_centres(self.h, level, _P(c), _P(par))
+        return c, par - 1
 
     def table(self, level, which):
         n = lib().or_level_table(self.h, level, which, None)
