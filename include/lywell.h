@@ -226,7 +226,7 @@ int lw_bench_batch(lw_batch *b, int reps, double *matvec_ms, double *lusol_ms);
  * reference starts each angle of a sweep from the previous angle's current
  * times exp(-j k (d - d_prev) . e_centro) (main.f90:1017-1022).  Here a
  * pointwise MLP (in -> hidden x depth (tanh) -> out; FP32 master weights,
- * bfloat16 GEMM operands, FP32 accumulation on MFMA) maps 16 features per
+ * bfloat16 GEMM operands, FP32 accumulation on MFMA) maps 19 features per
  * unknown to 4 outputs o, and the guess for a new angle is
  *   (1 + o0 + j o1) xt1 + (o2 + j o3) (xt1 - xt2),
  * xt1 and xt2 being the two nearest solved currents with that phase

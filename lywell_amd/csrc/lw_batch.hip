@@ -971,15 +971,26 @@ __global__ void k_b_copy_masked(int R, int n, const int* __restrict__ active, co
 // Per-column CGS control on the device, so that iterations run back to back
 // (and replay as one captured graph) with the host looking in only every few
 // iterations.  st[c] = {active, first, converged, iterations}.
-__global__ void k_b_cgs_init(int R, const double2* __restrict__ nrm, const double2* __restrict__ rhon,
+// ||r|| / ||b||; a zero right-hand side counts a zero residual as 0
+__device__ __forceinline__ double rel_res(double nr, double nb) {
+  return nb > 0.0 ? nr / nb : (nr == 0.0 ? 0.0 : HUGE_VAL);
+}
+
+// Column state after the initial residual: nrm = |b|^2, rhon = r.R~, nr0 =
+// |r|^2.  A column whose rho is zero stops before its first update, as
+// CGSold does; it has converged if its initial residual passes the test.
+__global__ void k_b_cgs_init(int R, double errmax, const double2* __restrict__ nrm,
+                             const double2* __restrict__ rhon, const double2* __restrict__ nr0,
                              double* __restrict__ nbv, double* __restrict__ rr, int4* __restrict__ stv,
                              double2* __restrict__ rho1) {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c >= R) return;
-  nbv[c] = sqrt(nrm[c].x);
-  rr[c] = 0.0;
-  const bool brk = rhon[c].x == 0.0 && rhon[c].y == 0.0;  // CGSold exits before its first update
-  stv[c] = make_int4(brk ? 0 : 1, 1, 0, 0);
+  const double nb = sqrt(nrm[c].x), nr = sqrt(nr0[c].x);
+  nbv[c] = nb;
+  rr[c] = rel_res(nr, nb);
+  const bool brk = rhon[c].x == 0.0 && rhon[c].y == 0.0;
+  const bool conv = brk && (nr < errmax * nb || nr == 0.0);
+  stv[c] = make_int4(brk ? 0 : 1, 1, conv ? 1 : 0, 0);
   rho1[c] = rhon[c];
 }
 
@@ -1007,7 +1018,7 @@ __global__ void k_b_cgs_check(int R, double errmax, const double2* __restrict__ 
   int4 v = stv[c];
   if (v.x) {
     const double nr = sqrt(nrm[c].x);
-    rr[c] = nr / nbv[c];
+    rr[c] = rel_res(nr, nbv[c]);
     v.y = 0;
     if (nr < errmax * nbv[c]) {
       v.z = 1;
@@ -1037,32 +1048,36 @@ __global__ void k_b_col_set(int R, int n, int c, const double2* __restrict__ bc,
   rt[at] = ri;
 }
 
-__global__ void __launch_bounds__(256) k_b_col_init(int R, int n, int c, const double2* __restrict__ bc,
-                                                    const double2* __restrict__ zx, double* __restrict__ nbv,
-                                                    double* __restrict__ rr, int4* __restrict__ stv,
-                                                    double2* __restrict__ rho1) {
-  __shared__ double sh[3][256];
-  double nb = 0.0, qr = 0.0, qi = 0.0;
+__global__ void __launch_bounds__(256) k_b_col_init(int R, int n, int c, double errmax,
+                                                    const double2* __restrict__ bc, const double2* __restrict__ zx,
+                                                    double* __restrict__ nbv, double* __restrict__ rr,
+                                                    int4* __restrict__ stv, double2* __restrict__ rho1) {
+  __shared__ double sh[4][256];
+  double nb = 0.0, qr = 0.0, qi = 0.0, nr = 0.0;
   for (int i = threadIdx.x; i < n; i += blockDim.x) {
     const double2 bi = bc[i], ri = dsub(bc[i], zx[i]);
     nb += bi.x * bi.x + bi.y * bi.y;
     qr += ri.x * ri.x - ri.y * ri.y;  // unconjugated r . r
     qi += 2.0 * ri.x * ri.y;
+    nr += ri.x * ri.x + ri.y * ri.y;
   }
   sh[0][threadIdx.x] = nb;
   sh[1][threadIdx.x] = qr;
   sh[2][threadIdx.x] = qi;
+  sh[3][threadIdx.x] = nr;
   __syncthreads();
   for (int o = blockDim.x / 2; o > 0; o >>= 1) {
     if ((int)threadIdx.x < o)
-      for (int q = 0; q < 3; ++q) sh[q][threadIdx.x] += sh[q][threadIdx.x + o];
+      for (int q = 0; q < 4; ++q) sh[q][threadIdx.x] += sh[q][threadIdx.x + o];
     __syncthreads();
   }
   if (threadIdx.x == 0) {
-    nbv[c] = sqrt(sh[0][0]);
-    rr[c] = 0.0;
+    const double nb = sqrt(sh[0][0]), nr = sqrt(sh[3][0]);
+    nbv[c] = nb;
+    rr[c] = rel_res(nr, nb);
     const bool brk = sh[1][0] == 0.0 && sh[2][0] == 0.0;
-    stv[c] = make_int4(brk ? 0 : 1, 1, 0, 0);
+    const bool conv = brk && (nr < errmax * nb || nr == 0.0);
+    stv[c] = make_int4(brk ? 0 : 1, 1, conv ? 1 : 0, 0);
     rho1[c] = make_double2(sh[1][0], sh[2][0]);
   }
 }
@@ -1199,6 +1214,7 @@ struct Batch {
   double graph_errmax[2] = {0.0, 0.0};
   int graph_maxit[2] = {0, 0};
   hipGraphExec_t cur_graph = nullptr;  // the graph of the solve in progress
+  double errmax = 0.0;                 // the stopping tolerance of the solve in progress
   long long done = 0, next_report = 0;  // iterations replayed since the start
   std::vector<void*> allocs;
 };
@@ -1528,14 +1544,15 @@ static int batch_cgs_start(Batch* bt, double errmax, bool use_prec, int maxit) {
   const long long nv = (long long)h->ne * R;
   hipStream_t st = h->stream;
   double2 *b = bt->w[0], *r = bt->w[1], *R_ = bt->w[2], *t = bt->w[10];
-  double2 *rho1 = bt->sc, *nrm = bt->sc + 4 * R, *rhon = bt->sc + 5 * R;
+  double2 *rho1 = bt->sc, *nrm = bt->sc + 4 * R, *rhon = bt->sc + 5 * R, *nr0 = bt->sc + 6 * R;
   int rc = batch_matvec(bt, bt->X, t);
   if (rc) return rc;
   hipLaunchKernelGGL(k_b_sub, dim3(nblk(nv, 256)), dim3(256), 0, st, nv, b, t, r);
   LW_HIP(hipMemcpyAsync(R_, r, sizeof(double2) * nv, hipMemcpyDeviceToDevice, st));
   if ((rc = batch_dot2(bt, b, b, 1, r, R_, 0, nrm, rhon))) return rc;
-  hipLaunchKernelGGL(k_b_cgs_init, dim3(nblk(R, 64)), dim3(64), 0, st, R, nrm, rhon, bt->nbv, bt->rrv, bt->stv,
-                     rho1);
+  if ((rc = batch_dot2(bt, r, r, 1, nullptr, nullptr, 0, nr0, nullptr))) return rc;
+  hipLaunchKernelGGL(k_b_cgs_init, dim3(nblk(R, 64)), dim3(64), 0, st, R, errmax, nrm, rhon, nr0, bt->nbv,
+                     bt->rrv, bt->stv, rho1);
   LW_HIP(hipGetLastError());
   // the iteration graph bakes in errmax and maxit: recapture when they change
   hipGraphExec_t& gx = bt->iter_graph[use_prec ? 1 : 0];
@@ -1560,6 +1577,7 @@ static int batch_cgs_start(Batch* bt, double errmax, bool use_prec, int maxit) {
     bt->graph_maxit[use_prec] = maxit;
   }
   bt->cur_graph = gx;
+  bt->errmax = errmax;
   bt->done = 0;
   return LW_OK;
 }
@@ -1994,8 +2012,8 @@ int lw_batch_restart_column(lw_batch* ob, int col, const double* b, const double
   if (rc) return rc;
   hipLaunchKernelGGL(k_b_col_set, dim3(nblk(h->ne, 256)), dim3(256), 0, st, bt->R, h->ne, col, h->d_w[0], h->d_x,
                      h->d_y, bt->w[0], bt->X, bt->w[1], bt->w[2]);
-  hipLaunchKernelGGL(k_b_col_init, dim3(1), dim3(256), 0, st, bt->R, h->ne, col, h->d_w[0], h->d_y, bt->nbv,
-                     bt->rrv, bt->stv, bt->sc);
+  hipLaunchKernelGGL(k_b_col_init, dim3(1), dim3(256), 0, st, bt->R, h->ne, col, bt->errmax, h->d_w[0], h->d_y,
+                     bt->nbv, bt->rrv, bt->stv, bt->sc);
   LW_HIP(hipGetLastError());
   LW_HIP(hipStreamSynchronize(st));
   return LW_OK;

@@ -817,10 +817,6 @@ int lw_mlfma_init(double lambda, double ladomax, int flags, int num_e, const dou
     return LW_ERR_ARG;
   }
   *out = nullptr;
-  if (lw_device_count() < 1) {
-    lw::set_error("lw_mlfma_init: no HIP device visible");
-    return LW_ERR_NODEV;
-  }
   lw::Geometry g;
   g.ne = num_e;
   g.nt = num_t;
@@ -846,11 +842,18 @@ int lw_mlfma_init(double lambda, double ladomax, int flags, int num_e, const dou
   h->flags = flags;
   h->fp64_near = (flags & LW_NEAR_FP64) != 0;
   std::string msg;
+  // the octree and the host tables come first (host code), so that argument
+  // and octree errors are reported whether or not a device is present
   int rc = lw::build_plan(g, lambda, ladomax, h->plan, msg);
   if (rc) {
     lw::set_error(msg);
     delete hh;
     return rc;
+  }
+  if (lw_device_count() < 1) {
+    lw::set_error("lw_mlfma_init: no HIP device visible");
+    delete hh;
+    return LW_ERR_NODEV;
   }
   rc = lw::init_device_state(h, g);
   if (rc) {
@@ -1035,12 +1038,10 @@ int lw_ilu0_set(lw_handle* hh, const float* alu, const double* alu64, const int*
     lw::set_error("lw_ilu0_set: jlu does not describe an MSR factor of this operator");
     return LW_ERR_ARG;
   }
-  const size_t na = (size_t)nnz + 1;
-  h->h_jlu.assign(na + 1, 0);
-  std::memcpy(h->h_jlu.data() + 1, jlu, sizeof(int) * na);
-  h->h_ju.assign(ju, ju + n);
+  // every check runs on the caller's arrays before the handle is touched, so
+  // a rejected factor leaves the previous one (and its host copy) in place
   for (int i = 0; i < n; ++i)
-    if (ju[i] < jlu[i] || ju[i] > jlu[i + 1]) {
+    if (jlu[i + 1] < jlu[i] || ju[i] < jlu[i] || ju[i] > jlu[i + 1]) {
       lw::set_error("lw_ilu0_set: ju out of range");
       return LW_ERR_ARG;
     }
@@ -1049,12 +1050,19 @@ int lw_ilu0_set(lw_handle* hh, const float* alu, const double* alu64, const int*
       lw::set_error("lw_ilu0_set: column index out of range");
       return LW_ERR_ARG;
     }
+  if (h->fp64_near ? !alu64 : !alu) {
+    lw::set_error(h->fp64_near ? "lw_ilu0_set: the handle stores complex(8) factors: pass alu64"
+                               : "lw_ilu0_set: the handle stores complex(4) factors: pass alu");
+    return LW_ERR_ARG;
+  }
+  const size_t na = (size_t)nnz + 1;
+  h->h_jlu.assign(na + 1, 0);
+  std::memcpy(h->h_jlu.data() + 1, jlu, sizeof(int) * na);
+  h->h_ju.assign(ju, ju + n);
   if (h->fp64_near) {
-    if (!alu64) return LW_ERR_ARG;
     h->h_alu64.assign(2 * (na + 1), 0.0);
     std::memcpy(h->h_alu64.data() + 2, alu64, sizeof(double) * 2 * na);
   } else {
-    if (!alu) return LW_ERR_ARG;
     h->h_alu32.assign(2 * (na + 1), 0.0f);
     std::memcpy(h->h_alu32.data() + 2, alu, sizeof(float) * 2 * na);
   }
@@ -1121,20 +1129,32 @@ int lw_cgs(lw_handle* hh, const double* b, const double* guess, double errmax, i
   const int vb = lw::blocks_for(n, 256);
   hipLaunchKernelGGL(lw::k_sub, dim3(vb), dim3(256), 0, st, n, d_b, t, r);
   LW_HIP(hipMemcpyAsync(R_, r, bytes, hipMemcpyDeviceToDevice, st));
-  // |b|^2 (conjugated) and rho1 = sum(r*R_) (unconjugated)
+  // |b|^2 (conjugated) and rho1 = sum(r*R_) (unconjugated), then |r|^2 of
+  // the initial residual, so that every exit reports the residual of the
+  // iterate it returns
   if ((rc = lw::run_dot2(h, d_b, d_b, 1, r, R_, 0, s + 4))) return rc;
-  double2 hs[2];
+  if ((rc = lw::run_dot2(h, r, r, 1, nullptr, nullptr, 0, s + 6))) return rc;
+  double2 hs[3];
   LW_HIP(hipMemcpyAsync(hs, s + 4, sizeof(hs), hipMemcpyDeviceToHost, st));
   LW_HIP(hipStreamSynchronize(st));
   const double nb = std::sqrt(hs[0].x);
   double2 rho1 = hs[1];
+  // ||r|| / ||b||; a zero right-hand side counts a zero residual as 0
+  auto relative = [nb](double nr) { return nb > 0.0 ? nr / nb : (nr == 0.0 ? 0.0 : HUGE_VAL); };
+  double nr = std::sqrt(hs[2].x);
+  double rr = relative(nr);
   LW_HIP(hipMemcpyAsync(s + 0, s + 5, sizeof(double2), hipMemcpyDeviceToDevice, st));
   int it = 0;
   bool conv = false;
-  double rr = 0.0;
   while (it < maxit) {
     ++it;
-    if (rho1.x == 0.0 && rho1.y == 0.0) break;  // breakdown exit (modIterativo.f90:392-395)
+    if (rho1.x == 0.0 && rho1.y == 0.0) {
+      // breakdown exit (modIterativo.f90:392-395): x keeps the last iterate,
+      // which has converged if its residual passes the stopping test (an
+      // exact guess, or a zero right-hand side from a zero guess)
+      conv = nr < errmax * nb || nr == 0.0;
+      break;
+    }
     hipLaunchKernelGGL(lw::k_cgs_up, dim3(vb), dim3(256), 0, st, n, it == 1 ? 1 : 0, s + 0, s + 1, r, q, u, p);
     if (use_prec) {
       if ((rc = lw::run_lusol(h, p, P_))) return rc;
@@ -1155,8 +1175,8 @@ int lw_cgs(lw_handle* hh, const double* b, const double* guess, double errmax, i
     if ((rc = lw::run_dot2(h, r, r, 1, r, R_, 0, s + 4))) return rc;
     LW_HIP(hipMemcpyAsync(hs, s + 4, sizeof(hs), hipMemcpyDeviceToHost, st));
     LW_HIP(hipStreamSynchronize(st));
-    const double nr = std::sqrt(hs[0].x);
-    rr = nr / nb;
+    nr = std::sqrt(hs[0].x);
+    rr = relative(nr);
     if (nr < errmax * nb) {
       conv = true;
       break;
