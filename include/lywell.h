@@ -127,6 +127,16 @@ int lw_ilu0_get(lw_handle *h, float *alu, double *alu64, int *jlu, int *ju);
  * build or edit the factor themselves.  alu is complex(4) unless the handle
  * stores complex(8) (then pass alu64). */
 int lw_ilu0_set(lw_handle *h, const float *alu, const double *alu64, const int *jlu, const int *ju);
+/* ilu0C (modIterativo.f90:1805-1976) on any n x n CSR matrix held by the
+ * caller: ia(n+1), ja(nnz) 1-based, every row with ascending columns and its
+ * diagonal; a(nnz) and the factor alu(nnz+1) are complex(4) interleaved, or
+ * complex(8) with LW_NEAR_FP64 (the reference's dp_near = dp_prec = 8).
+ * Outputs the MSR arrays alu, jlu(nnz+1), ju(n) as ilu0C leaves them,
+ * factorised on the device like lw_ilu0.  *ierr = 0, or the first row with a
+ * zero pivot (then LW_ERR_PIVOT is returned and the factor is not usable).
+ * Needs a device; no handle. */
+int lw_ilu0_csr(int n, const int *ia, const int *ja, const void *a, int flags, void *alu, int *jlu, int *ju,
+                int *ierr);
 /* lusol0C (modIterativo.f90:504-560) with the handle's factor, host buffers. */
 int lw_lusol0(lw_handle *h, const double *y, double *x);
 

@@ -60,6 +60,33 @@ def device_count():
     return lib().lw_device_count()
 
 
+def ilu0_csr(ia, ja, a, fp64=False):
+    """ilu0C on a caller's CSR matrix (1-based ia/ja, ascending columns with
+    the diagonal), factored on the device: (ierr, alu, jlu, ju), alu as
+    complex128 (stored complex(4) unless fp64)."""
+    ia = _np.ascontiguousarray(ia, _np.int32)
+    ja = _np.ascontiguousarray(ja, _np.int32)
+    n = ia.size - 1
+    nnz = int(ia[-1]) - 1
+    if fp64:
+        av = _np.ascontiguousarray(a, _np.complex128)
+        alu = _np.zeros(nnz + 1, _np.complex128)
+    else:
+        av = _np.ascontiguousarray(a, _np.complex64)
+        alu = _np.zeros(nnz + 1, _np.complex64)
+    if av.size != nnz:
+        raise ValueError("a must hold ia[-1] - 1 values")
+    jlu = _np.zeros(nnz + 1, _np.int32)
+    ju = _np.zeros(n, _np.int32)
+    ierr = _C.c_int()
+    rc = lib().lw_ilu0_csr(int(n), _P(ia), _P(ja), _P(av), LW_NEAR_FP64 if fp64 else 0, _P(alu), _P(jlu), _P(ju),
+                           _C.byref(ierr))
+    if rc == LW_ERR_PIVOT:
+        return ierr.value, alu.astype(_np.complex128), jlu, ju
+    _check(rc)
+    return 0, alu.astype(_np.complex128), jlu, ju
+
+
 class Mesh:
     """Closed triangulated surface with the reference's RWG arrays.
 

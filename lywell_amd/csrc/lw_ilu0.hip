@@ -247,7 +247,7 @@ int ilu0_structure(int n, const std::vector<int>& ia, const std::vector<int>& ja
     bool diag = false;
     for (int j = ia[i]; j < ia[i + 1]; ++j) {
       if (j > ia[i] && ja[j] <= ja[j - 1]) {
-        set_error("ilu0: near-field columns of a row are not ascending");
+        set_error("ilu0: the columns of row " + std::to_string(i + 1) + " are not ascending");
         return LW_ERR_STATE;
       }
       if (ja[j] == i) {
@@ -347,3 +347,84 @@ int ilu0_device(Handle* h, int* piv) {
 }
 
 }  // namespace lw
+
+extern "C" {
+
+// ilu0C on any CSR matrix (modIterativo.f90:1805-1976): the same device
+// factorisation as lw_ilu0, on a transient operator-less context.
+int lw_ilu0_csr(int n, const int* ia, const int* ja, const void* a, int flags, void* alu, int* jlu, int* ju,
+                int* ierr) {
+  using namespace lw;
+  if (n < 1 || !ia || !ja || !a || !alu || !jlu || !ju || (flags & ~LW_NEAR_FP64)) {
+    set_error("lw_ilu0_csr: missing array, n < 1 or unknown flags");
+    return LW_ERR_ARG;
+  }
+  if (ierr) *ierr = 0;
+  if (ia[0] != 1 || ia[n] < 1) {
+    set_error("lw_ilu0_csr: ia must be 1-based with ia(1) = 1");
+    return LW_ERR_ARG;
+  }
+  const long long nnz = (long long)ia[n] - 1;
+  for (int i = 0; i < n; ++i)
+    if (ia[i + 1] < ia[i]) {
+      set_error("lw_ilu0_csr: ia is not monotone");
+      return LW_ERR_ARG;
+    }
+  for (long long k = 0; k < nnz; ++k)
+    if (ja[k] < 1 || ja[k] > n) {
+      set_error("lw_ilu0_csr: column index out of range");
+      return LW_ERR_ARG;
+    }
+  if (lw_device_count() < 1) {
+    set_error("lw_ilu0_csr: no HIP device visible");
+    return LW_ERR_NODEV;
+  }
+  Handle h;
+  h.ne = n;
+  h.nnz = nnz;
+  h.fp64_near = (flags & LW_NEAR_FP64) != 0;
+  h.plan.ia.resize((size_t)n + 1);
+  h.plan.ja.resize((size_t)nnz);
+  for (int i = 0; i <= n; ++i) h.plan.ia[i] = ia[i] - 1;
+  for (long long k = 0; k < nnz; ++k) h.plan.ja[k] = ja[k] - 1;
+  const size_t esz = h.fp64_near ? sizeof(double2) : sizeof(float2);
+  int rc = LW_OK, piv = 0;
+  auto cleanup = [&h]() {
+    if (h.stream) (void)hipStreamSynchronize(h.stream);
+    for (void* p : h.allocs) (void)hipFree(p);
+    if (h.stream) (void)hipStreamDestroy(h.stream);
+  };
+  if (hipGetDevice(&h.device) != hipSuccess || hipStreamCreateWithFlags(&h.stream, hipStreamNonBlocking) != hipSuccess) {
+    set_error("lw_ilu0_csr: stream creation failed");
+    cleanup();
+    return LW_ERR_HIP;
+  }
+  void* dz = nullptr;
+  if (!(rc = upload(&h, &h.d_ia, h.plan.ia)) && !(rc = upload(&h, &h.d_ja, h.plan.ja)) &&
+      !(rc = dmalloc(&h, (char**)&dz, (size_t)nnz * esz))) {
+    h.d_z = dz;
+    if (hipMemcpyAsync(dz, a, (size_t)nnz * esz, hipMemcpyHostToDevice, h.stream) != hipSuccess) {
+      set_error("lw_ilu0_csr: upload failed");
+      rc = LW_ERR_HIP;
+    }
+  }
+  if (!rc) rc = ilu0_device(&h, &piv);
+  cleanup();
+  if (rc) return rc;
+  // the MSR arrays as ilu0C leaves them (the host copies carry a pad element)
+  const size_t na = (size_t)nnz + 1;
+  if (h.fp64_near)
+    std::copy(h.h_alu64.begin() + 2, h.h_alu64.begin() + 2 + 2 * na, (double*)alu);
+  else
+    std::copy(h.h_alu32.begin() + 2, h.h_alu32.begin() + 2 + 2 * na, (float*)alu);
+  std::copy(h.h_jlu.begin() + 1, h.h_jlu.begin() + 1 + na, jlu);
+  std::copy(h.h_ju.begin(), h.h_ju.end(), ju);
+  if (piv) {
+    if (ierr) *ierr = piv;
+    set_error("lw_ilu0_csr: zero pivot at row " + std::to_string(piv));
+    return LW_ERR_PIVOT;
+  }
+  return LW_OK;
+}
+
+}  // extern "C"

@@ -108,6 +108,15 @@ module lw_capi
       integer(c_int) :: rc
     end function
 
+    function lw_ilu0_csr(n, ia, ja, a, flags, alu, jlu, ju, ierr) result(rc) bind(C, name="lw_ilu0_csr")
+      import :: c_int, c_ptr
+      integer(c_int), value :: n, flags
+      integer(c_int), intent(in) :: ia(*), ja(*)
+      type(c_ptr), value :: a, alu
+      integer(c_int), intent(out) :: jlu(*), ju(*), ierr
+      integer(c_int) :: rc
+    end function
+
     function lw_ilu0_get(h, alu, alu64, jlu, ju) result(rc) bind(C, name="lw_ilu0_get")
       import :: c_int, c_ptr
       type(c_ptr), value :: h, alu, alu64

@@ -2,7 +2,8 @@
 ! with their F77-style dummy arguments, over the lywell C ABI.
 !   CGSold   modIterativo.f90:331-493   right-preconditioned CGS; with
 !            MatxVec_MLFMA as the operator the whole loop runs on the GPU
-!   ilu0C    modIterativo.f90:1805-1976 ILU(0) of the MLFMA near field (MSR)
+!   ilu0C    modIterativo.f90:1805-1976 ILU(0) in MSR form, of the MLFMA near
+!            field or of any CSR matrix with ascending columns
 !   lusol0C  modIterativo.f90:504-560   forward/backward solve with the factor
 ! Only the MLFMA operator of modMLFMA is accepted: there is no CPU path, and
 ! CGSold stops with a message for any other operator.
@@ -82,15 +83,17 @@ contains
     complex(kind=dp_prec), target :: alu(*)
     integer(kind=il) :: ia(n+1), ja(*), jlu(*), ju(*), iw(n)
     integer(kind=il) :: ierr
-    integer(c_int) :: rc, piv
+    integer(c_int) :: rc, piv, flags
 
-    if (.not. c_associated(lw_mlfma_handle) .or. n /= lw_mlfma_ne) &
-      call lw_fail('ilu0C: MLFMA operator not initialised for this size', LW_ERR_STATE)
-    if (.not. c_associated(c_loc(a(1)), c_loc(Zesparcida_MLFMA(1)))) then
-      print *, 'ilu0C: only the MLFMA near field (Zesparcida_MLFMA) is factored on the GPU path'
-      error stop 'lywell: unsupported matrix'
-    end if
     iw(1:n) = 0
+    if (.not. (c_associated(lw_mlfma_handle) .and. n == lw_mlfma_ne .and. allocated(Zesparcida_MLFMA))) then
+      call ilu0_any()
+      return
+    end if
+    if (.not. c_associated(c_loc(a(1)), c_loc(Zesparcida_MLFMA(1)))) then
+      call ilu0_any()
+      return
+    end if
     rc = lw_ilu0(lw_mlfma_handle, piv)
     if (rc == LW_ERR_PIVOT) then
       ierr = piv
@@ -105,6 +108,24 @@ contains
     if (rc /= LW_OK) call lw_fail('ilu0C', rc)
     factor_on_device = c_loc(alu(1))
     ierr = 0
+
+  contains
+
+    ! any other CSR matrix: factored on the device without an operator; the
+    ! factor stays with the caller (CGSold and lusol0C upload it)
+    subroutine ilu0_any()
+      flags = 0
+      if (dp_prec == 8) flags = LW_NEAR_FP64
+      if (dp_near /= dp_prec) call lw_fail('ilu0C: the matrix and the factor must share a precision', LW_ERR_ARG)
+      rc = lw_ilu0_csr(n, ia, ja, c_loc(a(1)), flags, c_loc(alu(1)), jlu, ju, piv)
+      if (rc == LW_ERR_PIVOT) then
+        ierr = piv
+        return
+      end if
+      if (rc /= LW_OK) call lw_fail('ilu0C', rc)
+      factor_on_device = c_null_ptr
+      ierr = 0
+    end subroutine
   end subroutine
 
   subroutine lusol0C(n, y, x, alu, jlu, ju)

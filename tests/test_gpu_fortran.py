@@ -1,0 +1,69 @@
+"""The Fortran boundary on the GPU: lywell_amd/lw_solve is a Fortran host
+program that runs the reference's testMLFMA flow (main.f90:1092-1154)
+through the modules modMLFMA / modIterativo / modRCS, whose routines call the
+C ABI over ISO_C_BINDING:
+
+  inicializar_MLFMA -> MatxVec_MLFMA(v) -> generar_onda -> ilu0C -> CGSold ->
+  inicializar_RCS / sigmaRCS
+
+Its output stream (y = MatxVec(v) for the deterministic v, the current, the
+iteration count and sigma at 181 theta in the phi = 0 cut) is compared with
+what the reference's own Fortran returned on the same 1,920-RWG sphere
+(tests/golden/ref_sphere1920.npz), with and without ILU0.
+"""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import lywell_amd as lw
+
+pytestmark = pytest.mark.gpu
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+EXE = os.path.join(ROOT, "lywell_amd", "lw_solve")
+GOLD = os.path.join(ROOT, "tests", "golden", "ref_sphere1920.npz")
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _need_device():
+    if lw.device_count() < 1:
+        pytest.skip("no HIP device")
+
+
+def run_solve(tmp_path, prec, ntheta=181):
+    out = str(tmp_path / f"solve_{prec}.bin")
+    cmd = [EXE, "3", "0.5", "0.5", "0.5", "1.0", "0.2", "1e-4", str(prec), str(ntheta), out]
+    print("[fortran]", " ".join(cmd), flush=True)
+    r = subprocess.run(cmd, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    with open(out, "rb") as f:
+        ne, iters = np.frombuffer(f.read(8), np.int32)
+        relres = np.frombuffer(f.read(8), np.float64)[0]
+        y = np.frombuffer(f.read(16 * ne), np.complex128)
+        cur = np.frombuffer(f.read(16 * ne), np.complex128)
+        sig = np.frombuffer(f.read(8 * ntheta), np.float64)
+    return int(ne), int(iters), relres, y, cur, sig, r.stdout
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+@pytest.mark.parametrize("prec", [1, 0])
+def test_fortran_testmlfma_flow_matches_the_reference(tmp_path, prec):
+    g = np.load(GOLD)
+    ne, iters, relres, y, cur, sig, stdout = run_solve(tmp_path, prec)
+    key = "prec" if prec else "noprec"
+    it_ref = int(g["nmv_" + key]) // 2
+    dsig = np.abs(10 * np.log10(sig) - 10 * np.log10(g["sigma_" + key])).max()
+    print(f"[fortran {key}] ne {ne}, iterations {iters} (reference {it_ref}), relres {relres:.3e}, "
+          f"MatxVec {rel(y, g['y1']):.2e}, current {rel(cur, g['x_' + key]):.2e}, RCS max {dsig:.2e} dB")
+    assert ne == 1920
+    assert rel(y, g["y1"]) < 1e-10
+    assert relres < 1e-4
+    assert abs(iters - it_ref) <= max(3, 0.05 * it_ref)
+    assert rel(cur, g["x_" + key]) < 2e-3
+    assert dsig < 0.01
+    assert "CGS iteraciones" in stdout

@@ -1,0 +1,163 @@
+"""The GPU path against the reference's own results on the 1,920-RWG sphere
+(radius 0.5 lambda, icosphere subdivision 3, ladomax 0.2, 4 levels).
+
+tests/golden/ref_sphere1920.npz holds what the reference's Fortran (built
+with flang) returned for this mesh: CGSold with ILU0 to 1e-4 (x_prec, the
+matvec count nmv_prec, sigma_prec at 181 theta in the phi = 0 cut) and
+without a preconditioner (x_noprec, nmv_noprec, sigma_noprec), for the
+x-polarised plane wave travelling along -z (main.f90:1092-1154).
+
+Also here: the all-FP64 variant (the reference's dp_near = dp_prec = 8,
+LW_NEAR_FP64) against the oracle built the same way, and the exits of CGSold
+(breakdown at rho = 0, modIterativo.f90:392-395, and the iteration cap).
+"""
+import os
+
+import numpy as np
+import pytest
+
+import lywell_amd as lw
+from _oracle import Oracle
+
+pytestmark = pytest.mark.gpu
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "ref_sphere1920.npz")
+_cache = {}
+
+
+def golden():
+    return np.load(GOLD)
+
+
+def op(fp64_near=False):
+    key = ("op", fp64_near)
+    if key not in _cache:
+        g = golden()
+        mesh = lw.Mesh(g["p"], g["t"], g["nrm"])
+        _cache[key] = (mesh, lw.MLFMA(mesh, lam=float(g["lam"]), ladomax=float(g["ladomax"]), fp64_near=fp64_near))
+    return _cache[key]
+
+
+def rel(a, b):
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+def db(s):
+    return 10 * np.log10(s)
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _need_device():
+    if lw.device_count() < 1:
+        pytest.skip("no HIP device")
+
+
+def test_rwg_and_matvec_match_the_reference():
+    g = golden()
+    mesh, gpu = op()
+    assert np.array_equal(mesh.e_t, g["e_t"]) and np.array_equal(mesh.e_po, g["e_po"])
+    assert rel(gpu.matvec(lw.det_vector(mesh.ne)), g["y1"]) < 1e-10
+    b = gpu.plane_wave(g["d"], g["pol"], complex(g["ctte"]))
+    assert rel(b, g["b"]) < 1e-12
+    assert rel(gpu.matvec(b), g["y2"]) < 1e-10
+
+
+@pytest.mark.parametrize("use_prec", [True, False])
+def test_cgs_to_1e4_against_the_reference(use_prec):
+    g = golden()
+    mesh, gpu = op()
+    b = gpu.plane_wave(g["d"], g["pol"], complex(g["ctte"]))
+    if use_prec:
+        assert gpu.ilu0() == 0
+    x, it, rr, conv = gpu.cgs(b, errmax=float(g["errmax"]), use_prec=use_prec, maxit=20000)
+    key = "prec" if use_prec else "noprec"
+    it_ref = int(g["nmv_" + key]) // 2  # two matvecs per iteration
+    theta = g["theta"]
+    s = gpu.rcs(x, g["pol"], theta, float(g["phi"]), complex(g["ctte"]))
+    dsig = np.abs(db(s) - db(g["sigma_" + key])).max()
+    print(f"[{key}] iterations {it} (reference {it_ref}), relres {rr:.3e}, solution vs reference "
+          f"{rel(x, g['x_' + key]):.2e}, RCS max {dsig:.2e} dB")
+    assert conv and rr < 1e-4
+    # the reference's count within 5%: CGS magnifies last-bit differences of
+    # the operator (complex(4) rounding of the near field, summation order)
+    assert abs(it - it_ref) <= max(3, 0.05 * it_ref)
+    # two solves stopped at 1e-4 agree to about the residual's size
+    assert rel(x, g["x_" + key]) < 2e-3
+    assert dsig < 0.01
+    # the true residual of the GPU solution
+    assert rel(gpu.matvec(x), b) < 2e-4
+
+
+def test_fp64_near_variant_against_the_oracle():
+    # dp_near = dp_prec = 8: complex(8) near field and ILU0 factor
+    mesh, gpu = op(fp64_near=True)
+    g = golden()
+    ora = Oracle(mesh, lam=float(g["lam"]), ladomax=float(g["ladomax"]), fp64_near=True)
+    try:
+        ia, ja, z = gpu.near_csr()
+        ia_o, ja_o, z_o = ora.near_csr()
+        assert np.array_equal(ia, ia_o) and np.array_equal(ja, ja_o)
+        # FP64 storage: the entries agree to the rounding of their sums
+        rowmax = np.maximum.reduceat(np.abs(z_o), ia[:-1] - 1)
+        assert (np.abs(z - z_o) / np.repeat(rowmax, np.diff(ia))).max() < 1e-13
+        for x in (lw.det_vector(mesh.ne), np.random.default_rng(3).standard_normal(mesh.ne) + 0j):
+            assert rel(gpu.matvec(x), ora.matvec(x)) < 1e-10
+            yf, yn = gpu.matvec_parts(x)
+            yf_o, yn_o = ora.matvec_parts(x)
+            assert rel(yf, yf_o) < 1e-12 and rel(yn, yn_o) < 1e-12
+        assert gpu.ilu0() == 0
+        alu, jlu, ju = gpu.ilu0_factor()
+        ierr, alu_o, jlu_o, ju_o = ora.ilu0(fp64=True)
+        assert ierr == 0
+        assert np.array_equal(jlu, jlu_o) and np.array_equal(ju, ju_o)
+        assert rel(alu, alu_o) < 1e-12
+        y = np.random.default_rng(4).standard_normal(mesh.ne) + 1j
+        assert rel(gpu.lusol0(y), ora.lusol0(y, factor=(alu, jlu, ju))) < 1e-12
+        b = gpu.plane_wave(g["d"], g["pol"])
+        ora.factor = (alu, jlu, ju)
+        for use_prec in (False, True):
+            x, it, rr, conv = gpu.cgs(b, errmax=1e-12, use_prec=use_prec, maxit=4)
+            x_o, it_o, rr_o, conv_o = ora.cgs(b, errmax=1e-12, use_prec=use_prec, maxit=4)
+            assert it == it_o == 4 and not conv and not conv_o
+            assert rel(x, x_o) < 1e-10
+    finally:
+        ora.close()
+
+
+def test_cgs_exits():
+    mesh, gpu = op()
+    n = mesh.ne
+    # zero right-hand side from a zero guess: rho = 0 at the first iteration,
+    # x = 0 is exact
+    x, it, rr, conv = gpu.cgs(np.zeros(n, complex), errmax=1e-4, use_prec=False, maxit=50)
+    assert it == 1 and conv and rr == 0.0 and not x.any()
+    # an exact guess: the operator is deterministic, so b - Z x0 is exactly 0
+    x0 = lw.det_vector(n)
+    b = gpu.matvec(x0)
+    x, it, rr, conv = gpu.cgs(b, guess=x0, errmax=1e-4, use_prec=False, maxit=50)
+    assert it == 1 and conv and rr == 0.0 and np.array_equal(x, x0)
+    # the iteration cap: not converged, relres is the returned iterate's
+    b = gpu.plane_wave([0, 0, -1.0], [1, 0, 0])
+    x, it, rr, conv = gpu.cgs(b, errmax=1e-12, use_prec=False, maxit=7)
+    assert it == 7 and not conv and rr > 1e-12
+    x8, it8, rr8, _ = gpu.cgs(b, errmax=1e-12, use_prec=False, maxit=8)
+    # the recurrence residual of iterate 7 is within rounding of its true residual here
+    assert abs(rr - rel(gpu.matvec(x), b)) < 1e-6 * max(rr, 1.0)
+    assert not np.array_equal(x, x8)
+
+
+def test_batched_cgs_exits():
+    mesh, gpu = op()
+    n = mesh.ne
+    bt = lw.Batch(gpu, 64, far_fp64=True)
+    try:
+        B = np.zeros((n, 64), complex)
+        b = gpu.plane_wave([0, 0, -1.0], [1, 0, 0])
+        B[:, 1:] = b[:, None] * (1 + 0.1j * np.arange(1, 64))[None, :]
+        X, it, rr, cv = bt.cgs(B, errmax=1e-12, use_prec=False, maxit=6)
+        # column 0 has a zero right-hand side: it stops at once, converged
+        assert it[0] == 0 and cv[0] and rr[0] == 0.0 and not X[:, 0].any()
+        # the others reach the cap unconverged with their recurrence residual
+        assert (it[1:] == 6).all() and not cv[1:].any() and (rr[1:] > 1e-12).all()
+    finally:
+        bt.close()
