@@ -158,6 +158,15 @@ int lw_rcs_init(lw_handle *h, const double *current, const double *pol, const do
 /* sigmaRCS (modRCS.f90:91-105) at n angle pairs (radians) -> sigma in m^2. */
 int lw_rcs_sigma(lw_handle *h, int n, const double *theta, const double *phi, double *sigma);
 
+/* sigmaRCS at n angle pairs like lw_rcs_sigma, allocating no device memory:
+ * no counterpart in the reference; for callers that cannot allocate during
+ * the call (stream capture, serving loops).  Its scratch (angles, partial
+ * fields, sigmas) is a fixed static device array per GPU, so
+ * n <= LW_RCS_NOALLOC_MAX_ANGLES (else LW_ERR_ARG), and calls that overlap in
+ * time on one GPU race.  Host buffers, synchronous like lw_rcs_sigma. */
+#define LW_RCS_NOALLOC_MAX_ANGLES 4096
+int lw_rcs_sigma_noalloc(lw_handle *h, int n, const double *theta, const double *phi, double *sigma);
+
 /* ------------------------------------------------------------------ measurement */
 
 /* Time `reps` device-resident matvecs (after one warm-up) with HIP events on

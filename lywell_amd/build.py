@@ -48,6 +48,7 @@ def build_hip():
         ("lw_device.hip", []),
         ("lw_batch.hip", []),
         ("lw_mlp.hip", []),
+        ("lw_rcs_static.hip", []),
         # bit-exact elimination order and roundings of ilu0C: no contraction
         ("lw_ilu0.hip", ["-ffp-contract=off"]),
         # bit-exact Legendre sums: no contraction into FMAs in this unit

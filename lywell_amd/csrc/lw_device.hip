@@ -649,6 +649,33 @@ int check_handle(lw_handle* hh) {
 
 int upload_factor(Handle* h);
 
+// calc_RCS (modRCS.f90:109-153): the far-zone distance r_zl = 10 max(2 D^2 /
+// lambda, 50 D, 20 lambda) and the centre of the bounding box of the edge
+// centres (limites_dispersor), D its diagonal.
+static void rcs_far_zone(const double* ec, int ne, double w, double& rzl, double* roo) {
+  int lim[6] = {0, 0, 0, 0, 0, 0};
+  for (int i = 1; i < ne; ++i) {
+    const double* c = &ec[3 * (size_t)i];
+    if (c[0] > ec[3 * lim[0]]) lim[0] = i;
+    if (c[0] < ec[3 * lim[1]]) lim[1] = i;
+    if (c[1] > ec[3 * lim[2] + 1]) lim[2] = i;
+    if (c[1] < ec[3 * lim[3] + 1]) lim[3] = i;
+    if (c[2] > ec[3 * lim[4] + 2]) lim[4] = i;
+    if (c[2] < ec[3 * lim[5] + 2]) lim[5] = i;
+  }
+  const double mx = ec[3 * lim[0]], mnx = ec[3 * lim[1]], my = ec[3 * lim[2] + 1], mny = ec[3 * lim[3] + 1],
+               mz = ec[3 * lim[4] + 2], mnz = ec[3 * lim[5] + 2];
+  const double lam = 2 * kPi / (w * std::sqrt(kMu * kEps));
+  const double D = std::sqrt(((mx - mnx) * (mx - mnx) + (my - mny) * (my - mny)) + (mz - mnz) * (mz - mnz));
+  rzl = 2 * (D * D) / lam;
+  if (50 * D > rzl) rzl = 50 * D;
+  if (20 * lam > rzl) rzl = 20 * lam;
+  rzl = rzl * 10;
+  roo[0] = 0.5 * (mx + mnx);
+  roo[1] = 0.5 * (my + mny);
+  roo[2] = 0.5 * (mz + mnz);
+}
+
 static int init_device_state(Handle* h, const Geometry& g) {
   Plan& P = h->plan;
   const int L = P.L;
@@ -768,6 +795,11 @@ static int init_device_state(Handle* h, const Geometry& g) {
   TRY(dmalloc(h, &h->d_y, (size_t)P.ne));
   for (auto& w : h->d_w) TRY(dmalloc(h, &w, (size_t)P.ne));
   TRY(dmalloc(h, &h->d_part, 4 * (size_t)kRedBlocks));
+  // the RCS current (lw_rcs_init copies into it), the far-zone distance and
+  // the static scratch of the allocation-free RCS entry point
+  TRY(dmalloc(h, &h->d_cur, (size_t)P.ne));
+  rcs_far_zone(g.e_centro, P.ne, P.phys.w, h->rcs_rzl, h->rcs_centre);
+  TRY(rcs_static_setup(h));
   h->scal_cap = 64;
   TRY(dmalloc(h, &h->d_scal, (size_t)h->scal_cap));
   LW_HIP(hipStreamSynchronize(h->stream));
@@ -1215,8 +1247,6 @@ int lw_plane_wave(lw_handle* hh, const double* dir, const double* ctte, const do
 int lw_rcs_init(lw_handle* hh, const double* current, const double* pol, const double* ctte) {
   if (lw::check_handle(hh) || !current || !pol || !ctte) return LW_ERR_ARG;
   Handle* h = &hh->h;
-  if (!h->d_cur)
-    if (int rc = lw::dmalloc(h, &h->d_cur, (size_t)h->ne)) return rc;
   LW_HIP(hipMemcpyAsync(h->d_cur, current, sizeof(double2) * h->ne, hipMemcpyHostToDevice, h->stream));
   for (int c = 0; c < 3; ++c) h->pol[c] = make_double2(pol[2 * c], pol[2 * c + 1]);
   h->ctte = make_double2(ctte[0], ctte[1]);

@@ -146,6 +146,11 @@ struct Handle {
   double2 pol[3] = {};
   double2 ctte = {};
   bool rcs_ready = false;
+  // far-zone distance r_zl and the centre of the edge-centre bounding box
+  // (calc_RCS, modRCS.f90:109-153), fixed by the geometry
+  double rcs_rzl = 0.0, rcs_centre[3] = {0.0, 0.0, 0.0};
+  // this device's static scratch of lw_rcs_sigma_noalloc (lw_rcs_static.hip)
+  double* rcs_scratch = nullptr;
   std::vector<void*> allocs;
 };
 
@@ -190,6 +195,10 @@ int upload_structure(Handle* h);
 // row with a zero pivot.
 int ilu0_device(Handle* h, int* piv);
 
+// Looks up this device's address of the static scratch array of
+// lw_rcs_sigma_noalloc (once per device) and keeps it in h->rcs_scratch
+// (lw_rcs_static.hip).
+int rcs_static_setup(Handle* h);
 // Parent-block tables of the single-RHS translation (lw_batch.hip). The far
 // lists of every level must regroup into parent-neighbour blocks (checked
 // against the plan's per-box lists); LW_ERR_STATE otherwise.
