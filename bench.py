@@ -4,8 +4,9 @@ Workload (the single-GPU configuration the metric is quoted on): PEC sphere of
 radius 4 lambda (lambda = 1 m), icosphere subdivision 6 = 81,920 triangles =
 122,880 RWG unknowns, ladomax 0.25 -> 6-level MLFMA, EFIE.
 
-value: the plain drop-in path, one plane wave (x-polarised, travelling along
--z) solved by CGSold semantics to relative residual 1e-4 from a zero guess:
+value: the plain drop-in path, one plane wave (travelling along -z, its
+polarisation turned in the xy-plane from step to step) solved by CGSold
+semantics to relative residual 1e-4 from a zero guess:
 FP64 vectors, complex(4)-stored near field, no preconditioner (ILU0-CGS
 diverges on this mesh, as the reference's own algorithm does; the ILU0 path
 is reported by a short probe).  A step is one such solve; inputs are resident
@@ -72,7 +73,8 @@ def main():
                          "weights)")
     ap.add_argument("--fp32-probe-iters", type=int, default=300,
                     help="batched CGS iterations of the FP32-far-field probe (0: skip)")
-    ap.add_argument("--cpu-iters", type=int, default=3)
+    ap.add_argument("--cpu-iters", type=int, default=10,
+                    help="CGS iterations of the CPU-baseline sample (about 1.2 s each at 122,880 RWG)")
     ap.add_argument("--ellipsoid-sweep", type=int, default=0,
                     help="also run the batched monostatic sweep of the 4 x 1 x 1 lambda prolate ellipsoid "
                          "(30,720 RWG) with this many azimuths per polarisation over 360 degrees, VV and HH "
@@ -236,7 +238,9 @@ def main():
         "config": {
             "workload": f"PEC sphere R={args.radius} lambda, icosphere s={args.subdiv}, {mesh.ne} RWG, "
                         f"{op.nlevels}-level MLFMA (ladomax {args.ladomax}), "
-                        f"{'ILU0-' if use_prec else ''}CGS to {args.errmax:g}, x-pol plane wave along -z",
+                        f"{'ILU0-' if use_prec else ''}CGS to {args.errmax:g} from zero, plane waves along -z "
+                        f"with the polarisation turned in the xy-plane from step to step "
+                        f"(step i: pol = (cos a_i, sin a_i, 0), a_i = pi i / (warmup + steps))",
             "unknowns": mesh.ne, "levels": op.nlevels, "near_nnz": nnz,
             "preconditioner": "ILU0" if use_prec else "none",
         },
@@ -548,7 +552,8 @@ def cpu_baseline(mesh, args, use_prec, gpu_iters):
     ora = _oracle.Oracle(mesh, lam=1.0, ladomax=args.ladomax)  # setup may use OpenMP threads
     if use_prec:
         ora.ilu0()
-    log(f"cpu oracle setup {time.time() - t0:.1f}s")
+    setup_s = time.time() - t0
+    log(f"cpu oracle setup {setup_s:.1f}s")
     b = ora.plane_wave([0, 0, -1.0], [1, 0, 0])
     n = max(1, args.cpu_iters)
     t0 = time.time()
@@ -556,10 +561,20 @@ def cpu_baseline(mesh, args, use_prec, gpu_iters):
     dt = time.time() - t0
     per_it = dt / n
     log(f"cpu oracle: {n} CGS iterations in {dt:.1f}s")
+    model = None
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
     return {"value": round(1.0 / (per_it * gpu_iters), 8), "unit": "RHS/s", "cores": 1, "kind": "port",
             "sample": f"{n} {'ILU0-' if use_prec else ''}CGS iterations of the C oracle at {mesh.ne} RWG "
                       f"(1 core, {per_it:.2f} s/iteration) scaled by the GPU's {gpu_iters:.0f} iterations to 1e-4",
-            "seconds_per_iteration": round(per_it, 3)}
+            "seconds_per_iteration": round(per_it, 3), "setup_s": round(setup_s, 1),
+            "cpu_model": model, "host_cpus_visible": ncores, "host_nproc": os.cpu_count()}
 
 
 if __name__ == "__main__":

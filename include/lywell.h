@@ -73,6 +73,22 @@ int lw_rwg_build(int np, const double *p_coord, int nt, const int *t_p, int *e_p
 int lw_tri_geometry(int np, const double *p_coord, int nt, const int *t_p, double *t_area,
                     double *t_baric, double *t_baric_sub);
 
+/* ------------------------------------------------------------------ result files
+ * The reference's text writers (modGlobalMetod.f90:186-306), same layout and
+ * F16.8 / I10 fields, so outputs diff against the reference's own files.
+ * Host code, no GPU use.  Fields that do not fit are asterisks; NaN and
+ * infinities are right-justified NaN / Inf / -Inf, as the flang build of the
+ * reference writes them (ifort writes Infinity). */
+/* guardarVectorC: v complex(8)[n] interleaved; one "re im" line per entry. */
+int lw_dat_write_vector_c(const char *path, int n, const double *v);
+/* guardarMatrizR: m(nrows, ncols) column-major (Fortran order); one line per row. */
+int lw_dat_write_matrix_r(const char *path, int nrows, int ncols, const double *m);
+/* guardarVectorR and guardarVectorI: one value per line. */
+int lw_dat_write_vector_r(const char *path, int n, const double *v);
+int lw_dat_write_vector_i(const char *path, int n, const int *v);
+/* One F16.8 field of v into out17 (16 characters + terminator). */
+int lw_dat_format_f16_8(double v, char *out17);
+
 /* ------------------------------------------------------------------ modMLFMA */
 
 /* inicializar_MLFMA (modMLFMA.f90:113-275).  Arguments mirror the
@@ -176,9 +192,12 @@ int lw_rcs_sigma_noalloc(lw_handle *h, int n, const double *theta, const double 
 int lw_bench_matvec(lw_handle *h, int reps, double *ms, double *stage_ms);
 /* Time `reps` ILU0 triangular solves on device-resident vectors. */
 int lw_bench_lusol(lw_handle *h, int reps, double *ms);
-/* Algorithmic HBM bytes of one matvec in the reference data layout
- * (pattern arrays twice, campo arrays 6x, transfer tables once, near CSR,
- * x and y), and of the dominant kernels. */
+/* Algorithmic HBM bytes of one matvec in this library's layout: the
+ * aggregation's phase rows (one exp(j k khat.(c_T - C)) row per leaf box and
+ * triangle of its bases) and RWG vectors, the receive's pattern array, campo
+ * arrays 6x, transfer tables once, near CSR, x and y; and of the dominant
+ * kernels.  (The reference's layout reads the pattern array twice instead of
+ * phase rows + patterns.) */
 int lw_matvec_bytes(lw_handle *h, double *total, double *aggregation, double *receive_near);
 /* Mean time in ms of a device-to-device copy of `bytes` bytes (a multiple of
  * 16) by the library's copy kernel on the handle's stream: the measured
@@ -204,12 +223,28 @@ typedef struct lw_batch lw_batch;
 
 /* Flags for lw_batch_create. */
 #define LW_BATCH_FAR_FP64 1 /* far field in complex(8) (default: complex(4)) */
+#define LW_BATCH_FP64_RESIDUAL 2 /* complex(4) far field in the iteration, judged by
+                                    true residuals b - Z x with the complex(8)
+                                    far field (a second operator on the handle):
+                                    a column counts as converged only when its
+                                    true residual meets errmax (otherwise CGSold
+                                    starts afresh from its iterate), and with
+                                    lw_batch_set_replacement the recurrence
+                                    residual of every active column is replaced
+                                    by its true residual periodically */
 
 /* nrhs in {64, 128}. */
 int lw_batch_create(lw_handle *h, int nrhs, int flags, lw_batch **out);
 int lw_batch_destroy(lw_batch *b);
 /* The block width, or LW_ERR_ARG. */
 int lw_batch_nrhs(lw_batch *b);
+/* LW_BATCH_FP64_RESIDUAL batches: replace the recurrence residual (and
+ * CGSold's rho) of every active column by its true FP64 residual every
+ * `period` iterations (checked every 8 iterations; 0 = never, the default).
+ * Reliable updating in the sense of van der Vorst and Ye (2000). */
+int lw_batch_set_replacement(lw_batch *b, int period);
+/* Number of FP64 true-residual passes since the batch was created. */
+long long lw_batch_true_residual_passes(lw_batch *b);
 /* Y = Z X column by column (MatxVec_MLFMA per column, FP32 far field). */
 int lw_batch_matvec(lw_batch *b, const double *x, double *y);
 /* lusol0C per column with the handle's ILU0 factor. */

@@ -60,6 +60,34 @@ def device_count():
     return lib().lw_device_count()
 
 
+def write_dat_vector_c(path, v):
+    """guardarVectorC (modGlobalMetod.f90:283-306): one "re im" F16.8 pair per line."""
+    v = _np.ascontiguousarray(v, _np.complex128).ravel()
+    _check(lib().lw_dat_write_vector_c(_os.fsencode(path), int(v.size), _P(v)))
+
+
+def write_dat_matrix_r(path, m):
+    """guardarMatrizR (modGlobalMetod.f90:252-277): each row of m as F16.8 fields."""
+    m = _np.asarray(m, _np.float64)
+    if m.ndim == 1:
+        m = m[:, None]
+    mf = _np.asfortranarray(m)
+    _check(lib().lw_dat_write_matrix_r(_os.fsencode(path), int(m.shape[0]), int(m.shape[1]),
+                                       mf.ctypes.data_as(_C.c_void_p)))
+
+
+def write_dat_vector_r(path, v):
+    """guardarVectorR (modGlobalMetod.f90:219-240): one F16.8 per line."""
+    v = _np.ascontiguousarray(v, _np.float64).ravel()
+    _check(lib().lw_dat_write_vector_r(_os.fsencode(path), int(v.size), _P(v)))
+
+
+def write_dat_vector_i(path, v):
+    """guardarVectorI (modGlobalMetod.f90:186-207): one I10 per line."""
+    v = _np.ascontiguousarray(v, _np.int32).ravel()
+    _check(lib().lw_dat_write_vector_i(_os.fsencode(path), int(v.size), _P(v)))
+
+
 def ilu0_csr(ia, ja, a, fp64=False):
     """ilu0C on a caller's CSR matrix (1-based ia/ja, ascending columns with
     the diagonal), factored on the device: (ierr, alu, jlu, ju), alu as
@@ -336,11 +364,15 @@ class Batch:
     FP32 (default) or FP64 (far_fp64=True); near field, ILU0 solves and CGS in
     FP64."""
 
-    def __init__(self, op, nrhs, far_fp64=False):
+    def __init__(self, op, nrhs, far_fp64=False, fp64_residual=False, replacement=0):
         L = lib()
         b = _C.c_void_p()
-        _check(L.lw_batch_create(op.h, int(nrhs), 1 if far_fp64 else 0, _C.byref(b)))
+        flags = (1 if far_fp64 else 0) | (2 if fp64_residual else 0)
+        _check(L.lw_batch_create(op.h, int(nrhs), flags, _C.byref(b)))
         self.far_fp64 = bool(far_fp64)
+        self.fp64_residual = bool(fp64_residual)
+        if fp64_residual and replacement:
+            _check(L.lw_batch_set_replacement(b, int(replacement)))
         self.b = b
         self.op = op
         self.R = int(nrhs)
@@ -350,6 +382,16 @@ class Batch:
         if self.b:
             lib().lw_batch_destroy(self.b)
             self.b = None
+
+    def set_replacement(self, period):
+        """Replace the active columns' recurrence residuals by FP64 true
+        residuals every `period` iterations (fp64_residual batches)."""
+        _check(lib().lw_batch_set_replacement(self.b, int(period)))
+
+    def true_residual_passes(self):
+        f = lib().lw_batch_true_residual_passes
+        f.restype = _C.c_longlong
+        return int(f(self.b))
 
     def __del__(self):
         try:

@@ -49,12 +49,16 @@ def build_hip():
         ("lw_batch.hip", []),
         ("lw_mlp.hip", []),
         ("lw_rcs_static.hip", []),
+        # the two triangular-solve kernels sum rows in the same order: no
+        # contraction, so that they give the same bits
+        ("lw_lusol.hip", ["-ffp-contract=off"]),
         # bit-exact elimination order and roundings of ilu0C: no contraction
         ("lw_ilu0.hip", ["-ffp-contract=off"]),
         # bit-exact Legendre sums: no contraction into FMAs in this unit
         ("lw_setup_kernels.hip", ["-ffp-contract=off"]),
         ("lw_plan.cpp", ["-x", "hip"]),
         ("lw_mesh.cpp", ["-x", "hip"]),
+        ("lw_dat.cpp", ["-x", "hip"]),
     ]
     for src, extra in units:
         obj = os.path.join(BUILD, src + ".o")

@@ -22,6 +22,7 @@
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -1087,6 +1088,75 @@ __global__ void k_b_col_get(int R, int n, int c, const double2* __restrict__ x, 
   if (i < n) xc[i] = x[(size_t)i * R + c];
 }
 
+// ---- true residuals with the FP64 operator (batches whose far field is FP32)
+//
+// A pass computes t = Z64 x for every column (the FP64-far-field operator)
+// and, per column, mode 1 (an active column: its recurrence residual r is
+// replaced by b - t and rho(1) recomputed from it, a reliable update of
+// CGSold's recurrence) or mode 2 (a column that met its tolerance on the
+// recurrence and has not been verified: it stays converged only if its true
+// residual also meets the tolerance; otherwise CGSold starts afresh from its
+// iterate, with R~ = r = b - t).
+__global__ void k_b_tr_mode(int R, int replace, const int4* __restrict__ stv, const int* __restrict__ verified,
+                            int* __restrict__ mode) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= R) return;
+  const int4 v = stv[c];
+  mode[c] = (v.x && replace) ? 1 : (v.z && !verified[c]) ? 2 : 0;
+}
+
+__global__ void k_b_tr_resid(int R, int n, const int* __restrict__ mode, const double2* __restrict__ b,
+                             const double2* __restrict__ t, double2* __restrict__ r) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (long long)n * R) return;
+  if (mode[i % R]) r[i] = dsub(b[i], t[i]);
+}
+
+// nrm = |r|^2, rhon = r.R~ (unconjugated), rrun = r.r (unconjugated)
+__global__ void k_b_tr_state(int R, double errmax, int maxit, const int* __restrict__ mode,
+                             const double2* __restrict__ nrm, const double2* __restrict__ rhon,
+                             const double2* __restrict__ rrun, const double* __restrict__ nbv,
+                             double* __restrict__ rr, int4* __restrict__ stv, int* __restrict__ verified,
+                             double2* __restrict__ rho1, int* __restrict__ restart) {
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= R) return;
+  restart[c] = 0;
+  const int m = mode[c];
+  if (!m) return;
+  const double nr = sqrt(nrm[c].x), nb = nbv[c];
+  rr[c] = rel_res(nr, nb);
+  const bool ok = nr < errmax * nb || nr == 0.0;
+  int4 v = stv[c];
+  if (m == 1) {
+    rho1[c] = rhon[c];
+    if (ok) {
+      v.x = 0;
+      v.z = 1;
+      verified[c] = 1;
+    } else if (rhon[c].x == 0.0 && rhon[c].y == 0.0) {
+      v.x = 0;  // CGSold's breakdown exit at the next iteration
+    }
+  } else if (ok) {
+    verified[c] = 1;
+  } else {
+    v.z = 0;
+    if (v.w < maxit && !(rrun[c].x == 0.0 && rrun[c].y == 0.0)) {
+      v.x = 1;
+      v.y = 1;
+      rho1[c] = rrun[c];
+      restart[c] = 1;
+    }
+  }
+  stv[c] = v;
+}
+
+__global__ void k_b_tr_restart(int R, int n, const int* __restrict__ restart, const double2* __restrict__ r,
+                               double2* __restrict__ rt) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= (long long)n * R) return;
+  if (restart[i % R]) rt[i] = r[i];
+}
+
 // ---------------------------------------------------------------- excitation and RCS per column
 
 __global__ void k_b_plane_wave(int R, int ne, const double* __restrict__ tb, const double* __restrict__ pc,
@@ -1215,6 +1285,15 @@ struct Batch {
   int graph_maxit[2] = {0, 0};
   hipGraphExec_t cur_graph = nullptr;  // the graph of the solve in progress
   double errmax = 0.0;                 // the stopping tolerance of the solve in progress
+  int maxit = 0;                       // and its iteration cap
+  // FP32 far field with FP64 true residuals: the FP64-far-field operator on
+  // the same handle, the replacement period (iterations, 0 = verification
+  // only) and the per-column verification state
+  Batch* ref = nullptr;
+  int ru_period = 0;
+  long long last_ru = 0;
+  int *verified = nullptr, *tr_mode = nullptr, *tr_restart = nullptr;
+  long long tr_passes = 0;
   long long done = 0, next_report = 0;  // iterations replayed since the start
   std::vector<void*> allocs;
 };
@@ -1382,14 +1461,21 @@ static bool far_blocks(const Plan& P, int n, FarBlocks& fb, std::string& msg) {
   return true;
 }
 
-static void free_batch(Batch* b) {
+static void release_batch(Batch* b) {
   if (!b) return;
   if (b->h && b->h->stream) (void)hipStreamSynchronize(b->h->stream);
   for (auto& g : b->iter_graph)
     if (g) (void)hipGraphExecDestroy(g);
   for (void* p : b->allocs) (void)hipFree(p);
-  delete b;
+  b->allocs.clear();
+  if (b->ref) {
+    release_batch(b->ref);
+    delete b->ref;
+    b->ref = nullptr;
+  }
 }
+
+
 
 // bases per LDS chunk of the aggregation kernel (x rows of R columns)
 template <typename S, int R>
@@ -1578,7 +1664,33 @@ static int batch_cgs_start(Batch* bt, double errmax, bool use_prec, int maxit) {
   }
   bt->cur_graph = gx;
   bt->errmax = errmax;
+  bt->maxit = maxit;
   bt->done = 0;
+  bt->last_ru = 0;
+  if (bt->ref) LW_HIP(hipMemsetAsync(bt->verified, 0, sizeof(int) * R, st));
+  return LW_OK;
+}
+
+// One true-residual pass (see k_b_tr_mode); replace: also replace the
+// recurrence residuals of the active columns.
+static int batch_true_residual(Batch* bt, bool replace) {
+  Handle* h = bt->h;
+  const int R = bt->R, n = h->ne;
+  hipStream_t st = h->stream;
+  const int vb = nblk((long long)n * R, 256), cb = nblk(R, 64);
+  double2 *b = bt->w[0], *r = bt->w[1], *R_ = bt->w[2], *t = bt->w[11];
+  double2 *rho1 = bt->sc, *nrm = bt->sc + 4 * R, *rhon = bt->sc + 5 * R, *rrun = bt->sc + 7 * R;
+  int rc = batch_matvec(bt->ref, bt->X, t);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_b_tr_mode, dim3(cb), dim3(64), 0, st, R, replace ? 1 : 0, bt->stv, bt->verified, bt->tr_mode);
+  hipLaunchKernelGGL(k_b_tr_resid, dim3(vb), dim3(256), 0, st, R, n, bt->tr_mode, b, t, r);
+  if ((rc = batch_dot2(bt, r, r, 1, r, R_, 0, nrm, rhon))) return rc;
+  if ((rc = batch_dot2(bt, r, r, 0, nullptr, nullptr, 0, rrun, nullptr))) return rc;
+  hipLaunchKernelGGL(k_b_tr_state, dim3(cb), dim3(64), 0, st, R, bt->errmax, bt->maxit, bt->tr_mode, nrm, rhon, rrun,
+                     bt->nbv, bt->rrv, bt->stv, bt->verified, rho1, bt->tr_restart);
+  hipLaunchKernelGGL(k_b_tr_restart, dim3(vb), dim3(256), 0, st, R, n, bt->tr_restart, r, R_);
+  LW_HIP(hipGetLastError());
+  ++bt->tr_passes;
   return LW_OK;
 }
 
@@ -1604,6 +1716,27 @@ static int batch_cgs_advance(Batch* bt, int iterations, std::vector<int4>& sv, s
     LW_HIP(hipMemcpyAsync(sv.data(), bt->stv, sizeof(int4) * R, hipMemcpyDeviceToHost, st));
     LW_HIP(hipMemcpyAsync(rr.data(), bt->rrv, sizeof(double) * R, hipMemcpyDeviceToHost, st));
     LW_HIP(hipStreamSynchronize(st));
+    if (bt->ref) {
+      // FP32 far field: replace the active columns' residuals every
+      // ru_period iterations, and verify newly converged columns with the
+      // FP64 operator before anyone sees them converged
+      std::vector<int> ver(R);
+      LW_HIP(hipMemcpyAsync(ver.data(), bt->verified, sizeof(int) * R, hipMemcpyDeviceToHost, st));
+      LW_HIP(hipStreamSynchronize(st));
+      bool verify = false, any_active = false;
+      for (int c = 0; c < R; ++c) {
+        verify = verify || (sv[c].z && !ver[c]);
+        any_active = any_active || sv[c].x;
+      }
+      const bool replace = any_active && bt->ru_period > 0 && bt->done - bt->last_ru >= bt->ru_period;
+      if (verify || replace) {
+        if (int rc = batch_true_residual(bt, replace)) return rc;
+        if (replace) bt->last_ru = bt->done;
+        LW_HIP(hipMemcpyAsync(sv.data(), bt->stv, sizeof(int4) * R, hipMemcpyDeviceToHost, st));
+        LW_HIP(hipMemcpyAsync(rr.data(), bt->rrv, sizeof(double) * R, hipMemcpyDeviceToHost, st));
+        LW_HIP(hipStreamSynchronize(st));
+      }
+    }
     int na = 0;
     double worst = 0.0;
     for (int c = 0; c < R; ++c)
@@ -1778,33 +1911,19 @@ struct lw_batch {
   lw::Batch b;
 };
 
-extern "C" {
-
-int lw_batch_create(lw_handle* hh, int nrhs, int flags, lw_batch** out) {
-  using namespace lw;
-  if (!hh || !out || !(nrhs == 64 || nrhs == 128) || (flags & ~LW_BATCH_FAR_FP64)) {
-    set_error("lw_batch_create: nrhs must be 64 or 128; flags: LW_BATCH_FAR_FP64");
-    return LW_ERR_ARG;
-  }
-  *out = nullptr;
-  Handle* h = &hh->h;
-  lw_batch* ob = new lw_batch();
-  Batch* bt = &ob->b;
-  bt->h = h;
-  bt->R = nrhs;
-  bt->fp64 = (flags & LW_BATCH_FAR_FP64) != 0;
-  const int L = h->L, R = nrhs;
+namespace lw {
+// The operator part of a batch (far-field tables in its precision, the near
+// columns of each leaf box, the far-field receive buffer).
+static int batch_operator(Batch* bt) {
+  Handle* h = bt->h;
+  const int L = h->L, R = bt->R;
   const Plan& P = h->plan;
   int rc;
-#define TRY(x)                 \
-  do {                         \
-    rc = (x);                  \
-    if (rc) {                  \
-      free_batch(bt);          \
-      return rc;               \
-    }                          \
+#define TRY(x)        \
+  do {                \
+    rc = (x);         \
+    if (rc) return rc; \
   } while (0)
-  hipStream_t st = h->stream;
   bt->lv.assign(L + 1, BatchLevel());
   TRY(bt->fp64 ? setup_far<double>(bt) : setup_far<float>(bt));
   for (int b = 0; b < P.lv[L].nb; ++b) bt->maxnf = std::max(bt->maxnf, P.box_slot[b + 1] - P.box_slot[b]);
@@ -1820,7 +1939,6 @@ int lw_batch_create(lw_handle* hh, int nrhs, int flags, lw_batch** out) {
         for (int j = 0; same && j < n0; ++j) same = P.ja[P.ia[m] + j] == P.ja[P.ia[m0] + j];
         if (!same) {
           set_error("lw_batch_create: rows of a leaf box have different near columns");
-          free_batch(bt);
           return LW_ERR_STATE;
         }
       }
@@ -1830,11 +1948,57 @@ int lw_batch_create(lw_handle* hh, int nrhs, int flags, lw_batch** out) {
     TRY(upload_b(bt, &bt->ncol_ptr, cp));
     TRY(upload_b(bt, &bt->ncols, cl));
   }
+  TRY(bmalloc(bt, (char**)&bt->Yf, (size_t)h->ne * R * (bt->fp64 ? sizeof(double2) : sizeof(float2))));
+#undef TRY
+  return LW_OK;
+}
+}  // namespace lw
+
+extern "C" {
+
+int lw_batch_create(lw_handle* hh, int nrhs, int flags, lw_batch** out) {
+  using namespace lw;
+  const int known = LW_BATCH_FAR_FP64 | LW_BATCH_FP64_RESIDUAL;
+  if (!hh || !out || !(nrhs == 64 || nrhs == 128) || (flags & ~known) ||
+      ((flags & LW_BATCH_FAR_FP64) && (flags & LW_BATCH_FP64_RESIDUAL))) {
+    set_error("lw_batch_create: nrhs must be 64 or 128; flags: LW_BATCH_FAR_FP64 or LW_BATCH_FP64_RESIDUAL");
+    return LW_ERR_ARG;
+  }
+  *out = nullptr;
+  Handle* h = &hh->h;
+  lw_batch* ob = new lw_batch();
+  Batch* bt = &ob->b;
+  bt->h = h;
+  bt->R = nrhs;
+  bt->fp64 = (flags & LW_BATCH_FAR_FP64) != 0;
+  const int R = nrhs;
+  int rc;
+#define TRY(x)                 \
+  do {                         \
+    rc = (x);                  \
+    if (rc) {                  \
+      release_batch(bt);       \
+      delete ob;               \
+      return rc;               \
+    }                          \
+  } while (0)
+  hipStream_t st = h->stream;
+  TRY(batch_operator(bt));
+  if (flags & LW_BATCH_FP64_RESIDUAL) {
+    // the FP64-far-field operator that judges the FP32 iteration
+    bt->ref = new Batch();
+    bt->ref->h = h;
+    bt->ref->R = R;
+    bt->ref->fp64 = true;
+    TRY(batch_operator(bt->ref));
+    TRY(bmalloc(bt, &bt->verified, (size_t)R));
+    TRY(bmalloc(bt, &bt->tr_mode, (size_t)R));
+    TRY(bmalloc(bt, &bt->tr_restart, (size_t)R));
+  }
   const size_t nv = (size_t)h->ne * R;
   TRY(bmalloc(bt, &bt->X, nv));
   TRY(bmalloc(bt, &bt->Y, nv));
   for (auto& w : bt->w) TRY(bmalloc(bt, &w, nv));
-  TRY(bmalloc(bt, (char**)&bt->Yf, nv * (bt->fp64 ? sizeof(double2) : sizeof(float2))));
   TRY(bmalloc(bt, &bt->part, (size_t)kRedBlocks * R * 4));
   TRY(bmalloc(bt, &bt->sc, (size_t)8 * R));
   TRY(bmalloc(bt, &bt->active, (size_t)R));
@@ -1845,7 +2009,8 @@ int lw_batch_create(lw_handle* hh, int nrhs, int flags, lw_batch** out) {
 #undef TRY
   if (hipStreamSynchronize(st) != hipSuccess || hipGetLastError() != hipSuccess) {
     set_error("lw_batch_create: conversion kernels failed");
-    free_batch(bt);
+    release_batch(bt);
+    delete ob;
     return LW_ERR_HIP;
   }
   *out = ob;
@@ -1854,16 +2019,24 @@ int lw_batch_create(lw_handle* hh, int nrhs, int flags, lw_batch** out) {
 
 int lw_batch_destroy(lw_batch* b) {
   if (b) {
-    lw::Batch* bt = &b->b;
-    if (bt->h && bt->h->stream) (void)hipStreamSynchronize(bt->h->stream);
-    for (auto& g : bt->iter_graph)
-      if (g) (void)hipGraphExecDestroy(g);
-    for (void* p : bt->allocs) (void)hipFree(p);
-    bt->allocs.clear();
+    lw::release_batch(&b->b);
     delete b;
   }
   return LW_OK;
 }
+
+int lw_batch_set_replacement(lw_batch* ob, int period) {
+  using namespace lw;
+  if (!ob || period < 0) return LW_ERR_ARG;
+  if (!ob->b.ref) {
+    set_error("lw_batch_set_replacement: the batch was not created with LW_BATCH_FP64_RESIDUAL");
+    return LW_ERR_STATE;
+  }
+  ob->b.ru_period = period;
+  return LW_OK;
+}
+
+long long lw_batch_true_residual_passes(lw_batch* ob) { return ob ? ob->b.tr_passes : LW_ERR_ARG; }
 
 int lw_batch_nrhs(lw_batch* b) { return b ? b->b.R : LW_ERR_ARG; }
 
@@ -2014,6 +2187,7 @@ int lw_batch_restart_column(lw_batch* ob, int col, const double* b, const double
                      h->d_y, bt->w[0], bt->X, bt->w[1], bt->w[2]);
   hipLaunchKernelGGL(k_b_col_init, dim3(1), dim3(256), 0, st, bt->R, h->ne, col, bt->errmax, h->d_w[0], h->d_y,
                      bt->nbv, bt->rrv, bt->stv, bt->sc);
+  if (bt->ref) LW_HIP(hipMemsetAsync(bt->verified + col, 0, sizeof(int), st));
   LW_HIP(hipGetLastError());
   LW_HIP(hipStreamSynchronize(st));
   return LW_OK;

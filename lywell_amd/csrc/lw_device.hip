@@ -75,43 +75,60 @@ __device__ __forceinline__ double wave_sum(double v) {
 // lists overlap, then share one XCD's L2.
 
 // Leaf aggregation (modMLFMA.f90:1805-1812): campoC_L(:,:,box) =
-// sum_f funcRAD(:,:,f) x(f).  One workgroup per leaf box.
-__global__ void __launch_bounds__(256) k_aggregate(int nk2, const int* __restrict__ box_slot,
-                                                   const int* __restrict__ slot_basis,
-                                                   const double2* __restrict__ rad,
-                                                   const double2* __restrict__ x, double2* __restrict__ C) {
+// sum_f funcRAD(:,:,f) x(f), each box's bases in box order.  The patterns are
+// assembled from phase rows: one exp(j ka khat . (c_T - C)) row per (leaf
+// box, triangle of one of its bases), 0.44 of the bytes of the pattern array
+// at 122,880 unknowns, shared by the box's bases through L2, combined with
+// each basis's RWG vectors by rad_from_phase (the function that tabulates the
+// patterns for the receive, so the values are the same bits).  One workgroup
+// per leaf box, one thread per sample (both polarisations).
+__global__ void __launch_bounds__(256) k_aggregate_ph(int nk, const int* __restrict__ box_slot,
+                                                      const int* __restrict__ slot_basis,
+                                                      const int2* __restrict__ slot_pair,
+                                                      const RadGeom* __restrict__ geo,
+                                                      const double* __restrict__ ath,
+                                                      const double* __restrict__ aph,
+                                                      const double2* __restrict__ tph,
+                                                      const double2* __restrict__ x, double2* __restrict__ C) {
   const int b = xcd_block(blockIdx.x, gridDim.x);
   const int s0 = box_slot[b], s1 = box_slot[b + 1];
-  // two samples per thread per pass (the second one clamped, so that the
-  // loop has no branch), and the bases in groups of kAggU whose pattern
-  // loads are all issued before their FMAs; each sample accumulates its
-  // bases in box order
-  constexpr int kAggU = 8;
-  for (int e = threadIdx.x; e < nk2; e += 2 * blockDim.x) {
-    const int e2 = e + blockDim.x, e2l = min(e2, nk2 - 1);
-    double2 acc = make_double2(0.0, 0.0), acc2 = make_double2(0.0, 0.0);
-    int s = s0;
-    for (; s + kAggU <= s1; s += kAggU) {
-      double2 xv[kAggU], r1[kAggU], r2[kAggU];
+  constexpr int kU = 4;  // bases whose phase loads are issued together
+  for (int k = threadIdx.x; k < nk; k += blockDim.x) {
+    double at[3], ap[3];
 #pragma unroll
-      for (int u = 0; u < kAggU; ++u) {
+    for (int d = 0; d < 3; ++d) {
+      at[d] = ath[3 * k + d];
+      ap[d] = aph[3 * k + d];
+    }
+    double2 acc0 = make_double2(0.0, 0.0), acc1 = make_double2(0.0, 0.0);
+    int s = s0;
+    for (; s + kU <= s1; s += kU) {
+      double2 pp[kU], pm[kU], xv[kU];
+#pragma unroll
+      for (int u = 0; u < kU; ++u) {
+        const int2 pr = slot_pair[s + u];
+        pp[u] = tph[(size_t)pr.x * nk + k];
+        pm[u] = tph[(size_t)pr.y * nk + k];
         xv[u] = x[slot_basis[s + u]];
-        r1[u] = rad[(size_t)(s + u) * nk2 + e];
-        r2[u] = rad[(size_t)(s + u) * nk2 + e2l];
       }
 #pragma unroll
-      for (int u = 0; u < kAggU; ++u) {
-        acc = cadd(acc, cmul(r1[u], xv[u]));
-        acc2 = cadd(acc2, cmul(r2[u], xv[u]));
+      for (int u = 0; u < kU; ++u) {
+        double2 r0, r1;
+        rad_from_phase(geo[s + u], pp[u], pm[u], at, ap, r0, r1);
+        acc0 = cadd(acc0, cmul(r0, xv[u]));
+        acc1 = cadd(acc1, cmul(r1, xv[u]));
       }
     }
     for (; s < s1; ++s) {
+      const int2 pr = slot_pair[s];
+      double2 r0, r1;
+      rad_from_phase(geo[s], tph[(size_t)pr.x * nk + k], tph[(size_t)pr.y * nk + k], at, ap, r0, r1);
       const double2 xv = x[slot_basis[s]];
-      acc = cadd(acc, cmul(rad[(size_t)s * nk2 + e], xv));
-      acc2 = cadd(acc2, cmul(rad[(size_t)s * nk2 + e2l], xv));
+      acc0 = cadd(acc0, cmul(r0, xv));
+      acc1 = cadd(acc1, cmul(r1, xv));
     }
-    C[(size_t)b * nk2 + e] = acc;
-    if (e2 < nk2) C[(size_t)b * nk2 + e2] = acc2;
+    C[((size_t)b * 2 + 0) * nk + k] = acc0;
+    C[((size_t)b * 2 + 1) * nk + k] = acc1;
   }
 }
 
@@ -310,48 +327,6 @@ __global__ void __launch_bounds__(256) k_copy(long long n, const double2* __rest
 }
 
 // ---------------------------------------------------------------- ILU0 solves
-
-// lusol0C (modIterativo.f90:504-560): forward solve with the unit-diagonal L
-// then backward solve with U and the inverted diagonal, rows taken level by
-// level (the level sets of the reference's natural order).  A single
-// workgroup runs every level; the barrier between levels orders the global
-// writes of x for the next level's reads (all waves share one CU's L1).
-template <typename V>
-__global__ void __launch_bounds__(1024) k_lusol(const double2* __restrict__ y, double2* __restrict__ x,
-                                                const V* __restrict__ alu, const int* __restrict__ jlu,
-                                                const int* __restrict__ ju, const int* __restrict__ lptr,
-                                                const int* __restrict__ lrows, int nlev_l,
-                                                const int* __restrict__ uptr, const int* __restrict__ urows,
-                                                int nlev_u) {
-  const int wave = threadIdx.x / kWave, lane = threadIdx.x % kWave, nw = blockDim.x / kWave;
-  // alu(k), jlu(k), ju(i) are 1-based as in the reference: arrays carry a pad
-  for (int l = 0; l < nlev_l; ++l) {
-    for (int q = lptr[l] + wave; q < lptr[l + 1]; q += nw) {
-      const int i = lrows[q] + 1;
-      double2 s = make_double2(0.0, 0.0);
-      for (int k = jlu[i] + lane; k <= ju[i] - 1; k += kWave) s = cadd(s, cmul(to_d2(alu[k]), x[jlu[k] - 1]));
-      s = wave_sum(s);
-      if (lane == 0) {
-        const double2 yi = y[i - 1];
-        x[i - 1] = make_double2(yi.x - s.x, yi.y - s.y);
-      }
-    }
-    __syncthreads();
-  }
-  for (int l = 0; l < nlev_u; ++l) {
-    for (int q = uptr[l] + wave; q < uptr[l + 1]; q += nw) {
-      const int i = urows[q] + 1;
-      double2 s = make_double2(0.0, 0.0);
-      for (int k = ju[i] + lane; k <= jlu[i + 1] - 1; k += kWave) s = cadd(s, cmul(to_d2(alu[k]), x[jlu[k] - 1]));
-      s = wave_sum(s);
-      if (lane == 0) {
-        const double2 xi = x[i - 1];
-        x[i - 1] = cmul(to_d2(alu[i]), make_double2(xi.x - s.x, xi.y - s.y));
-      }
-    }
-    __syncthreads();
-  }
-}
 
 // ---------------------------------------------------------------- CGS pieces
 
@@ -569,8 +544,8 @@ int run_matvec(Handle* h, const double2* x, double2* y, double2* y_near, hipEven
   hipStream_t st = h->stream;
   DevLevel& lf = h->lv[L];
   if (ev) LW_HIP(hipEventRecord(ev[0], st));
-  hipLaunchKernelGGL(k_aggregate, dim3(lf.nb), dim3(256), 0, st, 2 * lf.nk, h->d_box_slot, h->d_slot_basis,
-                     h->d_rad, x, lf.C);
+  hipLaunchKernelGGL(k_aggregate_ph, dim3(lf.nb), dim3(256), 0, st, lf.nk, h->d_box_slot, h->d_slot_basis,
+                     h->d_slot_pair, h->d_rgeo, h->d_ath, h->d_aph, h->d_tph, x, lf.C);
   if (ev) LW_HIP(hipEventRecord(ev[1], st));
   for (int n = L; n >= 4; --n) {
     DevLevel& c = h->lv[n];
@@ -616,14 +591,7 @@ int run_lusol(Handle* h, const double2* y, double2* x) {
     set_error("lusol0: no ILU0 factor (call lw_ilu0 first)");
     return LW_ERR_STATE;
   }
-  if (h->fp64_near)
-    hipLaunchKernelGGL(k_lusol<double2>, dim3(1), dim3(1024), 0, h->stream, y, x, (const double2*)h->d_alu,
-                       h->d_jlu, h->d_ju, h->d_lptr, h->d_lrows, h->nlev_l, h->d_uptr, h->d_urows, h->nlev_u);
-  else
-    hipLaunchKernelGGL(k_lusol<float2>, dim3(1), dim3(1024), 0, h->stream, y, x, (const float2*)h->d_alu,
-                       h->d_jlu, h->d_ju, h->d_lptr, h->d_lrows, h->nlev_l, h->d_uptr, h->d_urows, h->nlev_u);
-  LW_HIP(hipGetLastError());
-  return LW_OK;
+  return launch_lusol(h, y, x);
 }
 
 // dot products: out[0] = sum(a*b) (conj a if ca), out[1] = sum(a2*b2)
@@ -634,6 +602,23 @@ int run_dot2(Handle* h, const double2* a, const double2* b, int ca, const double
   LW_HIP(hipGetLastError());
   return LW_OK;
 }
+
+}  // namespace
+
+// LW_ERR_STATE if a dataflow triangular solve on this handle gave up a wait
+// (it then finished with wrong values; it cannot happen with a factor whose
+// level sets were built here); the flag is cleared.
+int lusol_status(Handle* h) {
+  int e = 0;
+  LW_HIP(hipMemcpyAsync(&e, h->d_lusol_err, sizeof(int), hipMemcpyDeviceToHost, h->stream));
+  LW_HIP(hipStreamSynchronize(h->stream));
+  if (!e) return LW_OK;
+  LW_HIP(hipMemsetAsync(h->d_lusol_err, 0, sizeof(int), h->stream));
+  set_error("lusol0: a triangular-solve wait reached its limit");
+  return LW_ERR_STATE;
+}
+
+namespace {
 
 int check_handle(lw_handle* hh) {
   if (!hh) {
@@ -777,6 +762,7 @@ static int init_device_state(Handle* h, const Geometry& g) {
   TRY(upload(h, &h->d_box_slot, P.box_slot));
   TRY(dmalloc(h, &h->d_rad, (size_t)P.lv[L].nk * 2 * P.ne));
   TRY(launch_setup_rad(h));
+  TRY(launch_setup_phases(h, g.e_t));
   // near field
   TRY(upload(h, &h->d_ia, P.ia));
   TRY(upload(h, &h->d_ja, P.ja));
@@ -790,6 +776,7 @@ static int init_device_state(Handle* h, const Geometry& g) {
     h->d_z = z;
   }
   TRY(launch_setup_near(h));
+
   // work vectors and reduction buffers
   TRY(dmalloc(h, &h->d_x, (size_t)P.ne));
   TRY(dmalloc(h, &h->d_y, (size_t)P.ne));
@@ -800,6 +787,8 @@ static int init_device_state(Handle* h, const Geometry& g) {
   TRY(dmalloc(h, &h->d_cur, (size_t)P.ne));
   rcs_far_zone(g.e_centro, P.ne, P.phys.w, h->rcs_rzl, h->rcs_centre);
   TRY(rcs_static_setup(h));
+  TRY(dmalloc(h, &h->d_lusol_err, 1));
+  LW_HIP(hipMemsetAsync(h->d_lusol_err, 0, sizeof(int), h->stream));
   h->scal_cap = 64;
   TRY(dmalloc(h, &h->d_scal, (size_t)h->scal_cap));
   LW_HIP(hipStreamSynchronize(h->stream));
@@ -1036,6 +1025,7 @@ int upload_structure(Handle* h) {
   lw::tri_levels(n, h->h_jlu, h->h_ju, true, uptr, urows);
   h->nlev_l = (int)lptr.size() - 1;
   h->nlev_u = (int)uptr.size() - 1;
+  if (int rc = lw::lusol_flow_setup(h)) return rc;
   if (int rc = lw::upload(h, &h->d_lptr, lptr)) return rc;
   if (int rc = lw::upload(h, &h->d_lrows, lrows)) return rc;
   if (int rc = lw::upload(h, &h->d_uptr, uptr)) return rc;
@@ -1133,7 +1123,7 @@ int lw_lusol0(lw_handle* hh, const double* y, double* x) {
   if (rc) return rc;
   LW_HIP(hipMemcpyAsync(x, h->d_y, bytes, hipMemcpyDeviceToHost, h->stream));
   LW_HIP(hipStreamSynchronize(h->stream));
-  return LW_OK;
+  return lw::lusol_status(h);
 }
 
 int lw_cgs(lw_handle* hh, const double* b, const double* guess, double errmax, int use_prec, int maxit, double* x,
@@ -1222,6 +1212,7 @@ int lw_cgs(lw_handle* hh, const double* b, const double* guess, double errmax, i
   }
   LW_HIP(hipMemcpyAsync(x, xv, bytes, hipMemcpyDeviceToHost, st));
   LW_HIP(hipStreamSynchronize(st));
+  if (use_prec && (rc = lw::lusol_status(h))) return rc;
   if (iters) *iters = it;
   if (relres) *relres = rr;
   if (converged) *converged = conv ? 1 : 0;
@@ -1424,9 +1415,12 @@ int lw_matvec_bytes(lw_handle* hh, double* total, double* aggregation, double* r
   const double zb = h->fp64_near ? 16.0 : 8.0;
   const double near = (zb + 4.0) * (double)P.nnz + 4.0 * (P.ne + 1);
   const double xy = 2.0 * P.ne * c16;
-  if (total) *total = 2 * pattern + 6 * campo + trans + near + xy;
-  // aggregation: reads the patterns and x once, writes campoC at the leaf
-  if (aggregation) *aggregation = pattern + P.ne * c16 + (double)P.lv[P.L].nk * 2 * P.lv[P.L].nb * c16;
+  // the aggregation reads the phase rows (one per leaf box and triangle of
+  // its bases) and the bases' RWG vectors instead of the pattern array
+  const double phases = (double)h->npairs * P.lv[P.L].nk * c16 + (double)P.ne * sizeof(lw::RadGeom);
+  if (total) *total = pattern + phases + 6 * campo + trans + near + xy;
+  // aggregation: reads the phase rows, RWG vectors and x once, writes campoC at the leaf
+  if (aggregation) *aggregation = phases + P.ne * c16 + (double)P.lv[P.L].nk * 2 * P.lv[P.L].nb * c16;
   // receive + near: patterns, leaf campoA, near CSR, x gathered once, y written
   if (receive_near)
     *receive_near = pattern + (double)P.lv[P.L].nk * 2 * P.lv[P.L].nb * c16 + near + 2.0 * P.ne * c16;
@@ -1440,8 +1434,9 @@ int lw_matvec_stage_bytes(lw_handle* hh, double* sb) {
   const int L = P.L;
   auto field = [&](int n) { return (double)P.lv[n].nk * 2 * P.lv[n].nb * 16.0; };
   const double pattern = (double)P.lv[L].nk * 2 * P.ne * 16.0;
+  const double phases = (double)h->npairs * P.lv[L].nk * 16.0 + (double)P.ne * sizeof(lw::RadGeom);
   const double zb = h->fp64_near ? 16.0 : 8.0;
-  sb[0] = pattern + 16.0 * P.ne + field(L) + 8.0 * P.ne;
+  sb[0] = phases + 16.0 * P.ne + field(L) + 16.0 * P.ne;
   sb[1] = 0.0;
   for (int n = L; n >= 4; --n) sb[1] += field(n) + field(n - 1) + 12.0 * 9 * P.lv[n - 1].nk + 16.0 * 8 * P.lv[n - 1].nk;
   sb[2] = 0.0;

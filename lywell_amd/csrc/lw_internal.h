@@ -26,21 +26,34 @@ struct RadGeom {
 // reference's non-maxPrecision formula with its operation order; used both
 // to tabulate the patterns and to regenerate them inside the matvec, with
 // contraction off so that both give the same bits.
-__device__ inline void rad_eval(const RadGeom& g, const double* __restrict__ vk, const double* __restrict__ at,
-                                const double* __restrict__ ap, double ka, double2& r0, double2& r1) {
+// The phase exp(j ka khat . d) of one triangle centroid offset d from the
+// leaf-box centre, as (cos, sin), with calcRad's operation order.
+__device__ inline double2 rad_phase(const double* __restrict__ vk, const double* __restrict__ d, double ka) {
 #pragma clang fp contract(off)
-  double sp, cp, sm, cm;
-  sincos(ka * ((vk[0] * g.dp[0] + vk[1] * g.dp[1]) + vk[2] * g.dp[2]), &sp, &cp);
-  sincos(ka * ((vk[0] * g.dm[0] + vk[1] * g.dm[1]) + vk[2] * g.dm[2]), &sm, &cm);
+  double s, c;
+  sincos(ka * ((vk[0] * d[0] + vk[1] * d[1]) + vk[2] * d[2]), &s, &c);
+  return make_double2(c, s);
+}
+// funcRAD of one basis at one sample from the phases of its two triangles
+// (pp = plus triangle, pm = minus triangle): both the tabulated patterns and
+// the patterns assembled inside the matvec from per-box triangle phases go
+// through this function, so both give the same bits.
+__device__ inline void rad_from_phase(const RadGeom& g, double2 pp, double2 pm, const double* __restrict__ at,
+                                      const double* __restrict__ ap, double2& r0, double2& r1) {
+#pragma clang fp contract(off)
   double vx[3], vy[3];
   for (int d = 0; d < 3; ++d) {
-    vx[d] = ((cp * g.rp[d] + cm * g.rm[d]) * g.len) * 0.5;
-    vy[d] = ((sp * g.rp[d] + sm * g.rm[d]) * g.len) * 0.5;
+    vx[d] = ((pp.x * g.rp[d] + pm.x * g.rm[d]) * g.len) * 0.5;
+    vy[d] = ((pp.y * g.rp[d] + pm.y * g.rm[d]) * g.len) * 0.5;
   }
   r0.x = (vx[0] * at[0] + vx[1] * at[1]) + vx[2] * at[2];
   r0.y = (vy[0] * at[0] + vy[1] * at[1]) + vy[2] * at[2];
   r1.x = (vx[0] * ap[0] + vx[1] * ap[1]) + vx[2] * ap[2];
   r1.y = (vy[0] * ap[0] + vy[1] * ap[1]) + vy[2] * ap[2];
+}
+__device__ inline void rad_eval(const RadGeom& g, const double* __restrict__ vk, const double* __restrict__ at,
+                                const double* __restrict__ ap, double ka, double2& r0, double2& r1) {
+  rad_from_phase(g, rad_phase(vk, g.dp, ka), rad_phase(vk, g.dm, ka), at, ap, r0, r1);
 }
 
 // Workgroup id w of n -> logical block: the hardware deals workgroups to the
@@ -122,6 +135,12 @@ struct Handle {
   // leaf patterns and ordering
   double2* d_rad = nullptr;  // (nk_L, 2, ne) in slot order
   RadGeom* d_rgeo = nullptr;  // (ne) per-basis pattern geometry, slot order
+  // the patterns as the matvec uses them: one phase exp(j ka khat . (c_T -
+  // C)) per (leaf box, triangle of one of its bases) and leaf sample, and per
+  // basis slot the phase rows of its plus and minus triangles
+  int npairs = 0;
+  double2* d_tph = nullptr;   // (npairs, nk_L) phase rows
+  int2* d_slot_pair = nullptr; // (ne) phase rows of each slot's two triangles
   double *d_ath = nullptr, *d_aph = nullptr;  // (3, nk_L) theta and phi unit vectors of the leaf samples
   int *d_slot_basis = nullptr, *d_basis_slot = nullptr, *d_basis_box = nullptr, *d_box_slot = nullptr;
   // near field CSR (natural order)
@@ -133,6 +152,8 @@ struct Handle {
   int *d_jlu = nullptr, *d_ju = nullptr;
   int *d_lptr = nullptr, *d_lrows = nullptr, *d_uptr = nullptr, *d_urows = nullptr;
   int nlev_l = 0, nlev_u = 0;
+  int* d_lusol_err = nullptr;  // set when a triangular solve's wait reached its limit
+  bool lusol_flow = false;      // the dataflow triangular solve fits (lw_lusol.hip)
   std::vector<float> h_alu32;
   std::vector<double> h_alu64;
   std::vector<int> h_jlu, h_ju;
@@ -184,6 +205,7 @@ int upload(Handle* h, T** p, const std::vector<T>& v) {
 
 // setup kernels (lw_setup_kernels.hip)
 int launch_setup_rad(Handle* h);
+int launch_setup_phases(Handle* h, const int* e_t);
 int launch_setup_transfer(Handle* h, int level, const double* d_binom2, const double* d_tu,
                           const int* d_tL, const double* d_th, int hstride);
 int launch_setup_near(Handle* h);
@@ -191,6 +213,12 @@ int launch_setup_near(Handle* h);
 // The MSR factor's structure (and value array) on the device, and the level
 // sets of both triangular sweeps (lw_device.hip).
 int upload_structure(Handle* h);
+// lusol0C on the device (lw_lusol.hip): the solve of y into x with the
+// handle's factor on its stream, and the set-up of its LDS request.
+int launch_lusol(Handle* h, const double2* y, double2* x);
+int lusol_flow_setup(Handle* h);
+// LW_ERR_STATE (and clears the flag) if a dataflow solve gave up a wait.
+int lusol_status(Handle* h);
 // ILU(0) of the near field on the device (lw_ilu0.hip); *piv = 0 or the first
 // row with a zero pivot.
 int ilu0_device(Handle* h, int* piv);

@@ -55,6 +55,22 @@ __global__ void k_rad_geom(int ne, const int* __restrict__ slot_basis, const int
   g[s] = r;
 }
 
+// The phase rows of the matvec's pattern form: row p = exp(j ka khat_k .
+// (c_T - C_b)) for the triangle T and leaf box b of pair p, at every leaf
+// sample k (the factors calcRad multiplies, with its operation order).
+__global__ void k_tri_phase(int npairs, int nk, const int* __restrict__ pair_tri, const int* __restrict__ pair_box,
+                            const double* __restrict__ box_c, const double* __restrict__ tb,
+                            const double* __restrict__ kap, double ka, double2* __restrict__ tph) {
+  const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= (long long)npairs * nk) return;
+  const int p = (int)(idx / nk), k = (int)(idx % nk);
+  const double* t = tb + 3 * pair_tri[p];
+  const double* c = box_c + 3 * pair_box[p];
+  double d[3];
+  for (int q = 0; q < 3; ++q) d[q] = t[q] - c[q];
+  tph[idx] = rad_phase(kap + 3 * k, d, ka);
+}
+
 // calcRad, non-maxPrecision branch: radiation pattern of basis slot s about
 // its leaf-box centre, theta and phi components.
 __global__ void k_rad(int ne, int nk, const double* __restrict__ kap, const double* __restrict__ ath,
@@ -199,6 +215,50 @@ int launch_setup_rad(Handle* h) {
   const int blk = 256;
   hipLaunchKernelGGL(k_rad, dim3((unsigned)((tot + blk - 1) / blk)), dim3(blk), 0, h->stream, h->ne, lf.nk,
                      d.kap, h->d_ath, h->d_aph, h->d_rgeo, (2.0 * kPi) / h->plan.phys.lambda, h->d_rad);
+  LW_HIP(hipGetLastError());
+  return LW_OK;
+}
+
+// The per-(leaf box, triangle) phase rows and each slot's two rows: the
+// triangles of a box's bases in first-seen slot order.
+int launch_setup_phases(Handle* h, const int* e_t) {
+  const Plan& P = h->plan;
+  const LevelPlan& lf = P.lv[h->L];
+  std::vector<int> pair_tri, pair_box;
+  std::vector<int2> slot_pair(P.ne);
+  for (int b = 0; b < lf.nb; ++b) {
+    const int first = (int)pair_tri.size();
+    for (int s = P.box_slot[b]; s < P.box_slot[b + 1]; ++s) {
+      const int f = P.slot_basis[s];
+      int rows[2];
+      for (int side = 0; side < 2; ++side) {
+        const int t = e_t[2 * f + side] - 1;
+        int r = -1;
+        for (int q = first; q < (int)pair_tri.size(); ++q)
+          if (pair_tri[q] == t) {
+            r = q;
+            break;
+          }
+        if (r < 0) {
+          r = (int)pair_tri.size();
+          pair_tri.push_back(t);
+          pair_box.push_back(b);
+        }
+        rows[side] = r;
+      }
+      slot_pair[s] = make_int2(rows[0], rows[1]);
+    }
+  }
+  h->npairs = (int)pair_tri.size();
+  int *d_pt = nullptr, *d_pb = nullptr;
+  int rc;
+  if ((rc = upload(h, &d_pt, pair_tri)) || (rc = upload(h, &d_pb, pair_box)) ||
+      (rc = upload(h, &h->d_slot_pair, slot_pair)) || (rc = dmalloc(h, &h->d_tph, (size_t)h->npairs * lf.nk)))
+    return rc;
+  const long long tot = (long long)h->npairs * lf.nk;
+  hipLaunchKernelGGL(k_tri_phase, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, h->stream, h->npairs, lf.nk,
+                     d_pt, d_pb, h->lv[h->L].centre, h->d_tb, h->lv[h->L].kap, (2.0 * kPi) / h->plan.phys.lambda,
+                     h->d_tph);
   LW_HIP(hipGetLastError());
   return LW_OK;
 }

@@ -45,4 +45,6 @@ done
 "$FC" -O2 -o "$OUT/ref_driver" "$TMP/ref_driver.o" $objs
 "$FC" -O2 -I"$TMP" -c "$HERE/ref_stl.f90" -o "$TMP/ref_stl.o"
 "$FC" -O2 -o "$OUT/ref_stl" "$TMP/ref_stl.o" $objs
-echo "built $OUT/ref_driver $OUT/ref_stl"
+"$FC" -O2 -I"$TMP" -c "$HERE/ref_dat.f90" -o "$TMP/ref_dat.o"
+"$FC" -O2 -o "$OUT/ref_dat" "$TMP/ref_dat.o" $objs
+echo "built $OUT/ref_driver $OUT/ref_stl $OUT/ref_dat"

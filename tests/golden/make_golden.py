@@ -18,6 +18,10 @@ that holds the reference at /root/reference or $LYWELL_REFERENCE).
      MatxVec_MLFMA of the deterministic vector and of the plane wave, a sample
      of the near field and of the ILU0 factor, lusol0C, and CGSold + sigmaRCS
      with and without ILU0 (matvec counts included).
+4. dat_writers.npz: the text the reference's .dat writers produce (flang
+   build, oracle/ref_dat.f90) for values that probe the F16.8 and I10 fields,
+   and samples of .dat files the reference's own runs wrote.
+Run with no argument for everything, or name parts: files stl build dat.
 """
 import os
 import subprocess
@@ -172,8 +176,59 @@ def stl_fixture():
     np.savez_compressed(os.path.join(HERE, "stl_sphere_ref.npz"), p=p, t=t, n=n)
 
 
+def dat_writer_inputs():
+    """Values that probe F16.8 and I10: signed zeros, negative values that round
+    to zero, exact decimal ties, the field-width limits, NaN and infinities,
+    then random values over 16 decades."""
+    rng = np.random.default_rng(0)
+    special = [0.0, -0.0, 1e-10, -1e-10, -4.9e-9, -5.1e-9, 4.9999999e-9, 5e-9, 1 / 512, -1 / 512, 3 / 1024, 0.5, -0.5,
+               1.0, -1.0, 9999999.0, 9999999.999999994, 9999999.999999996, -999999.99999999, -9999999.0, 12345678.9,
+               1e300, -1e300, 123.456789125, 2.0 ** -27, 0.125000005, np.pi, -np.e, np.nan, np.inf, -np.inf, 2.5e-9]
+    vals = np.concatenate([special, rng.standard_normal(200) * 10.0 ** rng.integers(-9, 7, 200),
+                           rng.uniform(-1, 1, 60) * 1e-8])
+    n = len(vals) - len(vals) % 6
+    vals = vals[:n].astype(np.float64)
+    iv = np.concatenate([[0, -1, 1, 2 ** 31 - 1, -2 ** 31, 999999999, -999999999, 1234567890, -123456789],
+                         rng.integers(-10 ** 9, 10 ** 9, n - 9)]).astype(np.int32)
+    return vals, iv
+
+
+def dat_writer_fixture():
+    """What the reference's guardarVectorC / guardarMatrizR / guardarVectorR /
+    guardarVectorI (modGlobalMetod.f90:186-306, flang build) write for
+    dat_writer_inputs(), and samples of the .dat files the reference's own
+    (ifort) runs wrote: a current (conkind8_IMLFMA.dat, guardarVectorC) and
+    monostatic tables (guardarMatrizR)."""
+    vals, iv = dat_writer_inputs()
+    texts = {}
+    with tempfile.TemporaryDirectory() as td:
+        with open(os.path.join(td, "in.bin"), "wb") as f:
+            np.array([vals.size], np.int32).tofile(f)
+            vals.tofile(f)
+            iv.tofile(f)
+        subprocess.run([os.path.join(ROOT, "oracle", "_ref", "ref_dat"), "in.bin", "out"], cwd=td, check=True)
+        for k in ("vc", "mr", "vr", "vi"):
+            with open(os.path.join(td, f"out_{k}.dat")) as f:
+                texts["flang_" + k] = np.array(f.read())
+    for name, key, lines in (("conkind8_IMLFMA.dat", "ifort_current", 3000),
+                             ("pruebaVH_RCSMONOVV.dat", "ifort_mono_vv", None),
+                             ("last_rcs_mono_incremental.dat", "ifort_mono_incremental", None)):
+        with open(os.path.join(REL, name)) as f:
+            txt = f.read()
+        if lines:
+            txt = "".join(txt.splitlines(keepends=True)[:lines])
+        texts[key] = np.array(txt)
+    np.savez_compressed(os.path.join(HERE, "dat_writers.npz"), vals=vals, iv=iv, **texts)
+
+
 if __name__ == "__main__":
-    fixtures_from_reference_files()
-    stl_fixture()
-    golden_from_reference_build()
+    only = sys.argv[1:]
+    if not only or "files" in only:
+        fixtures_from_reference_files()
+    if not only or "stl" in only:
+        stl_fixture()
+    if not only or "build" in only:
+        golden_from_reference_build()
+    if not only or "dat" in only:
+        dat_writer_fixture()
     print("fixtures written to", HERE)

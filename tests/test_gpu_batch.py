@@ -236,3 +236,34 @@ def test_persistent_sweep_over_the_circle_wraps():
     sw = Sweep(op, bt, errmax=1e-4, maxit=3000, guess="extrap")
     nb = sw._neighbours(10)
     assert nb(0, -1) is None and nb(9, 1) is None
+
+
+def test_fp32_far_field_with_fp64_true_residuals():
+    # LW_BATCH_FP64_RESIDUAL: the FP32-far-field iteration is judged by the
+    # FP64 operator; a column is reported converged only if its true FP64
+    # residual meets errmax, and periodic replacement keeps the recurrence
+    # residual tied to it
+    mesh = lw.Mesh.icosphere(3, 0.5)
+    op = lw.MLFMA(mesh, lam=1.0, ladomax=0.2)
+    try:
+        ph = np.linspace(0, np.pi, 64, endpoint=False)
+        dirs = np.stack([-np.cos(ph), -np.sin(ph), np.zeros(64)], axis=1)
+        pols = np.tile([0.0, 0.0, 1.0], (64, 1)).astype(complex)
+        for period in (0, 40):
+            bt = lw.Batch(op, 64, fp64_residual=True, replacement=period)
+            try:
+                B = bt.plane_wave(dirs, pols)
+                X, it, rr, cv = bt.cgs(B, errmax=1e-4, use_prec=False, maxit=3000)
+                true = np.array([rel(op.matvec(X[:, c]), B[:, c]) for c in range(64)])
+                print(f"[fp32 + fp64 residual, period {period}] converged {cv.sum()}/64, iterations median "
+                      f"{int(np.median(it))} max {it.max()}, true residual max {true[cv].max():.2e}, "
+                      f"passes {bt.true_residual_passes()}")
+                assert cv.sum() >= 60
+                assert (true[cv] < 1e-4).all()
+                assert bt.true_residual_passes() > 0
+                # the reported relres of a converged column is its true FP64 residual
+                assert np.allclose(rr[cv], true[cv], rtol=1e-6)
+            finally:
+                bt.close()
+    finally:
+        op.close()

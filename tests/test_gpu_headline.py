@@ -36,10 +36,6 @@ _cache = {}
 
 def setup_case(name):
     if name not in _cache:
-        for other in list(_cache):  # one full-size case in memory at a time
-            _, g, o = _cache.pop(other)
-            g.close()
-            o.close()
         sub, ax, lad = CASES[name]
         t0 = time.time()
         print(f"[case {name}] building mesh and operators", flush=True)
