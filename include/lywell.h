@@ -314,6 +314,30 @@ int lw_guess_add_samples(lw_mlp *m, lw_handle *h, int ncol, const double *dir, c
                          const double *xtrue);
 long long lw_guess_nsamples(lw_mlp *m);
 int lw_guess_train(lw_mlp *m, int steps, int batch_rows, float lr, unsigned long long seed, float *loss);
+/* A trained network from its file: an uncompressed .npz holding arch (int64
+ * [in, hidden, depth, out]) and params (float32, the layout above), as
+ * lywell_amd/guess_train.py writes it (lywell_amd/guess_mlp.npz). */
+int lw_guess_load(const char *path, lw_mlp **out);
+
+/* ---- monostatic sweep ------------------------------------------------------
+ * gen_rcs_monostatica (main.f90:967-1089) in one call: n incidence angles
+ * phi_a = a (span / n) (a = 0..n-1; the reference's span is pi) in the
+ * theta = 90 deg plane, propagation (-cos phi, -sin phi, 0), polarisation as
+ * ajustar_polariz sets it (main.f90:828-849: VV = z, HH = d x z / |d x z|),
+ * each solved by CGSold (no preconditioner) to errmax and sigma[a] =
+ * sigmaRCS(pi/2, phi_a) (main.f90:1048-1050; linear, as sigmaRCS returns it).
+ * The angles run as chains of consecutive angles through the batch b (created
+ * on h), each column restarted as soon as it stops, from the learned guess of
+ * two solved neighbours when net is given, else the reference's warm start
+ * from one (main.f90:1017-1022), else zero; a span of 2 pi wraps around.
+ * iterations, converged (per angle), currents (complex(8), (num_e, n) in
+ * Fortran order) and stats (restarts, steals, angles started from zero / one /
+ * two solved neighbours) are optional. */
+#define LW_POL_VV 0
+#define LW_POL_HH 1
+int lw_sweep_monostatic(lw_handle *h, lw_batch *b, lw_mlp *net, int n, double span, int pol, const double *ctte,
+                        double errmax, int maxit, double *sigma, int *iterations, int *converged, double *currents,
+                        int *stats);
 
 #ifdef __cplusplus
 }

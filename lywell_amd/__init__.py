@@ -46,6 +46,8 @@ def lib():
         L = _C.CDLL(LIB_PATH)
         L.lw_last_error.restype = _C.c_char_p
         L.lw_near_nnz.restype = _C.c_longlong
+        L.lw_mlp_nparams.restype = _C.c_longlong
+        L.lw_guess_nsamples.restype = _C.c_longlong
         _lib = L
     return _lib
 
@@ -481,6 +483,31 @@ class Batch:
         return mv.value, (lu.value if lusol else None)
 
 
+def sweep_monostatic(batch, n, pol="VV", span=_np.pi, net=None, errmax=1e-4, maxit=20000, ctte=1.0,
+                     keep_currents=False):
+    """The monostatic sweep of gen_rcs_monostatica (main.f90:967-1089) in one
+    native call (lw_sweep_monostatic): n angles phi_a = a (span / n) in the
+    theta = 90 deg plane, VV or HH, run as chains through `batch` with the
+    learned guess of `net` (an Mlp) or the reference's warm start.  Returns
+    phi, sigma (linear), iterations, converged, the guess counts and, with
+    keep_currents, the (num_e, n) currents."""
+    op = batch.op
+    pc = {"VV": 0, "HH": 1}[pol]
+    sig = _np.zeros(n)
+    its = _np.zeros(n, _np.int32)
+    cv = _np.zeros(n, _np.int32)
+    cur = _np.zeros((op.ne, n), _np.complex128, order="F") if keep_currents else None
+    st = _np.zeros(5, _np.int32)
+    c = _np.array([complex(ctte).real, complex(ctte).imag])
+    _check(lib().lw_sweep_monostatic(op.h, batch.b, net.m if net is not None else None, int(n), _C.c_double(span),
+                                     pc, _P(c), _C.c_double(errmax), int(maxit), _P(sig), _P(its), _P(cv),
+                                     _P(cur), _P(st)))
+    phi = _np.arange(n) * (span / n)
+    return {"phi": phi, "sigma": sig, "iterations": its.astype(_np.int64), "converged": cv.astype(bool),
+            "currents": cur, "restarts": int(st[0]), "steals": int(st[1]),
+            "guesses": {"zero": int(st[2]), "one_neighbour": int(st[3]), "two_neighbours": int(st[4])}}
+
+
 class Mlp:
     """Pointwise MLP of the learned initial guess (lw_mlp_*): in -> hidden x
     depth (tanh) -> out, FP32 master weights, bfloat16 GEMM operands with FP32
@@ -499,6 +526,20 @@ class Mlp:
         self.m = m
         self.in_dim, self.hidden, self.depth, self.out_dim = in_dim, hidden, depth, out_dim
         self.nparams = int(L.lw_mlp_nparams(m))
+
+    @classmethod
+    def load(cls, path):
+        """A trained network from its .npz file, read by the library
+        (lw_guess_load)."""
+        m = _C.c_void_p()
+        _check(lib().lw_guess_load(_os.fsencode(path), _C.byref(m)))
+        with _np.load(path) as z:
+            arch = [int(v) for v in z["arch"]]
+        self = cls.__new__(cls)
+        self.m = m
+        self.in_dim, self.hidden, self.depth, self.out_dim = arch
+        self.nparams = int(lib().lw_mlp_nparams(m))
+        return self
 
     def close(self):
         if self.m:

@@ -2,9 +2,11 @@
 
 Produces
   lywell_amd/liblywell.so      HIP kernels + C ABI (include/lywell.h)
-  lywell_amd/liblywell_f.so    Fortran modules modMLFMA / modIterativo / modRCS
+  lywell_amd/liblywell_f.so    Fortran modules modMLFMA / modIterativo / modRCS /
+                               modMonostatica
                                (ISO_C_BINDING shims over the C ABI)
-  lywell_amd/lw_solve          Fortran host program driving the shims
+  lywell_amd/lw_solve          Fortran host program driving the shims (testMLFMA)
+  lywell_amd/lw_rcsmono        Fortran host program of the monostatic sweep
 and, as test infrastructure, oracle/liboracle.so (the C oracle) and, when the
 reference sources are present, oracle/_ref/ref_driver.
 
@@ -59,6 +61,7 @@ def build_hip():
         ("lw_plan.cpp", ["-x", "hip"]),
         ("lw_mesh.cpp", ["-x", "hip"]),
         ("lw_dat.cpp", ["-x", "hip"]),
+        ("lw_sweep.cpp", ["-x", "hip"]),
     ]
     for src, extra in units:
         obj = os.path.join(BUILD, src + ".o")
@@ -75,7 +78,8 @@ def build_fortran():
         raise RuntimeError("flang not found at " + FLANG)
     fdir = os.path.join(BUILD, "fortran")
     os.makedirs(fdir, exist_ok=True)
-    srcs = ["modGlobalParam.f90", "lw_capi.f90", "modMLFMA.f90", "modIterativo.f90", "modRCS.f90"]
+    srcs = ["modGlobalParam.f90", "lw_capi.f90", "modMLFMA.f90", "modIterativo.f90", "modRCS.f90",
+            "modMonostatica.f90"]
     objs = []
     for s in srcs:
         obj = os.path.join(fdir, s + ".o")
@@ -83,10 +87,11 @@ def build_fortran():
         objs.append(obj)
     lib = os.path.join(PKG, "liblywell_f.so")
     run([FLANG, "-shared", "-fPIC", "-o", lib] + objs + ["-L" + PKG, "-llywell", "-Wl,-rpath,$ORIGIN"])
-    drv_obj = os.path.join(fdir, "lw_solve.f90.o")
-    run([FLANG, "-O2", "-J", fdir, "-c", os.path.join(FSRC, "lw_solve.f90"), "-o", drv_obj])
-    exe = os.path.join(PKG, "lw_solve")
-    run([FLANG, "-o", exe, drv_obj, "-L" + PKG, "-llywell_f", "-llywell", "-Wl,-rpath,$ORIGIN"])
+    for prog in ("lw_solve", "lw_rcsmono"):
+        drv_obj = os.path.join(fdir, prog + ".f90.o")
+        run([FLANG, "-O2", "-J", fdir, "-c", os.path.join(FSRC, prog + ".f90"), "-o", drv_obj])
+        exe = os.path.join(PKG, prog)
+        run([FLANG, "-o", exe, drv_obj, "-L" + PKG, "-llywell_f", "-llywell", "-Wl,-rpath,$ORIGIN"])
     return lib
 
 

@@ -1911,6 +1911,8 @@ struct lw_batch {
   lw::Batch b;
 };
 
+lw::Handle* lw::batch_handle(lw_batch* b) { return b ? b->b.h : nullptr; }
+
 namespace lw {
 // The operator part of a batch (far-field tables in its precision, the near
 // columns of each leaf box, the far-field receive buffer).

@@ -237,6 +237,8 @@ void launch_translate_parent(Handle* h, int n);
 // One single-RHS matvec (MatxVec_MLFMA) on device vectors, on the handle's
 // stream (lw_device.hip).
 int run_matvec(Handle* h, const double2* x, double2* y, double2* y_near, hipEvent_t* ev);
+// The operator a batch was created on (lw_batch.hip); nullptr for nullptr.
+Handle* batch_handle(lw_batch* b);
 
 }  // namespace lw
 

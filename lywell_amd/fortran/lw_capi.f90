@@ -8,6 +8,8 @@ module lw_capi
   integer(c_int), parameter :: LW_OK = 0, LW_ERR_ARG = -1, LW_ERR_HIP = -2, LW_ERR_OCTREE = -3, &
                                LW_ERR_NOMEM = -4, LW_ERR_STATE = -5, LW_ERR_PIVOT = -6, LW_ERR_NODEV = -7
   integer(c_int), parameter :: LW_NEAR_FP64 = 1
+  integer(c_int), parameter :: LW_BATCH_FAR_FP64 = 1, LW_BATCH_FP64_RESIDUAL = 2
+  integer(c_int), parameter :: LW_POL_VV = 0, LW_POL_HH = 1
 
   interface
     function lw_last_error() result(p) bind(C, name="lw_last_error")
@@ -174,6 +176,163 @@ module lw_capi
       integer(c_int), value :: n
       real(c_double), intent(in) :: theta(*), phi(*)
       real(c_double), intent(out) :: sigma(*)
+      integer(c_int) :: rc
+    end function
+    ! ---- result files (guardarMatrizR / guardarVectorC, modGlobalMetod.f90:192-306)
+    function lw_dat_write_matrix_r(path, nrows, ncols, m) result(rc) bind(C, name="lw_dat_write_matrix_r")
+      import :: c_int, c_char, c_double
+      character(kind=c_char), intent(in) :: path(*)
+      integer(c_int), value :: nrows, ncols
+      real(c_double), intent(in) :: m(*)
+      integer(c_int) :: rc
+    end function
+
+    function lw_dat_write_vector_c(path, n, v) result(rc) bind(C, name="lw_dat_write_vector_c")
+      import :: c_int, c_char, c_double_complex
+      character(kind=c_char), intent(in) :: path(*)
+      integer(c_int), value :: n
+      complex(c_double_complex), intent(in) :: v(*)
+      integer(c_int) :: rc
+    end function
+
+    ! ---- multi-RHS form: blocks are complex(8) (nrhs, num_e) arrays; a
+    ! ---- c_null_ptr guess is a zero guess
+    function lw_batch_create(h, nrhs, flags, b) result(rc) bind(C, name="lw_batch_create")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: h
+      integer(c_int), value :: nrhs, flags
+      type(c_ptr), intent(out) :: b
+      integer(c_int) :: rc
+    end function
+
+    function lw_batch_destroy(b) result(rc) bind(C, name="lw_batch_destroy")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: b
+      integer(c_int) :: rc
+    end function
+
+    function lw_batch_nrhs(b) result(n) bind(C, name="lw_batch_nrhs")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: b
+      integer(c_int) :: n
+    end function
+
+    function lw_batch_set_replacement(b, period) result(rc) bind(C, name="lw_batch_set_replacement")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: b
+      integer(c_int), value :: period
+      integer(c_int) :: rc
+    end function
+
+    function lw_batch_matvec(b, x, y) result(rc) bind(C, name="lw_batch_matvec")
+      import :: c_int, c_ptr, c_double_complex
+      type(c_ptr), value :: b
+      complex(c_double_complex), intent(in) :: x(*)
+      complex(c_double_complex), intent(out) :: y(*)
+      integer(c_int) :: rc
+    end function
+
+    function lw_batch_cgs(b, rhs, guess, errmax, use_prec, maxit, x, iters, relres, converged) result(rc) &
+        bind(C, name="lw_batch_cgs")
+      import :: c_int, c_ptr, c_double, c_double_complex
+      type(c_ptr), value :: b, guess
+      complex(c_double_complex), intent(in) :: rhs(*)
+      real(c_double), value :: errmax
+      integer(c_int), value :: use_prec, maxit
+      complex(c_double_complex), intent(out) :: x(*)
+      integer(c_int), intent(out) :: iters(*), converged(*)
+      real(c_double), intent(out) :: relres(*)
+      integer(c_int) :: rc
+    end function
+
+    function lw_batch_cgs_start(b, rhs, guess, errmax, use_prec, maxit) result(rc) &
+        bind(C, name="lw_batch_cgs_start")
+      import :: c_int, c_ptr, c_double, c_double_complex
+      type(c_ptr), value :: b, guess
+      complex(c_double_complex), intent(in) :: rhs(*)
+      real(c_double), value :: errmax
+      integer(c_int), value :: use_prec, maxit
+      integer(c_int) :: rc
+    end function
+
+    function lw_batch_cgs_advance(b, iterations, state, relres) result(rc) bind(C, name="lw_batch_cgs_advance")
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: b
+      integer(c_int), value :: iterations
+      integer(c_int), intent(out) :: state(4, *)
+      real(c_double), intent(out) :: relres(*)
+      integer(c_int) :: rc
+    end function
+
+    function lw_batch_column(b, col, x) result(rc) bind(C, name="lw_batch_column")
+      import :: c_int, c_ptr, c_double_complex
+      type(c_ptr), value :: b
+      integer(c_int), value :: col
+      complex(c_double_complex), intent(out) :: x(*)
+      integer(c_int) :: rc
+    end function
+
+    function lw_batch_restart_column(b, col, rhs, guess) result(rc) bind(C, name="lw_batch_restart_column")
+      import :: c_int, c_ptr, c_double_complex
+      type(c_ptr), value :: b, guess
+      integer(c_int), value :: col
+      complex(c_double_complex), intent(in) :: rhs(*)
+      integer(c_int) :: rc
+    end function
+
+    function lw_batch_plane_wave(b, dirs, pols, ctte, rhs) result(rc) bind(C, name="lw_batch_plane_wave")
+      import :: c_int, c_ptr, c_double, c_double_complex
+      type(c_ptr), value :: b
+      real(c_double), intent(in) :: dirs(3, *)
+      complex(c_double_complex), intent(in) :: pols(3, *), ctte
+      complex(c_double_complex), intent(out) :: rhs(*)
+      integer(c_int) :: rc
+    end function
+
+    function lw_batch_rcs(b, cur, pols, ctte, theta, phi, sigma) result(rc) bind(C, name="lw_batch_rcs")
+      import :: c_int, c_ptr, c_double, c_double_complex
+      type(c_ptr), value :: b
+      complex(c_double_complex), intent(in) :: cur(*), pols(3, *), ctte
+      real(c_double), intent(in) :: theta(*), phi(*)
+      real(c_double), intent(out) :: sigma(*)
+      integer(c_int) :: rc
+    end function
+
+    ! ---- learned initial guess
+    function lw_guess_load(path, m) result(rc) bind(C, name="lw_guess_load")
+      import :: c_int, c_ptr, c_char
+      character(kind=c_char), intent(in) :: path(*)
+      type(c_ptr), intent(out) :: m
+      integer(c_int) :: rc
+    end function
+
+    function lw_mlp_destroy(m) result(rc) bind(C, name="lw_mlp_destroy")
+      import :: c_int, c_ptr
+      type(c_ptr), value :: m
+      integer(c_int) :: rc
+    end function
+
+    function lw_guess_predict(m, h, ncol, dir, pol, dir1, x1, dir2, x2, guess) result(rc) &
+        bind(C, name="lw_guess_predict")
+      import :: c_int, c_ptr, c_double, c_double_complex
+      type(c_ptr), value :: m, h
+      integer(c_int), value :: ncol
+      real(c_double), intent(in) :: dir(3, *), pol(3, *), dir1(3, *), dir2(3, *)
+      complex(c_double_complex), intent(in) :: x1(*), x2(*)
+      complex(c_double_complex), intent(out) :: guess(*)
+      integer(c_int) :: rc
+    end function
+
+    ! ---- monostatic sweep (gen_rcs_monostatica, main.f90:967-1089)
+    function lw_sweep_monostatic(h, b, net, n, span, pol, ctte, errmax, maxit, sigma, iterations, converged, &
+                                 currents, stats) result(rc) bind(C, name="lw_sweep_monostatic")
+      import :: c_int, c_ptr, c_double, c_double_complex
+      type(c_ptr), value :: h, b, net, currents
+      integer(c_int), value :: n, pol, maxit
+      real(c_double), value :: span, errmax
+      complex(c_double_complex), intent(in) :: ctte
+      real(c_double), intent(out) :: sigma(*)
+      integer(c_int), intent(out) :: iterations(*), converged(*), stats(5)
       integer(c_int) :: rc
     end function
   end interface

@@ -175,3 +175,32 @@ def test_random_waves_are_transverse_unit_vectors():
         d, p = random_wave(rng)
         assert abs(np.linalg.norm(d) - 1) < 1e-12 and abs(np.linalg.norm(p) - 1) < 1e-12
         assert abs(d @ p) < 1e-12
+
+
+def test_network_file_errors_are_reported_before_any_device_use(tmp_path):
+    import lywell_amd as lw
+    with pytest.raises(lw.LywellError, match="cannot open"):
+        lw.Mlp.load(str(tmp_path / "missing.npz"))
+    bad = tmp_path / "bad.npz"
+    # a valid archive with the wrong dtype for arch
+    np.savez(bad, arch=np.array([19.0, 256, 3, 4]), params=np.zeros(10, np.float32))
+    with pytest.raises(lw.LywellError, match="not a network file"):
+        lw.Mlp.load(str(bad))
+    # a compressed archive is refused, not misread
+    np.savez_compressed(bad, arch=np.array([19, 256, 3, 4]), params=np.zeros(10, np.float32))
+    with pytest.raises(lw.LywellError, match="not a network file"):
+        lw.Mlp.load(str(bad))
+    # a shape beyond any network this library trains
+    np.savez(bad, arch=np.array([19, 1 << 30, 3, 4]), params=np.zeros(10, np.float32))
+    with pytest.raises(lw.LywellError, match="implausible"):
+        lw.Mlp.load(str(bad))
+
+
+def test_native_sweep_checks_its_arguments():
+    import ctypes
+    import lywell_amd as lw
+    s = np.zeros(4)
+    c = np.array([1.0, 0.0])
+    rc = lw.lib().lw_sweep_monostatic(None, None, None, 4, ctypes.c_double(np.pi), 0, lw._P(c),
+                                      ctypes.c_double(1e-4), 100, lw._P(s), None, None, None, None)
+    assert rc == lw.LW_ERR_ARG
