@@ -330,7 +330,8 @@ def batched_sweep(lw, op, args, plain_rhs_per_s):
     best, times = autotune.tune(op, far_fp64=fp64, log=log)
     R = args.batch if args.batch in (64, 128) else best
     bt = lw.Batch(op, R, far_fp64=fp64)
-    mv_ms = times.get(R) or bt.bench(reps=3)[0]
+    tiles = autotune.tune_tiles(bt, log=log)
+    mv_ms = tiles["ms_after"]
     log(f"batched matvec (R={R}, far field {'FP64' if fp64 else 'FP32'}): {mv_ms:.3f} ms = "
         f"{mv_ms / R * 1e3:.1f} us per RHS")
     net = guess_train.load() if args.guess == "mlp" else None
@@ -386,7 +387,7 @@ def batched_sweep(lw, op, args, plain_rhs_per_s):
             "speedup_vs_plain_single": round(value / plain_rhs_per_s, 2),
             "plain_sweep_rhs_per_s": round(plain_sweep_rhs_s, 4), "plain_sweep_iterations": plain_its,
             "angles": n, "nrhs_per_batch": R, "chain_length": m, "seconds": round(res["seconds"], 2),
-            "guess": guess, "far_field": "FP64" if fp64 else "FP32", "autotune_ms": times,
+            "guess": guess, "far_field": "FP64" if fp64 else "FP32", "autotune_ms": times, "autotune_tiles": tiles,
             "matvec_ms": round(mv_ms, 3), "matvec_ms_per_rhs": round(mv_ms / R, 4),
             "iterations_mean": float(its.mean()), "iterations_max": int(its.max()),
             "iterations_by_guess": by_kind, "restarts": res["restarts"], "steals": res["steals"],

@@ -274,6 +274,25 @@ int lw_batch_rcs(lw_batch *b, const double *cur, const double *pols, const doubl
                  const double *phi, double *sigma);
 /* Mean batched matvec time and one batched lusol0 time in ms (HIP events). */
 int lw_bench_batch(lw_batch *b, int reps, double *matvec_ms, double *lusol_ms);
+/* Tile shapes of the batched operator, chosen by timing each candidate on
+ * this batch (HIP events, `reps` launches each): per level, the theta block
+ * of the upward pass (4, 8 or 16 coarse rows per wave) and, for the fused
+ * translation + downward pass, the sources per pipelined group and columns
+ * per lane ((4,2), (2,2), (4,1) or (2,1)), whether the column chunks run
+ * outermost, whether the transfer row is read from LDS, whether the
+ * downward pass runs in a kernel of its own, and the parent ranges over the
+ * XCDs (1, 2 or 8).  Every choice gives the same results.  ms (optional, 2):
+ * the batched matvec time before and after; config (optional,
+ * 3 (lw_mlfma_nlevels + 1)): per level n, (theta block, 10000 separate +
+ * 1000 LDS + 100 chunks-outermost + 10 sources + columns, ranges), zeros
+ * where a level has no such pass.  It overwrites the batch's
+ * iterate: call it before a solve (a solve in progress must be started
+ * again).  No reference counterpart. */
+int lw_batch_tune(lw_batch *b, int reps, double *ms, int *config);
+/* Sets the tile shapes explicitly, in lw_batch_tune's config layout (levels
+ * 0-2 ignored; theta block 0 at level 3): pins a tuned configuration, or
+ * compares shapes.  LW_ERR_ARG for a shape outside the lists above. */
+int lw_batch_set_tiles(lw_batch *b, const int *config);
 
 /* ---- learned initial guess for monostatic sweeps ---------------------------
  * No reference counterpart except the warm start it generalises: the

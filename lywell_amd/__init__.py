@@ -475,6 +475,33 @@ class Batch:
         _check(lib().lw_batch_rcs(self.b, _P(I), _P(p), _P(c), _P(th), _P(ph), _P(s)))
         return s
 
+    def tune(self, reps=3):
+        """Choose the tile shapes of every level by timing (lw_batch_tune).
+        Returns (ms before, ms after, {level: (theta block, (sources per
+        group, columns per lane, chunks outermost, transfer row in LDS,
+        downward pass in its own kernel), parent ranges)})."""
+        L = self.op.nlevels
+        ms = _np.zeros(2)
+        cfg = _np.zeros(3 * (L + 1), _np.int32)
+        _check(lib().lw_batch_tune(self.b, int(reps), _P(ms), _P(cfg)))
+        c = cfg.reshape(L + 1, 3)
+        levels = {n: (int(c[n, 0]), (int(c[n, 1]) % 100 // 10, int(c[n, 1]) % 10, int(c[n, 1]) // 100 % 10,
+                                     int(c[n, 1]) // 1000 % 10, int(c[n, 1]) // 10000), int(c[n, 2]))
+                  for n in range(3, L + 1)}
+        return float(ms[0]), float(ms[1]), levels
+
+    def set_tiles(self, levels):
+        """Set the tile shapes per level, {level: (theta block, (sources per
+        group, columns per lane[, chunks outermost[, transfer row in LDS[,
+        downward pass in its own kernel]]]), parent ranges)}
+        (lw_batch_set_tiles)."""
+        L = self.op.nlevels
+        cfg = _np.zeros(3 * (L + 1), _np.int32)
+        for n, (tb, far, ns) in levels.items():
+            su, cpl, co, tl, sep = (tuple(far) + (0, 0, 0))[:5]
+            cfg[3 * n: 3 * n + 3] = (tb, 10000 * sep + 1000 * tl + 100 * co + 10 * su + cpl, ns)
+        _check(lib().lw_batch_set_tiles(self.b, _P(cfg)))
+
     def bench(self, reps=5, lusol=False):
         """Mean batched matvec time in ms (and one batched lusol0 time when
         lusol is set and the operator has an ILU0 factor)."""

@@ -188,17 +188,18 @@ __global__ void __launch_bounds__(256) k_b_aggregate(int nk, int nb, const int* 
 }
 
 // Upward pass with the factored stencil (W = Wphi (x) Wtheta), one wave per
-// (parent P, coarse phi column ip, block of kThBlock coarse theta rows,
-// 64-column chunk):
+// (parent P, coarse phi column ip, block of TB coarse theta rows, 64-column
+// chunk):
 //   C_{n-1}[(ip, it)][P] = sum_c ph(oct_c, (ip, it))
 //                          sum_a wphi[ip][a] sum_b wth[it][b] C_n[(fa, tb)][c]
 // The wave walks its theta rows in order; the 3x3 window of fine samples
 // slides with the theta stencil, so most steps load 3 new values (one per
 // fine phi column) instead of 9, and the values of the next step are fetched
 // before the current one is computed.  Accumulators stay in registers.
+// Theta blocks the autotuner chooses from (kThBlock is the default).
 constexpr int kThBlock = 8;
 
-template <typename S>
+template <typename S, int TB>
 __global__ void __launch_bounds__(256) k_b_interp_up(int ncol, int nthc, int nkc, int nbc, int nthf, int nbf,
                                                      int nphc, const int* __restrict__ child_ptr,
                                                      const int* __restrict__ oct, const int* __restrict__ sth_idx,
@@ -209,7 +210,7 @@ __global__ void __launch_bounds__(256) k_b_interp_up(int ncol, int nthc, int nkc
                                                      typename Cx<S>::T* __restrict__ Cc) {
   typedef typename Cx<S>::T T2;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave), lane = threadIdx.x % kWave;
-  const int nchunk = ncol / kWave, ntb = (nthc + kThBlock - 1) / kThBlock;
+  const int nchunk = ncol / kWave, ntb = (nthc + TB - 1) / TB;
   int t = xcd_block(blockIdx.x, gridDim.x) * (blockDim.x / kWave) + wave;
   const int chunk = t % nchunk;
   t /= nchunk;
@@ -218,10 +219,10 @@ __global__ void __launch_bounds__(256) k_b_interp_up(int ncol, int nthc, int nkc
   const int ip = t % nphc, P = t / nphc;
   if (P >= nbc) return;
   const int col = chunk * kWave + lane;
-  const int it0 = tbk * kThBlock, nit = min(kThBlock, nthc - it0);
-  T2 acc[kThBlock];
+  const int it0 = tbk * TB, nit = min(TB, nthc - it0);
+  T2 acc[TB];
 #pragma unroll
-  for (int u = 0; u < kThBlock; ++u) acc[u] = mk<S>(0, 0);
+  for (int u = 0; u < TB; ++u) acc[u] = mk<S>(0, 0);
   const int f0 = sph_idx[3 * ip], f1 = sph_idx[3 * ip + 1], f2 = sph_idx[3 * ip + 2];
   const S w0 = sph_w[3 * ip], w1 = sph_w[3 * ip + 1], w2 = sph_w[3 * ip + 2];
   const size_t ks = (size_t)nbf * ncol;  // stride of one fine sample
@@ -241,7 +242,7 @@ __global__ void __launch_bounds__(256) k_b_interp_up(int ncol, int nthc, int nkc
       v[2][b] = s2[(size_t)tb * ks];
     }
 #pragma unroll
-    for (int u = 0; u < kThBlock; ++u) {
+    for (int u = 0; u < TB; ++u) {
       if (u < nit) {
         const int it = it0 + u;
         int tn[3] = {cur[0], cur[1], cur[2]};
@@ -289,12 +290,12 @@ __global__ void __launch_bounds__(256) k_b_interp_up(int ncol, int nthc, int nkc
   }
   T2* out = Cc + ((size_t)(ip * nthc + it0) * nbc + P) * ncol + col;
 #pragma unroll
-  for (int u = 0; u < kThBlock; ++u)
+  for (int u = 0; u < TB; ++u)
     if (u < nit) out[(size_t)u * nbc * ncol] = acc[u];
 }
 
 // Translation fused with the downward pass, one wave per (sample k, parent
-// box P, 64*kFarCpl-column chunk) at level n:
+// box P, 64*CPL-column chunk: CPL columns per lane) at level n:
 //   A_n[k][c] = sum_{far d} pes[k] T(off(c,d))[k] C_n[k][d]
 //             + sum_{(i,w) in W^T(k)} w conj(ph_{n-1}(oct_c, i)) A_{n-1}[i][P]
 // for the children c of P.  The far boxes of every child of P are the
@@ -307,18 +308,19 @@ __global__ void __launch_bounds__(256) k_b_interp_up(int ncol, int nthc, int nkc
 //
 // Per parent the sources are one flat list of (box d, code part
 // 49 (2ox+3+bx_d) + 7 (2oy+3+by_d) + (2oz+3+bz_d) - 57), padded to a multiple
-// of kSrcU with entries whose codes are near offsets (T = 0).  Everything that
+// of kSrcU = 4 with entries whose codes are near offsets (T = 0).  Everything that
 // selects a transfer value (k, P, the source entry, the child's octant) is
 // uniform over the wave, so the values come in through scalar loads from the
 // Tk row and enter the FMAs as scalar operands; the children of P are
 // compacted (nch = 1..8 accumulators, one code path per count), so no FMA is
 // spent on absent octants and the inner loop has no branches.  Source rows
-// are loaded one group of kSrcU ahead.  The parents are cut into 1, 2, 4 or 8
+// are loaded one group of SU (2 or 4) ahead.  Every variant adds each child's
+// terms in the same order, so all give the same bits.  The parents are cut into 1, 2, 4 or 8
 // contiguous ranges per level, each range on the XCDs (workgroup id mod 8)
 // that take its samples (see k_b_far).
 constexpr int kOffCodes = 343;  // (dx, dy, dz) in [-3, 3]^3
-constexpr int kSrcU = 4;        // sources per pipelined group
-constexpr int kFarCpl = 2;      // columns per lane
+constexpr int kSrcU = 4;        // source-list padding: the lists are a multiple of this
+constexpr int kFarCpl = 2;      // columns per lane of the default far-kernel variant
 // 49*2 + 7*2 + 2: plus any child's octant part 57 - oct_code(o), an offset in
 // [-1, 1]^3, which has no transfer vector (T = 0)
 constexpr int kPadCode = 114;
@@ -346,8 +348,8 @@ __device__ __forceinline__ void cmac(T2& acc, const T2 t, const T2 v) {
       typename Cx<S>::T* __restrict__ A
 #define LW_FAR_ARGS nb, ncol, nbc, nwt, pc_ptr, bcd, boct, ps_ptr, ps_ent, Tk, C, wt_ptr, wt_row, wp, Ac, A
 
-template <typename S, int NCH>
-__device__ __forceinline__ void far_parent(int k, int P, int col, LW_FAR_PARAMS) {
+template <typename S, int NCH, int SU, int CPL>
+__device__ __forceinline__ void far_parent(int k, int P, int col, const typename Cx<S>::T* Tkk, LW_FAR_PARAMS) {
   typedef typename Cx<S>::T T2;
   constexpr int kMaxCq = oct_code(7);  // 57
   const int c0 = pc_ptr[P];
@@ -357,35 +359,34 @@ __device__ __forceinline__ void far_parent(int k, int P, int col, LW_FAR_PARAMS)
     coff[i] = kMaxCq - bcd[c0 + i];
     wo[i] = boct[c0 + i] * nwt;
   }
-  T2 acc[NCH][kFarCpl];
+  T2 acc[NCH][CPL];
 #pragma unroll
   for (int i = 0; i < NCH; ++i)
 #pragma unroll
-    for (int c = 0; c < kFarCpl; ++c) acc[i][c] = mk<S>(0, 0);
-  const T2* Tkk = Tk + (size_t)k * kOffCodes;
+    for (int c = 0; c < CPL; ++c) acc[i][c] = mk<S>(0, 0);
   // offsets within a level's sample slice fit 32 bits (checked at set-up)
   const T2* Ck = C + (size_t)k * nb * ncol + col;
   const int s0 = ps_ptr[P], s1 = ps_ptr[P + 1];
   if (s0 < s1) {
-    int2 ea[kSrcU], eb[kSrcU];
-    T2 va[kSrcU][kFarCpl], vb[kSrcU][kFarCpl];
-    auto load = [&](int s, int2(&e)[kSrcU], T2(&v)[kSrcU][kFarCpl]) {
+    int2 ea[SU], eb[SU];
+    T2 va[SU][CPL], vb[SU][CPL];
+    auto load = [&](int s, int2(&e)[SU], T2(&v)[SU][CPL]) {
 #pragma unroll
-      for (int u = 0; u < kSrcU; ++u) {
+      for (int u = 0; u < SU; ++u) {
         e[u] = ps_ent[s + u];
 #pragma unroll
-        for (int c = 0; c < kFarCpl; ++c) v[u][c] = Ck[(unsigned)(e[u].x * ncol + c * kWave)];
+        for (int c = 0; c < CPL; ++c) v[u][c] = Ck[(unsigned)(e[u].x * ncol + c * kWave)];
       }
     };
-    auto apply = [&](const int2(&e)[kSrcU], const T2(&v)[kSrcU][kFarCpl]) {
+    auto apply = [&](const int2(&e)[SU], const T2(&v)[SU][CPL]) {
 #pragma unroll
-      for (int u = 0; u < kSrcU; ++u) {
+      for (int u = 0; u < SU; ++u) {
         const T2* tb = Tkk + e[u].y;
 #pragma unroll
         for (int i = 0; i < NCH; ++i) {
           const T2 t = tb[coff[i]];
 #pragma unroll
-          for (int c = 0; c < kFarCpl; ++c) cmac(acc[i][c], t, v[u][c]);
+          for (int c = 0; c < CPL; ++c) cmac(acc[i][c], t, v[u][c]);
         }
       }
     };
@@ -396,15 +397,15 @@ __device__ __forceinline__ void far_parent(int k, int P, int col, LW_FAR_PARAMS)
     // back down to their uses)
     load(s0, ea, va);
     for (int s = s0;;) {
-      load(min(s + kSrcU, s1 - kSrcU), eb, vb);
+      load(min(s + SU, s1 - SU), eb, vb);
       __builtin_amdgcn_sched_barrier(0);
       apply(ea, va);
-      s += kSrcU;
+      s += SU;
       if (s >= s1) break;
-      load(min(s + kSrcU, s1 - kSrcU), ea, va);
+      load(min(s + SU, s1 - SU), ea, va);
       __builtin_amdgcn_sched_barrier(0);
       apply(eb, vb);
-      s += kSrcU;
+      s += SU;
       if (s >= s1) break;
     }
   }
@@ -412,23 +413,23 @@ __device__ __forceinline__ void far_parent(int k, int P, int col, LW_FAR_PARAMS)
   if (e0 < e1) {
     // entries come in groups of 4 (padded with zero weights), loaded one
     // group ahead like the sources
-    T2 aa[4][kFarCpl], ab[4][kFarCpl];
-    auto load = [&](int e, T2(&a)[4][kFarCpl]) {
+    T2 aa[4][CPL], ab[4][CPL];
+    auto load = [&](int e, T2(&a)[4][CPL]) {
 #pragma unroll
       for (int u = 0; u < 4; ++u) {
         const T2* ar = Ac + (unsigned)((wt_row[e + u] * nbc + P) * ncol + col);
 #pragma unroll
-        for (int c = 0; c < kFarCpl; ++c) a[u][c] = ar[c * kWave];
+        for (int c = 0; c < CPL; ++c) a[u][c] = ar[c * kWave];
       }
     };
-    auto apply = [&](int e, const T2(&a)[4][kFarCpl]) {
+    auto apply = [&](int e, const T2(&a)[4][CPL]) {
 #pragma unroll
       for (int u = 0; u < 4; ++u)
 #pragma unroll
         for (int i = 0; i < NCH; ++i) {
           const T2 w = wp[wo[i] + e + u];
 #pragma unroll
-          for (int c = 0; c < kFarCpl; ++c) cmac(acc[i][c], w, a[u][c]);
+          for (int c = 0; c < CPL; ++c) cmac(acc[i][c], w, a[u][c]);
         }
     };
     load(e0, aa);
@@ -449,38 +450,149 @@ __device__ __forceinline__ void far_parent(int k, int P, int col, LW_FAR_PARAMS)
 #pragma unroll
   for (int i = 0; i < NCH; ++i)
 #pragma unroll
-    for (int c = 0; c < kFarCpl; ++c) Ak[(unsigned)((c0 + i) * ncol + c * kWave)] = acc[i][c];
+    for (int c = 0; c < CPL; ++c) Ak[(unsigned)((c0 + i) * ncol + c * kWave)] = acc[i][c];
 }
 
-template <typename S>
-__global__ void __launch_bounds__(256) k_b_far(int nk, int nP, int nchunk, int wpx, int nsplit, LW_FAR_PARAMS) {
+template <typename S, int SU, int CPL, bool TL>
+__global__ void __launch_bounds__(256) k_b_far(int nk, int nP, int nchunk, int wpx, int nsplit, int chunk_outer,
+                                               LW_FAR_PARAMS) {
+  typedef typename Cx<S>::T T2;
   // Workgroup map: within a workgroup one sample k and one column chunk,
   // 4 waves = 4 consecutive parents.  The parents are cut into nsplit
   // contiguous ranges (nsplit divides 8) and XCD x owns range x % nsplit for
   // the samples k = x / nsplit mod 8 / nsplit, so that its L2 holds the
   // sources of a box range (1/nsplit of a k-slice plus a halo) for the few
   // samples it has in flight.  nsplit = 1 gives each XCD whole k-slices,
-  // nsplit = 8 a range of every k-slice.
+  // nsplit = 8 a range of every k-slice.  The column chunks of a sample
+  // follow each other (chunk_outer = 0), or each chunk runs over all samples
+  // before the next (chunk_outer = 1), which halves the rows in flight.
+  // TL: the workgroup's transfer row Tk[k][.] is staged in LDS and read from
+  // there (broadcast reads into vector registers) instead of through scalar
+  // loads from the table.
   const int g = blockIdx.x, x = g % 8, j = g / 8;
   const int kstep = 8 / nsplit, pr = x % nsplit;
   const int nPx = (nP + nsplit - 1) / nsplit, nPBx = wpx / nchunk;
-  const int k = (j / wpx) * kstep + x / nsplit;
-  const int chunk = (j % wpx) / nPBx, pb = (j % wpx) % nPBx;
+  int k, chunk, pb;
+  if (chunk_outer) {
+    const int per = ((nk + kstep - 1) / kstep) * nPBx;
+    chunk = j / per;
+    k = ((j % per) / nPBx) * kstep + x / nsplit;
+    pb = (j % per) % nPBx;
+  } else {
+    k = (j / wpx) * kstep + x / nsplit;
+    chunk = (j % wpx) / nPBx;
+    pb = (j % wpx) % nPBx;
+  }
   const int P0 = pr * nPx + pb * 4, P1 = min((pr + 1) * nPx, nP);
   if (k >= nk) return;  // uniform over the workgroup
+  const T2* Tkk = Tk + (size_t)k * kOffCodes;
+  if constexpr (TL) {
+    __shared__ T2 tsh[kOffCodes];
+    for (int i = threadIdx.x; i < kOffCodes; i += blockDim.x) tsh[i] = Tkk[i];
+    __syncthreads();
+    Tkk = tsh;
+  }
   const int P = P0 + __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
   if (P >= P1) return;
-  const int col = chunk * kWave * kFarCpl + (threadIdx.x % kWave);
+  const int col = chunk * kWave * CPL + (threadIdx.x % kWave);
   switch (pc_ptr[P + 1] - pc_ptr[P]) {
-    case 1: far_parent<S, 1>(k, P, col, LW_FAR_ARGS); break;
-    case 2: far_parent<S, 2>(k, P, col, LW_FAR_ARGS); break;
-    case 3: far_parent<S, 3>(k, P, col, LW_FAR_ARGS); break;
-    case 4: far_parent<S, 4>(k, P, col, LW_FAR_ARGS); break;
-    case 5: far_parent<S, 5>(k, P, col, LW_FAR_ARGS); break;
-    case 6: far_parent<S, 6>(k, P, col, LW_FAR_ARGS); break;
-    case 7: far_parent<S, 7>(k, P, col, LW_FAR_ARGS); break;
-    case 8: far_parent<S, 8>(k, P, col, LW_FAR_ARGS); break;
+    case 1: far_parent<S, 1, SU, CPL>(k, P, col, Tkk, LW_FAR_ARGS); break;
+    case 2: far_parent<S, 2, SU, CPL>(k, P, col, Tkk, LW_FAR_ARGS); break;
+    case 3: far_parent<S, 3, SU, CPL>(k, P, col, Tkk, LW_FAR_ARGS); break;
+    case 4: far_parent<S, 4, SU, CPL>(k, P, col, Tkk, LW_FAR_ARGS); break;
+    case 5: far_parent<S, 5, SU, CPL>(k, P, col, Tkk, LW_FAR_ARGS); break;
+    case 6: far_parent<S, 6, SU, CPL>(k, P, col, Tkk, LW_FAR_ARGS); break;
+    case 7: far_parent<S, 7, SU, CPL>(k, P, col, Tkk, LW_FAR_ARGS); break;
+    case 8: far_parent<S, 8, SU, CPL>(k, P, col, Tkk, LW_FAR_ARGS); break;
     default: break;
+  }
+}
+
+// The downward pass of the batched form as its own kernel (the second half
+// of far_parent, run after the translation has stored A_n = sum over far
+// boxes): A_n[k][c] += sum_{(i,w) in W^T(k)} w conj(ph_{n-1}(oct_c, i))
+// A_{n-1}[i][P] for the children c of P, each term added in far_parent's
+// order, so the result has the same bits as the fused kernel.  A workgroup
+// takes one parent, one 64-column chunk and kAntK consecutive samples (four
+// per wave), and consecutive workgroups on one XCD take the same parent, so
+// a parent row A_{n-1}[i][P], which the stencils of several neighbouring
+// samples read, comes from the XCD's L2 instead of memory.
+constexpr int kAntK = 16;
+
+template <typename S, int NCH>
+__device__ __forceinline__ void anterp_parent(int k, int P, int col, int nb, int ncol, int nbc, int nwt,
+                                              const int* __restrict__ pc_ptr, const int* __restrict__ boct,
+                                              const int* __restrict__ wt_ptr, const int* __restrict__ wt_row,
+                                              const typename Cx<S>::T* __restrict__ wp,
+                                              const typename Cx<S>::T* __restrict__ Ac, typename Cx<S>::T* A) {
+  typedef typename Cx<S>::T T2;
+  const int c0 = pc_ptr[P];
+  int wo[NCH];
+#pragma unroll
+  for (int i = 0; i < NCH; ++i) wo[i] = boct[c0 + i] * nwt;
+  T2* Ak = A + (size_t)k * nb * ncol + col;
+  T2 acc[NCH];
+#pragma unroll
+  for (int i = 0; i < NCH; ++i) acc[i] = Ak[(unsigned)((c0 + i) * ncol)];
+  const int e0 = wt_ptr[k], e1 = wt_ptr[k + 1];
+  if (e0 < e1) {
+    T2 aa[4], ab[4];
+    auto load = [&](int e, T2(&a)[4]) {
+#pragma unroll
+      for (int u = 0; u < 4; ++u) a[u] = Ac[(unsigned)((wt_row[e + u] * nbc + P) * ncol + col)];
+    };
+    auto apply = [&](int e, const T2(&a)[4]) {
+#pragma unroll
+      for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int i = 0; i < NCH; ++i) cmac(acc[i], wp[wo[i] + e + u], a[u]);
+    };
+    load(e0, aa);
+    for (int e = e0;;) {
+      load(min(e + 4, e1 - 4), ab);
+      __builtin_amdgcn_sched_barrier(0);
+      apply(e, aa);
+      e += 4;
+      if (e >= e1) break;
+      load(min(e + 4, e1 - 4), aa);
+      __builtin_amdgcn_sched_barrier(0);
+      apply(e, ab);
+      e += 4;
+      if (e >= e1) break;
+    }
+  }
+#pragma unroll
+  for (int i = 0; i < NCH; ++i) Ak[(unsigned)((c0 + i) * ncol)] = acc[i];
+}
+
+template <typename S>
+__global__ void __launch_bounds__(256) k_b_anterp(int nk, int nP, int nchunk, int nb, int ncol, int nbc, int nwt,
+                                                  const int* __restrict__ pc_ptr, const int* __restrict__ boct,
+                                                  const int* __restrict__ wt_ptr, const int* __restrict__ wt_row,
+                                                  const typename Cx<S>::T* __restrict__ wp,
+                                                  const typename Cx<S>::T* __restrict__ Ac,
+                                                  typename Cx<S>::T* A) {
+  const int nkb = (nk + kAntK - 1) / kAntK;
+  int t = xcd_block(blockIdx.x, gridDim.x);
+  const int kb = t % nkb;
+  t /= nkb;
+  const int chunk = t % nchunk, P = t / nchunk;
+  if (P >= nP) return;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+  const int col = chunk * kWave + (threadIdx.x % kWave);
+  const int nch = pc_ptr[P + 1] - pc_ptr[P];
+  for (int m = 0; m < kAntK / 4; ++m) {
+    const int k = kb * kAntK + wave * (kAntK / 4) + m;
+    if (k >= nk) break;
+    switch (nch) {
+#define LW_ANT(c)                                                                                                \
+  case c:                                                                                                        \
+    anterp_parent<S, c>(k, P, col, nb, ncol, nbc, nwt, pc_ptr, boct, wt_ptr, wt_row, wp, Ac, A);                 \
+    break;
+      LW_ANT(1) LW_ANT(2) LW_ANT(3) LW_ANT(4) LW_ANT(5) LW_ANT(6) LW_ANT(7) LW_ANT(8)
+#undef LW_ANT
+      default: break;
+    }
   }
 }
 
@@ -1271,8 +1383,13 @@ struct Batch {
   int *ncol_ptr = nullptr, *ncols = nullptr;  // near columns of each leaf box (shared by its rows)
   void* rad = nullptr;                        // leaf patterns in the far-field precision
   std::vector<BatchLevel> lv;
-  // parent ranges of the far kernel per level (1, 2, 4 or 8; see k_b_far)
+  // tile shapes per level, set at creation and by lw_batch_tune: the parent
+  // ranges of the far kernel (1, 2, 4 or 8; see k_b_far), its variant
+  // (kFarVariants) and the theta block of the upward pass (kUpBlocks)
   int far_split[16] = {};
+  int far_var[16] = {};
+  int far_sep[16] = {};  // downward pass in its own kernel (k_b_anterp)
+  int up_tb[16] = {};
   double2 *X = nullptr, *Y = nullptr, *w[12] = {};
   void* Yf = nullptr;
   double* part = nullptr;
@@ -1498,6 +1615,87 @@ static void launch_leaf(Batch* bt, bool up, const double2* X) {
   }
 }
 
+// Tile shapes the autotuner chooses from, per level: the theta block of the
+// upward pass and the (sources per group, columns per lane) of the far kernel.
+constexpr int kUpBlocks[3] = {4, 8, 16};
+// (sources per group, columns per lane, column chunks outermost, transfer
+// row in LDS)
+constexpr int kNFar = 12;
+constexpr int kFarVariants[kNFar][4] = {{4, 2, 0, 0}, {2, 2, 0, 0}, {4, 1, 0, 0}, {2, 1, 0, 0},
+                                        {4, 2, 1, 0}, {2, 2, 1, 0}, {4, 1, 1, 0}, {2, 1, 1, 0},
+                                        {4, 2, 0, 1}, {2, 2, 0, 1}, {4, 1, 0, 1}, {4, 1, 1, 1}};
+
+template <typename S, int TB>
+static void launch_up_tb(Batch* bt, int n) {
+  typedef typename Cx<S>::T T2;
+  Handle* h = bt->h;
+  BatchLevel& c = bt->lv[n];
+  BatchLevel& p = bt->lv[n - 1];
+  const LevelPlan& pp = h->plan.lv[n - 1];
+  const LevelPlan& cp = h->plan.lv[n];
+  const int ncol = 2 * bt->R, nchunk = ncol / kWave;
+  const long long waves = (long long)p.nb * pp.nph * ((pp.nth + TB - 1) / TB) * nchunk;
+  hipLaunchKernelGGL((k_b_interp_up<S, TB>), dim3(nblk(waves, 4)), dim3(256), 0, h->stream, ncol, pp.nth, p.nk,
+                     p.nb, cp.nth, c.nb, pp.nph, p.pcu_ptr, c.oct, p.sth_idx, (const S*)p.sth_w, p.sph_idx,
+                     (const S*)p.sph_w, (const T2*)p.ph, (const T2*)c.C, (T2*)p.C);
+}
+
+// C_{n-1} from C_n (the upward pass into level n-1) with theta block tb
+template <typename S>
+static void launch_up(Batch* bt, int n, int tb) {
+  if (tb == 4)
+    launch_up_tb<S, 4>(bt, n);
+  else if (tb == 16)
+    launch_up_tb<S, 16>(bt, n);
+  else
+    launch_up_tb<S, 8>(bt, n);
+}
+
+template <typename S, int SU, int CPL, bool TL>
+static void launch_far_v(Batch* bt, int n, int ns, int cout, bool split) {
+  typedef typename Cx<S>::T T2;
+  Handle* h = bt->h;
+  BatchLevel& d = bt->lv[n];
+  const BatchLevel* pl = n > 3 ? &bt->lv[n - 1] : nullptr;
+  // split: the translation alone here, the downward pass by k_b_anterp
+  const T2* Ac = (pl && !split) ? (const T2*)pl->A : nullptr;
+  const int ncol = 2 * bt->R, fnchunk = ncol / (kWave * CPL);
+  const int wpx = nblk(nblk(d.nP, ns), 4) * fnchunk;
+  const long long grid = 8LL * nblk(d.nk, 8 / ns) * wpx;
+  hipLaunchKernelGGL((k_b_far<S, SU, CPL, TL>), dim3((unsigned)grid), dim3(256), 0, h->stream, d.nk, d.nP, fnchunk,
+                     wpx, ns, cout, d.nb, ncol, pl ? pl->nb : 0, d.nwt, d.pc_ptr, d.bcd, d.boct, d.ps_ptr, d.ps_ent,
+                     (const T2*)d.Tk, (const T2*)d.C, d.wtp_ptr, d.wtp_row, (const T2*)d.wp, Ac, (T2*)d.A);
+  if (pl && split) {
+    const int achunk = ncol / kWave;
+    const long long ag = (long long)d.nP * achunk * nblk(d.nk, kAntK);
+    hipLaunchKernelGGL(k_b_anterp<S>, dim3((unsigned)ag), dim3(256), 0, h->stream, d.nk, d.nP, achunk, d.nb, ncol,
+                       pl->nb, d.nwt, d.pc_ptr, d.boct, d.wtp_ptr, d.wtp_row, (const T2*)d.wp, (const T2*)pl->A,
+                       (T2*)d.A);
+  }
+}
+
+// A_n (translation + downward pass at level n) with far variant v, ns
+// parent ranges, and the downward pass fused or in its own kernel (split)
+template <typename S>
+static void launch_far(Batch* bt, int n, int v, int ns, bool split) {
+  const int* f = kFarVariants[v];
+  const int cout = f[2], shape = f[0] * 10 + f[1];
+  if (f[3]) {
+    switch (shape) {
+      case 22: launch_far_v<S, 2, 2, true>(bt, n, ns, cout, split); break;
+      case 41: launch_far_v<S, 4, 1, true>(bt, n, ns, cout, split); break;
+      default: launch_far_v<S, 4, 2, true>(bt, n, ns, cout, split); break;
+    }
+    return;
+  }
+  switch (shape) {
+    case 22: launch_far_v<S, 2, 2, false>(bt, n, ns, cout, split); break;
+    case 41: launch_far_v<S, 4, 1, false>(bt, n, ns, cout, split); break;
+    case 21: launch_far_v<S, 2, 1, false>(bt, n, ns, cout, split); break;
+    default: launch_far_v<S, 4, 2, false>(bt, n, ns, cout, split); break;
+  }
+}
+
 template <typename S>
 static int batch_matvec_t(Batch* bt, const double2* X, double2* Y) {
   typedef typename Cx<S>::T T2;
@@ -1510,27 +1708,9 @@ static int batch_matvec_t(Batch* bt, const double2* X, double2* Y) {
   else
     launch_leaf<S, 128>(bt, true, X);
   const int ncol = 2 * R, nchunk = ncol / kWave;
-  for (int n = L; n >= 4; --n) {
-    BatchLevel& c = bt->lv[n];
-    BatchLevel& p = bt->lv[n - 1];
-    const LevelPlan& pp = h->plan.lv[n - 1];
-    const LevelPlan& cp = h->plan.lv[n];
-    const long long waves = (long long)p.nb * pp.nph * ((pp.nth + kThBlock - 1) / kThBlock) * nchunk;
-    hipLaunchKernelGGL(k_b_interp_up<S>, dim3(nblk(waves, 4)), dim3(256), 0, st, ncol, pp.nth, p.nk, p.nb, cp.nth,
-                       c.nb, pp.nph, p.pcu_ptr, c.oct, p.sth_idx, (const S*)p.sth_w, p.sph_idx, (const S*)p.sph_w,
-                       (const T2*)p.ph, (const T2*)c.C, (T2*)p.C);
-  }
-  for (int n = 3; n <= L; ++n) {
-    BatchLevel& d = bt->lv[n];
-    const BatchLevel* pl = n > 3 ? &bt->lv[n - 1] : nullptr;
-    const int ns = bt->far_split[n], fnchunk = ncol / (kWave * kFarCpl);
-    const int wpx = nblk(nblk(d.nP, ns), 4) * fnchunk;
-    const long long grid = 8LL * nblk(d.nk, 8 / ns) * wpx;
-    hipLaunchKernelGGL(k_b_far<S>, dim3((unsigned)grid), dim3(256), 0, st, d.nk, d.nP, fnchunk, wpx, ns,
-                       d.nb, ncol, pl ? pl->nb : 0, d.nwt, d.pc_ptr, d.bcd, d.boct, d.ps_ptr, d.ps_ent,
-                       (const T2*)d.Tk, (const T2*)d.C, d.wtp_ptr, d.wtp_row, (const T2*)d.wp,
-                       pl ? (const T2*)pl->A : nullptr, (T2*)d.A);
-  }
+  (void)nchunk;
+  for (int n = L; n >= 4; --n) launch_up<S>(bt, n, bt->up_tb[n]);
+  for (int n = 3; n <= L; ++n) launch_far<S>(bt, n, bt->far_var[n], bt->far_split[n], bt->far_sep[n] != 0);
   if (R == 64)
     launch_leaf<S, 64>(bt, false, nullptr);
   else
@@ -1817,6 +1997,9 @@ static int setup_far(Batch* bt) {
     }
     b.nP = P.lv[n - 1].nb;
     bt->far_split[n] = b.nP >= 64 ? 8 : 1;
+    bt->far_var[n] = 0;
+    bt->far_sep[n] = 0;
+    bt->up_tb[n] = kThBlock;
     int* d_code_tr = nullptr;
     if ((rc = upload_b(bt, &b.pc_ptr, P.lv[n - 1].child_ptr)) || (rc = upload_b(bt, &b.bcd, fb.bcd)) ||
         (rc = upload_b(bt, &b.boct, fb.bo)) || (rc = upload_b(bt, &b.ps_ptr, fb.ps_ptr)) ||
@@ -2331,6 +2514,113 @@ int lw_bench_batch(lw_batch* ob, int reps, double* matvec_ms, double* lusol_ms) 
   }
   (void)hipEventDestroy(e0);
   (void)hipEventDestroy(e1);
+  return LW_OK;
+}
+
+int lw_batch_set_tiles(lw_batch* ob, const int* config) {
+  using namespace lw;
+  if (!ob || !config) return LW_ERR_ARG;
+  Batch* bt = &ob->b;
+  const int L = bt->h->L;
+  int up[16] = {}, fv[16] = {}, fs[16] = {}, fp[16] = {};
+  for (int n = 3; n <= L; ++n) {
+    const int tb = config[3 * n], sc = config[3 * n + 1] % 10000, ns = config[3 * n + 2];
+    const int sep = config[3 * n + 1] / 10000;
+    int v = -1;
+    for (int i = 0; i < kNFar; ++i)
+      if (sc == kFarVariants[i][3] * 1000 + kFarVariants[i][2] * 100 + kFarVariants[i][0] * 10 + kFarVariants[i][1])
+        v = i;
+    const bool tb_ok = n < 4 ? tb == 0 : (tb == 4 || tb == 8 || tb == 16);
+    if (!tb_ok || v < 0 || !(ns == 1 || ns == 2 || ns == 4 || ns == 8) || sep < 0 || sep > (n > 3 ? 1 : 0)) {
+      set_error("lw_batch_set_tiles: level " + std::to_string(n) +
+                ": theta block 4, 8 or 16 (0 at level 3); a far shape of the list; ranges 1, 2, 4 or 8");
+      return LW_ERR_ARG;
+    }
+    up[n] = n >= 4 ? tb : kThBlock, fv[n] = v, fs[n] = ns, fp[n] = sep;
+  }
+  LW_HIP(hipStreamSynchronize(bt->h->stream));
+  for (auto& g : bt->iter_graph)
+    if (g) {
+      LW_HIP(hipGraphExecDestroy(g));
+      g = nullptr;
+    }
+  bt->cur_graph = nullptr;
+  for (int n = 3; n <= L; ++n)
+    bt->up_tb[n] = up[n], bt->far_var[n] = fv[n], bt->far_split[n] = fs[n], bt->far_sep[n] = fp[n];
+  return LW_OK;
+}
+
+int lw_batch_tune(lw_batch* ob, int reps, double* ms, int* config) {
+  using namespace lw;
+  if (!ob || reps < 1) return LW_ERR_ARG;
+  Batch* bt = &ob->b;
+  Handle* h = bt->h;
+  hipStream_t st = h->stream;
+  const int L = h->L;
+  // a solve in progress would continue with other kernels in its captured
+  // graph: the graphs are recaptured by the next lw_batch_cgs / _start
+  LW_HIP(hipStreamSynchronize(st));
+  for (auto& g : bt->iter_graph)
+    if (g) {
+      LW_HIP(hipGraphExecDestroy(g));
+      g = nullptr;
+    }
+  bt->cur_graph = nullptr;
+  double before = 0.0, after = 0.0;
+  int rc = lw_bench_batch(ob, reps, &before, nullptr);  // also fills X and every level's fields
+  if (rc) return rc;
+  hipEvent_t e0, e1;
+  LW_HIP(hipEventCreate(&e0));
+  LW_HIP(hipEventCreate(&e1));
+  // mean time of `reps` launches of f, after one untimed launch
+  auto timed = [&](auto f, double* t) -> int {
+    f();
+    LW_HIP(hipEventRecord(e0, st));
+    for (int k = 0; k < reps; ++k) f();
+    LW_HIP(hipEventRecord(e1, st));
+    LW_HIP(hipEventSynchronize(e1));
+    LW_HIP(hipGetLastError());
+    float v = 0.f;
+    LW_HIP(hipEventElapsedTime(&v, e0, e1));
+    *t = (double)v / reps;
+    return LW_OK;
+  };
+  // every variant computes the same values: the choice changes only the time
+  for (int n = L; n >= 4 && !rc; --n) {
+    double best = 1e30;
+    for (int tb : kUpBlocks) {
+      double t = 0.0;
+      rc = timed([&] { bt->fp64 ? launch_up<double>(bt, n, tb) : launch_up<float>(bt, n, tb); }, &t);
+      if (rc) break;
+      if (t < best) best = t, bt->up_tb[n] = tb;
+    }
+  }
+  for (int n = 3; n <= L && !rc; ++n) {
+    double best = 1e30;
+    for (int sep = 0; sep < (n > 3 ? 2 : 1) && !rc; ++sep)
+      for (int v = 0; v < kNFar && !rc; ++v)
+        for (int ns : {1, 2, 8}) {
+          if (ns > 1 && bt->lv[n].nP < 8 * ns) continue;  // ranges of a few parents only
+          double t = 0.0;
+          rc = timed([&] { bt->fp64 ? launch_far<double>(bt, n, v, ns, sep) : launch_far<float>(bt, n, v, ns, sep); },
+                     &t);
+          if (rc) break;
+          if (t < best) best = t, bt->far_var[n] = v, bt->far_split[n] = ns, bt->far_sep[n] = sep;
+        }
+  }
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if (rc) return rc;
+  if ((rc = lw_bench_batch(ob, reps, &after, nullptr))) return rc;
+  if (ms) ms[0] = before, ms[1] = after;
+  if (config)
+    for (int n = 0; n <= L; ++n) {
+      const bool up = n >= 4, far = n >= 3;
+      config[3 * n] = up ? bt->up_tb[n] : 0;
+      const int* fv = kFarVariants[bt->far_var[n]];
+      config[3 * n + 1] = far ? bt->far_sep[n] * 10000 + fv[3] * 1000 + fv[2] * 100 + fv[0] * 10 + fv[1] : 0;
+      config[3 * n + 2] = far ? bt->far_split[n] : 0;
+    }
   return LW_OK;
 }
 

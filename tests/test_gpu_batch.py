@@ -267,3 +267,33 @@ def test_fp32_far_field_with_fp64_true_residuals():
                 bt.close()
     finally:
         op.close()
+
+
+@pytest.mark.parametrize("fp64", [False, True])
+def test_tile_shapes_give_the_same_bits(fp64):
+    # every tile shape of the upward pass and of the fused translation adds
+    # the same terms in the same order: the products agree bit for bit
+    mesh, op, bt = setup(64, fp64=fp64)
+    L = op.nlevels
+    rng = np.random.default_rng(21)
+    X = rng.standard_normal((mesh.ne, 64)) + 1j * rng.standard_normal((mesh.ne, 64))
+    default = {n: (0 if n == 3 else 8, (4, 2), 1) for n in range(3, L + 1)}
+    bt.set_tiles(default)
+    Y0 = bt.matvec(X)
+    for tb, far, ns in [(4, (2, 2), 8), (16, (4, 1), 2), (8, (2, 1), 1), (4, (4, 1, 1), 8), (8, (2, 2, 1), 2), (4, (4, 2, 0, 1), 8), (16, (4, 1, 1, 1), 1), (8, (4, 2, 0, 0, 1), 8), (4, (4, 1, 1, 1, 1), 2)]:
+        # level 3 has no parent level: no separate downward pass there
+        bt.set_tiles({n: (0 if n == 3 else tb, far[:4] if n == 3 else far, ns) for n in range(3, L + 1)})
+        assert np.array_equal(bt.matvec(X), Y0), (tb, far, ns)
+    # a solve with the default shapes, then the tuner, then the same solve:
+    # the recaptured iteration graph runs the tuned kernels, with the same bits
+    b = op.plane_wave([0, 0, -1.0], [1, 0, 0])
+    B = b[:, None] * (1 + 0.01j * np.arange(64))[None, :]
+    Xa, ita, rra, _ = bt.cgs(B, errmax=1e-12, use_prec=False, maxit=40)
+    before, after, levels = bt.tune(reps=1)
+    print(f"[tiles fp64={fp64}] {before:.3f} -> {after:.3f} ms, {levels}")
+    assert set(levels) == set(range(3, L + 1))
+    assert np.array_equal(bt.matvec(X), Y0)
+    Xb, itb, rrb, _ = bt.cgs(B, errmax=1e-12, use_prec=False, maxit=40)
+    assert np.array_equal(Xa, Xb) and np.array_equal(ita, itb) and np.array_equal(rra, rrb)
+    with pytest.raises(lw.LywellError):
+        bt.set_tiles({n: (5, (4, 2), 1) for n in range(3, L + 1)})
