@@ -454,37 +454,58 @@ __device__ __forceinline__ void far_parent(int k, int P, int col, const typename
 }
 
 template <typename S, int SU, int CPL, bool TL>
-__global__ void __launch_bounds__(256) k_b_far(int nk, int nP, int nchunk, int wpx, int nsplit, int chunk_outer,
+__global__ void __launch_bounds__(256) k_b_far(int nk, int nP, int nchunk, int wpx, int nsplit, int order,
                                                LW_FAR_PARAMS) {
   typedef typename Cx<S>::T T2;
-  // Workgroup map: within a workgroup one sample k and one column chunk,
-  // 4 waves = 4 consecutive parents.  The parents are cut into nsplit
-  // contiguous ranges (nsplit divides 8) and XCD x owns range x % nsplit for
-  // the samples k = x / nsplit mod 8 / nsplit, so that its L2 holds the
-  // sources of a box range (1/nsplit of a k-slice plus a halo) for the few
-  // samples it has in flight.  nsplit = 1 gives each XCD whole k-slices,
-  // nsplit = 8 a range of every k-slice.  The column chunks of a sample
-  // follow each other (chunk_outer = 0), or each chunk runs over all samples
-  // before the next (chunk_outer = 1), which halves the rows in flight.
-  // TL: the workgroup's transfer row Tk[k][.] is staged in LDS and read from
-  // there (broadcast reads into vector registers) instead of through scalar
-  // loads from the table.
+  // Workgroup map (order 0 and 1): within a workgroup one sample k and one
+  // column chunk, 4 waves = 4 consecutive parents.  The parents are cut into
+  // nsplit contiguous ranges (nsplit divides 8) and XCD x owns range
+  // x % nsplit for the samples k = x / nsplit mod 8 / nsplit, so that its L2
+  // holds the sources of a box range (1/nsplit of a k-slice plus a halo) for
+  // the few samples it has in flight.  nsplit = 1 gives each XCD whole
+  // k-slices, nsplit = 8 a range of every k-slice.  The column chunks of a
+  // sample follow each other (order 0), or each chunk runs over all samples
+  // before the next (order 1), which halves the rows in flight.
+  // Order 2: 4 waves = 4 consecutive samples of one parent, whose downward
+  // stencils share most of their parent rows A_{n-1}[i][P] (read once per
+  // workgroup through L1 instead of once per wave); the parents of a range
+  // follow each other for a block of samples.
+  // TL (orders 0 and 1): the workgroup's transfer row Tk[k][.] is staged in
+  // LDS and read from there (broadcast reads into vector registers) instead
+  // of through scalar loads from the table.
+  if constexpr (TL) {
+    // not a TL shape (each wave of order 2 has its own sample); never
+    // launched, and returning here keeps the transfer-row pointer below an
+    // LDS pointer on every path
+    if (order == 2) return;
+  }
   const int g = blockIdx.x, x = g % 8, j = g / 8;
   const int kstep = 8 / nsplit, pr = x % nsplit;
   const int nPx = (nP + nsplit - 1) / nsplit, nPBx = wpx / nchunk;
-  int k, chunk, pb;
-  if (chunk_outer) {
-    const int per = ((nk + kstep - 1) / kstep) * nPBx;
-    chunk = j / per;
-    k = ((j % per) / nPBx) * kstep + x / nsplit;
-    pb = (j % per) % nPBx;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
+  int k, chunk, P, P1 = min((pr + 1) * nPx, nP);
+  if (order == 2) {
+    const int per = nPx * nchunk;
+    const int kb = (j / per) * kstep + x / nsplit;
+    chunk = (j % per) / nPx;
+    P = pr * nPx + (j % per) % nPx;
+    k = kb * 4 + wave;
+    if (P >= P1 || k >= nk) return;
   } else {
-    k = (j / wpx) * kstep + x / nsplit;
-    chunk = (j % wpx) / nPBx;
-    pb = (j % wpx) % nPBx;
+    int pb;
+    if (order == 1) {
+      const int per = ((nk + kstep - 1) / kstep) * nPBx;
+      chunk = j / per;
+      k = ((j % per) / nPBx) * kstep + x / nsplit;
+      pb = (j % per) % nPBx;
+    } else {
+      k = (j / wpx) * kstep + x / nsplit;
+      chunk = (j % wpx) / nPBx;
+      pb = (j % wpx) % nPBx;
+    }
+    if (k >= nk) return;  // uniform over the workgroup
+    P = pr * nPx + pb * 4 + wave;
   }
-  const int P0 = pr * nPx + pb * 4, P1 = min((pr + 1) * nPx, nP);
-  if (k >= nk) return;  // uniform over the workgroup
   const T2* Tkk = Tk + (size_t)k * kOffCodes;
   if constexpr (TL) {
     __shared__ T2 tsh[kOffCodes];
@@ -492,7 +513,6 @@ __global__ void __launch_bounds__(256) k_b_far(int nk, int nP, int nchunk, int w
     __syncthreads();
     Tkk = tsh;
   }
-  const int P = P0 + __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
   if (P >= P1) return;
   const int col = chunk * kWave * CPL + (threadIdx.x % kWave);
   switch (pc_ptr[P + 1] - pc_ptr[P]) {
@@ -1618,12 +1638,13 @@ static void launch_leaf(Batch* bt, bool up, const double2* X) {
 // Tile shapes the autotuner chooses from, per level: the theta block of the
 // upward pass and the (sources per group, columns per lane) of the far kernel.
 constexpr int kUpBlocks[3] = {4, 8, 16};
-// (sources per group, columns per lane, column chunks outermost, transfer
+// (sources per group, columns per lane, workgroup order (k_b_far), transfer
 // row in LDS)
-constexpr int kNFar = 12;
+constexpr int kNFar = 16;
 constexpr int kFarVariants[kNFar][4] = {{4, 2, 0, 0}, {2, 2, 0, 0}, {4, 1, 0, 0}, {2, 1, 0, 0},
                                         {4, 2, 1, 0}, {2, 2, 1, 0}, {4, 1, 1, 0}, {2, 1, 1, 0},
-                                        {4, 2, 0, 1}, {2, 2, 0, 1}, {4, 1, 0, 1}, {4, 1, 1, 1}};
+                                        {4, 2, 0, 1}, {2, 2, 0, 1}, {4, 1, 0, 1}, {4, 1, 1, 1},
+                                        {4, 2, 2, 0}, {2, 2, 2, 0}, {4, 1, 2, 0}, {2, 1, 2, 0}};
 
 template <typename S, int TB>
 static void launch_up_tb(Batch* bt, int n) {
@@ -1652,7 +1673,7 @@ static void launch_up(Batch* bt, int n, int tb) {
 }
 
 template <typename S, int SU, int CPL, bool TL>
-static void launch_far_v(Batch* bt, int n, int ns, int cout, bool split) {
+static void launch_far_v(Batch* bt, int n, int ns, int order, bool split) {
   typedef typename Cx<S>::T T2;
   Handle* h = bt->h;
   BatchLevel& d = bt->lv[n];
@@ -1661,9 +1682,10 @@ static void launch_far_v(Batch* bt, int n, int ns, int cout, bool split) {
   const T2* Ac = (pl && !split) ? (const T2*)pl->A : nullptr;
   const int ncol = 2 * bt->R, fnchunk = ncol / (kWave * CPL);
   const int wpx = nblk(nblk(d.nP, ns), 4) * fnchunk;
-  const long long grid = 8LL * nblk(d.nk, 8 / ns) * wpx;
+  const long long grid = order == 2 ? 8LL * nblk(nblk(d.nk, 4), 8 / ns) * nblk(d.nP, ns) * fnchunk
+                                    : 8LL * nblk(d.nk, 8 / ns) * wpx;
   hipLaunchKernelGGL((k_b_far<S, SU, CPL, TL>), dim3((unsigned)grid), dim3(256), 0, h->stream, d.nk, d.nP, fnchunk,
-                     wpx, ns, cout, d.nb, ncol, pl ? pl->nb : 0, d.nwt, d.pc_ptr, d.bcd, d.boct, d.ps_ptr, d.ps_ent,
+                     wpx, ns, order, d.nb, ncol, pl ? pl->nb : 0, d.nwt, d.pc_ptr, d.bcd, d.boct, d.ps_ptr, d.ps_ent,
                      (const T2*)d.Tk, (const T2*)d.C, d.wtp_ptr, d.wtp_row, (const T2*)d.wp, Ac, (T2*)d.A);
   if (pl && split) {
     const int achunk = ncol / kWave;
@@ -1679,7 +1701,7 @@ static void launch_far_v(Batch* bt, int n, int ns, int cout, bool split) {
 template <typename S>
 static void launch_far(Batch* bt, int n, int v, int ns, bool split) {
   const int* f = kFarVariants[v];
-  const int cout = f[2], shape = f[0] * 10 + f[1];
+  const int cout = f[2], shape = f[0] * 10 + f[1];  // cout: the workgroup order
   if (f[3]) {
     switch (shape) {
       case 22: launch_far_v<S, 2, 2, true>(bt, n, ns, cout, split); break;

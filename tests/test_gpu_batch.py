@@ -280,7 +280,7 @@ def test_tile_shapes_give_the_same_bits(fp64):
     default = {n: (0 if n == 3 else 8, (4, 2), 1) for n in range(3, L + 1)}
     bt.set_tiles(default)
     Y0 = bt.matvec(X)
-    for tb, far, ns in [(4, (2, 2), 8), (16, (4, 1), 2), (8, (2, 1), 1), (4, (4, 1, 1), 8), (8, (2, 2, 1), 2), (4, (4, 2, 0, 1), 8), (16, (4, 1, 1, 1), 1), (8, (4, 2, 0, 0, 1), 8), (4, (4, 1, 1, 1, 1), 2)]:
+    for tb, far, ns in [(4, (2, 2), 8), (16, (4, 1), 2), (8, (2, 1), 1), (4, (4, 1, 1), 8), (8, (2, 2, 1), 2), (4, (4, 2, 0, 1), 8), (16, (4, 1, 1, 1), 1), (8, (4, 2, 0, 0, 1), 8), (4, (4, 1, 1, 1, 1), 2), (8, (4, 2, 2), 1), (4, (2, 1, 2, 0, 1), 8)]:
         # level 3 has no parent level: no separate downward pass there
         bt.set_tiles({n: (0 if n == 3 else tb, far[:4] if n == 3 else far, ns) for n in range(3, L + 1)})
         assert np.array_equal(bt.matvec(X), Y0), (tb, far, ns)
