@@ -45,6 +45,20 @@ def log(*a):
     print(*a, flush=True)
 
 
+# mean edge of the icosphere of subdivision s on the unit sphere: 0.1507 / 2^(s-3)
+_EDGE3 = 0.1507
+
+
+def resolved_subdiv(size, subdivs, max_edge=0.1):
+    """The smallest subdivision in `subdivs` whose icosphere, scaled to the
+    largest semi-axis `size` (in wavelengths), has a mean edge of at most
+    `max_edge` wavelengths (lambda/10 by default), or the largest one."""
+    for s in sorted(subdivs):
+        if _EDGE3 * size / 2 ** (s - 3) <= max_edge:
+            return s
+    return max(subdivs)
+
+
 def random_body(rng, subdiv, amin, amax, kind):
     if kind == "ellipsoid":
         ax = rng.uniform(amin, amax, 3)

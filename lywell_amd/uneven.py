@@ -1,9 +1,11 @@
 """An uneven batch of bodies solved side by side on one GPU.
 
-Seeded closed PEC bodies (icosphere subdivision drawn from `subdivs`, radius
-uniform in [rmin, rmax] wavelengths, radially perturbed by real spherical
-harmonics of degree 2-4 with 5% total amplitude), one plane wave each with a
-uniformly random direction and polarisation.  Each body has its own MLFMA
+Seeded closed PEC bodies (radius uniform in [rmin, rmax] wavelengths,
+radially perturbed by real spherical harmonics of degree 2-4 with 5% total
+amplitude, meshed with the smallest icosphere subdivision from `subdivs` whose
+mean edge is at most lambda/10), one plane wave each with a uniformly random
+direction and polarisation.  The unknown counts follow the sizes: 1,920,
+7,680 or 30,720 RWG.  Each body has its own MLFMA
 operator (its own octree, level sizes and device stream); the bodies' CGSold
 solves run concurrently from a pool of host threads, each driving one body's
 solve on its own stream, so the small bodies' kernels share the GPU instead
@@ -16,7 +18,7 @@ import time
 import numpy as np
 
 import lywell_amd as lw
-from lywell_amd.guess_train import random_body
+from lywell_amd.guess_train import random_body, resolved_subdiv
 
 
 def random_wave(rng):
@@ -36,8 +38,11 @@ def bodies(n=64, seed=7, subdivs=(3, 4, 5), rmin=0.5, rmax=2.5, ladomax=0.2):
     rng = np.random.default_rng(seed)
     out = []
     for i in range(n):
-        sub = int(rng.choice(subdivs))
-        mesh, meta = random_body(rng, sub, rmin, rmax, "perturbed_sphere")
+        radius = rng.uniform(rmin, rmax)
+        # the radius drawn first fixes the mesh density; random_body then
+        # draws the perturbation for that radius (amin = amax)
+        sub = resolved_subdiv(radius * 1.05, subdivs)
+        mesh, meta = random_body(rng, sub, radius, radius, "perturbed_sphere")
         op = lw.MLFMA(mesh, lam=1.0, ladomax=ladomax)
         d, p = random_wave(rng)
         meta.update({"subdiv": sub, "ne": mesh.ne, "levels": op.nlevels, "direction": d.tolist(),

@@ -32,3 +32,20 @@ def test_uneven_batch_concurrent_matches_sequential():
         x, it, rr, cv = op.cgs(b, errmax=1e-4, use_prec=False, maxit=6000)
         assert cv and np.linalg.norm(op.matvec(x) - b) / np.linalg.norm(b) < 3e-4
         op.close()
+
+
+def test_uneven_batch_with_a_30720_unknown_body():
+    # bodies meshed at lambda/10: the radius fixes the subdivision, so the
+    # batch spans 1,920 to 30,720 unknowns
+    batch = uneven.bodies(4, seed=13, rmin=0.55, rmax=2.4)
+    ne = [m["ne"] for _, _, m in batch]
+    print("[uneven 30720] unknowns", ne, [round(m["radius"], 3) for _, _, m in batch], flush=True)
+    assert max(ne) == 30720 and min(ne) < 30720
+    t, res = uneven.solve(batch, errmax=1e-4, maxit=20000, workers=4)
+    print(f"[uneven 30720] concurrent {t:.2f}s, iterations {[r[0] for r in res]}", flush=True)
+    for (op, b, meta), (it, rr, cv) in zip(batch, res):
+        assert cv and rr < 1e-4
+        x, it2, rr2, cv2 = op.cgs(b, errmax=1e-4, use_prec=False, maxit=20000)
+        assert it2 == it
+        assert np.linalg.norm(op.matvec(x) - b) / np.linalg.norm(b) < 3e-4
+        op.close()
