@@ -171,17 +171,18 @@ def main():
     # dominant kernel of a solve: per CGS iteration 2 matvecs + 2 lusol
     it_mean = float(np.mean(iters))
     per_iter_ms = wall * 1e3 / max(1.0, float(np.sum(iters)))
-    stage_names = ["aggregation", "interp_up", "translation", "interp_down", "receive_near"]
+    stage_names = ["aggregation", "interp_up", "translation", "interp_down", "near", "receive"]
     cand = {name: 2 * t for name, t in zip(stage_names, stages)}
     if use_prec:
         cand["lusol"] = 2 * lus_ms
     dom = max(cand, key=cand.get)
     nnz = op.nnz
     sbytes = op.matvec_stage_bytes()
-    kernel_of = {"aggregation": "k_aggregate", "interp_up": "k_interp_up", "translation": "k_translate_parent",
-                 "interp_down": "k_interp_down", "receive_near": "k_receive_near", "lusol": "k_lusol"}
+    kernel_of = {"aggregation": "k_aggregate_ph", "interp_up": "k_interp_up", "translation": "k_translate_parent",
+                 "interp_down": "k_interp_down", "near": "k_near_part", "receive": "k_receive_ph",
+                 "lusol": "k_lusol_flow"}
     launches = {"aggregation": 1, "interp_up": op.nlevels - 3, "translation": op.nlevels - 2,
-                "interp_down": op.nlevels - 3, "receive_near": 1, "lusol": 1}
+                "interp_down": op.nlevels - 3, "near": 1, "receive": 1, "lusol": 1}
     if dom == "lusol":
         # forward + backward sweep over the MSR factor: complex(4) values and
         # int32 columns once, the row pointers, y read, x written and gathered

@@ -186,26 +186,30 @@ int lw_rcs_sigma_noalloc(lw_handle *h, int n, const double *theta, const double 
 /* ------------------------------------------------------------------ measurement */
 
 /* Time `reps` device-resident matvecs (after one warm-up) with HIP events on
- * the handle's stream.  stage_ms (may be NULL) receives the mean time of each
- * stage: [0] aggregation, [1] interpolation up, [2] translation,
- * [3] anterpolation down, [4] receive + near field.  Returns mean ms in *ms. */
+ * the handle's stream.  *ms receives the mean time of a matvec as the solver
+ * runs it (the near field and the translations of levels 4..L on side
+ * streams beside the rest).  stage_ms (may be NULL) receives the mean time
+ * of each stage when the stages run one after the other: [0] aggregation,
+ * [1] interpolation up, [2] translation, [3] anterpolation down, [4] near
+ * field, [5] receive (0 in [4] where the receive and the near field are one
+ * kernel). */
 int lw_bench_matvec(lw_handle *h, int reps, double *ms, double *stage_ms);
 /* Time `reps` ILU0 triangular solves on device-resident vectors. */
 int lw_bench_lusol(lw_handle *h, int reps, double *ms);
-/* Algorithmic HBM bytes of one matvec in this library's layout: the
- * aggregation's phase rows (one exp(j k khat.(c_T - C)) row per leaf box and
- * triangle of its bases) and RWG vectors, the receive's pattern array, campo
- * arrays 6x, transfer tables once, near CSR, x and y; and of the dominant
- * kernels.  (The reference's layout reads the pattern array twice instead of
- * phase rows + patterns.) */
+/* Algorithmic HBM bytes of one matvec in this library's layout (the sum of
+ * lw_matvec_stage_bytes): the phase rows (one exp(j k khat.(c_T - C)) row per
+ * leaf box and triangle of its bases) and RWG vectors, read by the
+ * aggregation and by the receive, campo arrays 6x, transfer tables once, near
+ * CSR, x and y; and of the aggregation and of the near field + receive.
+ * (The reference's layout reads the pattern array twice instead.) */
 int lw_matvec_bytes(lw_handle *h, double *total, double *aggregation, double *receive_near);
 /* Mean time in ms of a device-to-device copy of `bytes` bytes (a multiple of
  * 16) by the library's copy kernel on the handle's stream: the measured
  * bandwidth reference of the roofline figures. */
 int lw_bench_copy(lw_handle *h, long long bytes, int reps, double *ms);
-/* Algorithmic bytes of each matvec stage, in the order of lw_bench_matvec's
- * stage_ms: aggregation, upward passes, translations, downward passes,
- * receive + near field. */
+/* Algorithmic bytes of each matvec stage (6 values), in the order of
+ * lw_bench_matvec's stage_ms: aggregation, upward passes, translations,
+ * downward passes, near field, receive. */
 int lw_matvec_stage_bytes(lw_handle *h, double *stage_bytes);
 
 /* ---- multi-RHS form ------------------------------------------------------

@@ -341,7 +341,7 @@ class MLFMA:
 
     def bench_matvec(self, reps=10):
         ms = _C.c_double()
-        st = _np.zeros(5)
+        st = _np.zeros(6)
         _check(lib().lw_bench_matvec(self.h, int(reps), _C.byref(ms), _P(st)))
         return ms.value, st
 
@@ -351,7 +351,7 @@ class MLFMA:
         return ms.value
 
     def matvec_stage_bytes(self):
-        sb = _np.zeros(5)
+        sb = _np.zeros(6)
         _check(lib().lw_matvec_stage_bytes(self.h, _P(sb)))
         return sb
 

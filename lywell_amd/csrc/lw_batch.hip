@@ -2103,10 +2103,10 @@ int setup_far_parent(Handle* h) {
   return LW_OK;
 }
 
-void launch_translate_parent(Handle* h, int n) {
+void launch_translate_parent(Handle* h, int n, hipStream_t st) {
   DevLevel& d = h->lv[n];
   const int nP = h->plan.lv[n - 1].nb, nchk = nblk(d.nk, kWave);
-  hipLaunchKernelGGL(k_translate_parent, dim3(nP * nchk), dim3(256), 0, h->stream, d.nk, nchk,
+  hipLaunchKernelGGL(k_translate_parent, dim3(nP * nchk), dim3(256), 0, st, d.nk, nchk,
                      d.pc_ptr, d.bcd, d.ps_ptr, d.ps_ent, d.Tc, d.pes, d.C, d.B);
 }
 

@@ -67,6 +67,23 @@ __device__ __forceinline__ double wave_sum(double v) {
   return v;
 }
 
+// Sum over the 16 lanes of a DPP row (lanes 16q..16q+15), every lane of the
+// row receiving it: rotations by 8 and 4 within the row, then the quad
+// permutations [2,3,0,1] and [1,0,3,2].  All 16 lanes must be active.
+template <int CTRL>
+__device__ __forceinline__ double dpp_mov(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double row16_sum(double v) {
+  v += dpp_mov<0x128>(v);  // row_ror:8
+  v += dpp_mov<0x124>(v);  // row_ror:4
+  v += dpp_mov<0x4E>(v);   // quad_perm [2,3,0,1]
+  v += dpp_mov<0xB1>(v);   // quad_perm [1,0,3,2]
+  return v;
+}
+
 // ---------------------------------------------------------------- matvec
 
 // Workgroups are dispatched to the 8 XCDs round-robin by linear id.  This maps
@@ -129,6 +146,81 @@ __global__ void __launch_bounds__(256) k_aggregate_ph(int nk, const int* __restr
     }
     C[((size_t)b * 2 + 0) * nk + k] = acc0;
     C[((size_t)b * 2 + 1) * nk + k] = acc1;
+  }
+}
+
+// Leaf aggregation from the phase rows, factored: with funcRAD(k,p,f) =
+// l/2 (e+(k) rho+ + e-(k) rho-) . u_p(k) (calcRad, modMLFMA.f90:669-699),
+//   campoC_L(k,p) = sum_f funcRAD(k,p,f) x(f) = u_p(k) . sum_T e_T(k) H_T,
+//   H_T = sum of l/2 x(f) rho+ over the box's bases whose plus triangle is T
+//         and of l/2 x(f) rho- over those whose minus triangle is T,
+// so each phase row of the box is read once and the pattern is never formed
+// (the terms are summed in another order than the reference's; same value
+// to rounding).  One workgroup per leaf box: H of its rows in LDS (a thread
+// per row over the bases the row serves, in slot order), then one thread
+// per sample.
+__global__ void __launch_bounds__(256) k_aggregate_fac(int nk, const int* __restrict__ box_row,
+                                                       const int* __restrict__ row_ptr,
+                                                       const int* __restrict__ row_ent,
+                                                       const int* __restrict__ slot_basis,
+                                                       const RadGeom* __restrict__ geo,
+                                                       const double* __restrict__ ath,
+                                                       const double* __restrict__ aph,
+                                                       const double2* __restrict__ tph,
+                                                       const double2* __restrict__ x, double2* __restrict__ C) {
+  extern __shared__ double2 hs[];  // [row of the box][3]
+  const int b = xcd_block(blockIdx.x, gridDim.x);
+  const int r0 = box_row[b], nr = box_row[b + 1] - r0;
+  for (int t = threadIdx.x; t < nr; t += blockDim.x) {
+    double2 h[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) h[d] = make_double2(0.0, 0.0);
+    for (int e = row_ptr[r0 + t]; e < row_ptr[r0 + t + 1]; ++e) {
+      const int v = row_ent[e], s = v >> 1;
+      const RadGeom& g = geo[s];
+      const double2 xv = x[slot_basis[s]];
+      const double hl = 0.5 * g.len;
+      const double2 xs = make_double2(xv.x * hl, xv.y * hl);
+      const double* rho = (v & 1) ? g.rm : g.rp;
+#pragma unroll
+      for (int d = 0; d < 3; ++d) {
+        h[d].x += xs.x * rho[d];
+        h[d].y += xs.y * rho[d];
+      }
+    }
+#pragma unroll
+    for (int d = 0; d < 3; ++d) hs[3 * t + d] = h[d];
+  }
+  __syncthreads();
+  for (int k = threadIdx.x; k < nk; k += blockDim.x) {
+    double2 a[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) a[d] = make_double2(0.0, 0.0);
+    constexpr int kU = 8;  // phase loads issued together
+    for (int t0 = 0; t0 < nr; t0 += kU) {
+      double2 e[kU];
+#pragma unroll
+      for (int u = 0; u < kU; ++u)
+        if (t0 + u < nr) e[u] = tph[(size_t)(r0 + t0 + u) * nk + k];
+#pragma unroll
+      for (int u = 0; u < kU; ++u)
+        if (t0 + u < nr) {
+          const double2* h = hs + 3 * (t0 + u);
+#pragma unroll
+          for (int d = 0; d < 3; ++d) a[d] = cadd(a[d], cmul(e[u], h[d]));
+        }
+    }
+    double2 c0 = make_double2(0.0, 0.0), c1 = c0;
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const double t = ath[3 * k + d], p = aph[3 * k + d];
+      c0.x += t * a[d].x;
+      c0.y += t * a[d].y;
+      c1.x += p * a[d].x;
+      c1.y += p * a[d].y;
+    }
+    C[((size_t)b * 2 + 0) * nk + k] = c0;
+    C[((size_t)b * 2 + 1) * nk + k] = c1;
   }
 }
 
@@ -316,6 +408,190 @@ __global__ void __launch_bounds__(256) k_receive_near(int ne, int nk2, const int
     } else {
       y[m] = cadd(far, nsum);
     }
+  }
+}
+
+// Near-field CSR product alone (modMLFMA.f90:1935-1942): y_near(m) = the sum
+// of row m's terms, one wavefront per row, rows in leaf-slot order (the rows
+// of a box gather the same x entries).  The lane-strided partial sums and
+// their wavefront reduction are those of k_receive_near, so its near sums
+// are the same bits.  It reads only x, so it runs beside the far field.
+template <typename V>
+__global__ void __launch_bounds__(256) k_near_part(int ne, const int* __restrict__ slot_basis,
+                                                   const int* __restrict__ ia, const int* __restrict__ ja,
+                                                   const V* __restrict__ z, const double2* __restrict__ x,
+                                                   double2* __restrict__ y_near) {
+  const int slot = xcd_block(blockIdx.x, gridDim.x) * (blockDim.x / kWave) + threadIdx.x / kWave;
+  const int lane = threadIdx.x % kWave;
+  if (slot >= ne) return;
+  const int m = slot_basis[slot];
+  const int k0 = ia[m], k1 = ia[m + 1];
+  double2 nsum = make_double2(0.0, 0.0);
+  // eight entries per lane per pass: the column and value loads of the pass,
+  // then its x gathers, are issued together; each lane adds its entries in
+  // the order of the one-entry loop (k = k0 + lane, + 64, ...)
+  constexpr int kU = 8;
+  for (int kb = k0 + lane; kb < k1; kb += kU * kWave) {
+    int j[kU];
+    V zz[kU];
+#pragma unroll
+    for (int u = 0; u < kU; ++u) {
+      const int k = kb + u * kWave;
+      j[u] = k < k1 ? ja[k] : -1;
+      if (k < k1) zz[u] = z[k];
+    }
+    double2 xv[kU];
+#pragma unroll
+    for (int u = 0; u < kU; ++u)
+      if (j[u] >= 0) xv[u] = x[j[u]];
+#pragma unroll
+    for (int u = 0; u < kU; ++u)
+      if (j[u] >= 0) nsum = cadd(nsum, cmul(xv[u], to_d2(zz[u])));
+  }
+  nsum = wave_sum(nsum);
+  if (lane == 0) y_near[m] = nsum;
+}
+
+// Near-field CSR product by leaf box: every row of a leaf box has the same
+// near columns (the box's near boxes' bases, ascending), so the box's x
+// entries are gathered once into LDS and each row (a wavefront) streams only
+// its values.  Each lane adds the entries of the one-entry loop of
+// k_near_part in its order, and the reduction is the same: the sums are the
+// same bits.
+template <typename V>
+__global__ void __launch_bounds__(256) k_near_box(const int* __restrict__ box_slot,
+                                                  const int* __restrict__ slot_basis, const int* __restrict__ ia,
+                                                  const int* __restrict__ ja, const V* __restrict__ z,
+                                                  const double2* __restrict__ x, double2* __restrict__ y_near) {
+  extern __shared__ double2 xs[];
+  const int b = xcd_block(blockIdx.x, gridDim.x);
+  const int s0 = box_slot[b], s1 = box_slot[b + 1];
+  const int c0 = ia[slot_basis[s0]], nc = ia[slot_basis[s0] + 1] - c0;
+  for (int e = threadIdx.x; e < nc; e += blockDim.x) xs[e] = x[ja[c0 + e]];
+  __syncthreads();
+  const int lane = threadIdx.x % kWave;
+  const int nw = blockDim.x / kWave;
+  constexpr int kU = 8;
+  // two rows per wavefront at a time (s and s + nw): the value loads of both
+  // are issued together
+  for (int s = s0 + (int)(threadIdx.x / kWave); s < s1; s += 2 * nw) {
+    const bool two = s + nw < s1;
+    const int m0 = slot_basis[s], m1 = two ? slot_basis[s + nw] : m0;
+    const V* zr0 = z + ia[m0];
+    const V* zr1 = z + ia[m1];
+    double2 n0 = make_double2(0.0, 0.0), n1 = n0;
+    for (int eb = lane; eb < nc; eb += kU * kWave) {
+      V z0[kU], z1[kU];
+#pragma unroll
+      for (int u = 0; u < kU; ++u)
+        if (eb + u * kWave < nc) {
+          z0[u] = zr0[eb + u * kWave];
+          z1[u] = zr1[eb + u * kWave];
+        }
+#pragma unroll
+      for (int u = 0; u < kU; ++u)
+        if (eb + u * kWave < nc) {
+          const double2 xv = xs[eb + u * kWave];
+          n0 = cadd(n0, cmul(xv, to_d2(z0[u])));
+          n1 = cadd(n1, cmul(xv, to_d2(z1[u])));
+        }
+    }
+    n0 = wave_sum(n0);
+    n1 = wave_sum(n1);
+    if (lane == 0) {
+      y_near[m0] = n0;
+      if (two) y_near[m1] = n1;
+    }
+  }
+}
+
+// Receive (modMLFMA.f90:1915-1933) from the phase rows, one workgroup per
+// leaf box.  With funcREC = conj(funcRAD) and funcRAD(k,p,f) =
+// l/2 (e+(k) rho+ + e-(k) rho-) . u_p(k) (calcRad, :669-699; e+- the phase
+// rows of the basis's triangles, u_1 = theta-hat, u_2 = phi-hat), the sum
+// over samples and polarisations factors as
+//   sum_k sum_p conj(funcRAD(k,p,f)) campoA(k,p) = l/2 (rho+ . G+ + rho- . G-),
+//   G_T = sum_k conj(e_T(k)) F(k),  F(k) = sum_p u_p(k) campoA(k,p),
+// so each phase row is read once (0.44 of the pattern bytes at 122,880
+// unknowns) and contracted with the box's 3-vector field F in LDS.  The
+// terms are summed in another order than the reference's (same value to
+// rounding).  y(f) = (j w mu / 4 pi) (...) + y_near(f) when y_near is given
+// (the near sum of k_near_part, added as k_receive_near adds it), else the
+// far part alone.
+__global__ void __launch_bounds__(256) k_receive_ph(int nk, const int* __restrict__ box_slot,
+                                                    const int* __restrict__ box_row,
+                                                    const int* __restrict__ slot_basis,
+                                                    const int2* __restrict__ slot_pair,
+                                                    const RadGeom* __restrict__ geo,
+                                                    const double* __restrict__ ath,
+                                                    const double* __restrict__ aph,
+                                                    const double2* __restrict__ tph,
+                                                    const double2* __restrict__ AL, double2 cc,
+                                                    const double2* __restrict__ y_near, double2* __restrict__ y) {
+  extern __shared__ double2 sm[];
+  double2* F = sm;           // [3][nk]
+  double2* G = sm + 3 * nk;  // [row of the box][3]
+  const int b = xcd_block(blockIdx.x, gridDim.x);
+  const double2* a = AL + (size_t)b * 2 * nk;
+  for (int k = threadIdx.x; k < nk; k += blockDim.x) {
+    const double2 a0 = a[k], a1 = a[nk + k];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      const double t = ath[3 * k + d], p = aph[3 * k + d];
+      F[d * nk + k] = make_double2(t * a0.x + p * a1.x, t * a0.y + p * a1.y);
+    }
+  }
+  __syncthreads();
+  const int r0 = box_row[b], r1 = box_row[b + 1];
+  // a group of 16 lanes (one DPP row) per phase row: lane j takes the
+  // samples j, j + 16, ...; the group's partial sums are combined by DPP
+  const int j = threadIdx.x % 16, ng = blockDim.x / 16;
+  for (int r = r0 + (int)(threadIdx.x / 16); r < r1; r += ng) {
+    const double2* ph = tph + (size_t)r * nk;
+    double g[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    constexpr int kU = 8;  // row loads issued together
+    for (int kb = j; kb < nk; kb += kU * 16) {
+      double2 p[kU];
+#pragma unroll
+      for (int u = 0; u < kU; ++u)
+        if (kb + 16 * u < nk) p[u] = ph[kb + 16 * u];
+#pragma unroll
+      for (int u = 0; u < kU; ++u) {
+        const int k = kb + 16 * u;
+        if (k < nk) {
+#pragma unroll
+          for (int d = 0; d < 3; ++d) {
+            const double2 v = cmulc(p[u], F[d * nk + k]);
+            g[2 * d] += v.x;
+            g[2 * d + 1] += v.y;
+          }
+        }
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < 6; ++q) g[q] = row16_sum(g[q]);
+    if (j == 0) {
+      double2* o = G + 3 * (size_t)(r - r0);
+#pragma unroll
+      for (int d = 0; d < 3; ++d) o[d] = make_double2(g[2 * d], g[2 * d + 1]);
+    }
+  }
+  __syncthreads();
+  for (int s = box_slot[b] + threadIdx.x; s < box_slot[b + 1]; s += blockDim.x) {
+    const int2 pr = slot_pair[s];
+    const RadGeom& g = geo[s];
+    const double2* gp = G + 3 * (size_t)(pr.x - r0);
+    const double2* gm = G + 3 * (size_t)(pr.y - r0);
+    double2 v = make_double2(0.0, 0.0);
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+      v.x += g.rp[d] * gp[d].x + g.rm[d] * gm[d].x;
+      v.y += g.rp[d] * gp[d].y + g.rm[d] * gm[d].y;
+    }
+    const double hl = 0.5 * g.len;
+    const double2 far = cmul(make_double2(v.x * hl, v.y * hl), cc);
+    const int m = slot_basis[s];
+    y[m] = y_near ? cadd(far, y_near[m]) : far;
   }
 }
 
@@ -543,43 +819,122 @@ int run_matvec(Handle* h, const double2* x, double2* y, double2* y_near, hipEven
   const int L = h->L;
   hipStream_t st = h->stream;
   DevLevel& lf = h->lv[L];
-  if (ev) LW_HIP(hipEventRecord(ev[0], st));
-  hipLaunchKernelGGL(k_aggregate_ph, dim3(lf.nb), dim3(256), 0, st, lf.nk, h->d_box_slot, h->d_slot_basis,
-                     h->d_slot_pair, h->d_rgeo, h->d_ath, h->d_aph, h->d_tph, x, lf.C);
-  if (ev) LW_HIP(hipEventRecord(ev[1], st));
-  for (int n = L; n >= 4; --n) {
+  const bool serial = ev != nullptr;
+  // the near part goes to the caller's buffer (parts) or to the handle's,
+  // from which the receive adds it
+  double2* yn = y_near ? y_near : h->d_ynear;
+  auto aggregate = [&](hipStream_t s) {
+    if (h->agg_lds)
+      hipLaunchKernelGGL(k_aggregate_fac, dim3(lf.nb), dim3(256), h->agg_lds, s, lf.nk, h->d_box_row, h->d_row_ptr,
+                         h->d_row_ent, h->d_slot_basis, h->d_rgeo, h->d_ath, h->d_aph, h->d_tph, x, lf.C);
+    else
+      hipLaunchKernelGGL(k_aggregate_ph, dim3(lf.nb), dim3(256), 0, s, lf.nk, h->d_box_slot, h->d_slot_basis,
+                         h->d_slot_pair, h->d_rgeo, h->d_ath, h->d_aph, h->d_tph, x, lf.C);
+  };
+  // campoC of level n-1 from level n
+  auto up = [&](int n, hipStream_t s) {
     DevLevel& c = h->lv[n];
     DevLevel& p = h->lv[n - 1];
     dim3 grid(p.nb, blocks_for(p.nk, 256));
-    hipLaunchKernelGGL(k_interp_up, grid, dim3(256), 2 * c.nk * sizeof(double2), st, p.nk, c.nk, p.child_ptr, c.oct,
+    hipLaunchKernelGGL(k_interp_up, grid, dim3(256), 2 * c.nk * sizeof(double2), s, p.nk, c.nk, p.child_ptr, c.oct,
                        p.ph, p.wint, p.iint, c.C, p.C);
-  }
-  if (ev) LW_HIP(hipEventRecord(ev[2], st));
-  for (int n = 3; n <= L; ++n) launch_translate_parent(h, n);
-  if (ev) LW_HIP(hipEventRecord(ev[3], st));
-  for (int n = 4; n <= L; ++n) {
+  };
+  // campoA of level n from level n-1 and campoB of level n
+  auto down = [&](int n, hipStream_t s) {
     DevLevel& c = h->lv[n];
     DevLevel& p = h->lv[n - 1];
     const double2* src = (n == 4) ? p.B : p.A;
     if (2 * p.nk * (int)sizeof(double2) <= kDownLdsBytes)
-      hipLaunchKernelGGL(k_interp_down_lds, dim3(c.nb), dim3(512), 2 * p.nk * sizeof(double2), st, p.nk, c.nk,
+      hipLaunchKernelGGL(k_interp_down_lds, dim3(c.nb), dim3(512), 2 * p.nk * sizeof(double2), s, p.nk, c.nk,
                          c.parent, c.oct, p.ph, p.wt_ptr, p.wt, src, c.B, c.A);
     else
-      hipLaunchKernelGGL(k_interp_down, dim3(c.nb * blocks_for(c.nk, 256)), dim3(256), 0, st, p.nk, c.nk,
+      hipLaunchKernelGGL(k_interp_down, dim3(c.nb * blocks_for(c.nk, 256)), dim3(256), 0, s, p.nk, c.nk,
                          c.parent, c.oct, p.ph, p.wt_ptr, p.wt, src, c.B, c.A);
-  }
-  if (ev) LW_HIP(hipEventRecord(ev[4], st));
+  };
   const double w = h->plan.phys.w;
   const double2 cc = make_double2(0.0, w * kMu / (4.0 * kPi));
   const int rows_per_block = 4;
-  dim3 grid(blocks_for(h->ne, rows_per_block)), block(kWave * rows_per_block);
-  if (h->fp64_near)
-    hipLaunchKernelGGL(k_receive_near<double2>, grid, block, 0, st, h->ne, 2 * lf.nk, h->d_slot_basis,
-                       h->d_basis_box, h->d_rad, lf.A, h->d_ia, h->d_ja, (const double2*)h->d_z, x, cc, y, y_near);
-  else
-    hipLaunchKernelGGL(k_receive_near<float2>, grid, block, 0, st, h->ne, 2 * lf.nk, h->d_slot_basis,
-                       h->d_basis_box, h->d_rad, lf.A, h->d_ia, h->d_ja, (const float2*)h->d_z, x, cc, y, y_near);
-  if (ev) LW_HIP(hipEventRecord(ev[5], st));
+  dim3 rgrid(blocks_for(h->ne, rows_per_block)), rblock(kWave * rows_per_block);
+  auto near = [&](hipStream_t s) {
+    if (!h->recv_lds) return;  // summed by k_receive_near with the receive
+    if (h->near_lds) {
+      if (h->fp64_near)
+        hipLaunchKernelGGL(k_near_box<double2>, dim3(lf.nb), dim3(256), h->near_lds, s, h->d_box_slot,
+                           h->d_slot_basis, h->d_ia, h->d_ja, (const double2*)h->d_z, x, yn);
+      else
+        hipLaunchKernelGGL(k_near_box<float2>, dim3(lf.nb), dim3(256), h->near_lds, s, h->d_box_slot,
+                           h->d_slot_basis, h->d_ia, h->d_ja, (const float2*)h->d_z, x, yn);
+    } else if (h->fp64_near)
+      hipLaunchKernelGGL(k_near_part<double2>, rgrid, rblock, 0, s, h->ne, h->d_slot_basis, h->d_ia, h->d_ja,
+                         (const double2*)h->d_z, x, yn);
+    else
+      hipLaunchKernelGGL(k_near_part<float2>, rgrid, rblock, 0, s, h->ne, h->d_slot_basis, h->d_ia, h->d_ja,
+                         (const float2*)h->d_z, x, yn);
+  };
+  auto receive = [&](hipStream_t s) {
+    if (h->recv_lds) {
+      hipLaunchKernelGGL(k_receive_ph, dim3(lf.nb), dim3(256), h->recv_lds, s, lf.nk, h->d_box_slot, h->d_box_row,
+                         h->d_slot_basis, h->d_slot_pair, h->d_rgeo, h->d_ath, h->d_aph, h->d_tph, lf.A, cc,
+                         y_near ? nullptr : yn, y);
+    } else if (h->fp64_near) {
+      hipLaunchKernelGGL(k_receive_near<double2>, rgrid, rblock, 0, s, h->ne, 2 * lf.nk, h->d_slot_basis,
+                         h->d_basis_box, h->d_rad, lf.A, h->d_ia, h->d_ja, (const double2*)h->d_z, x, cc, y, y_near);
+    } else {
+      hipLaunchKernelGGL(k_receive_near<float2>, rgrid, rblock, 0, s, h->ne, 2 * lf.nk, h->d_slot_basis,
+                         h->d_basis_box, h->d_rad, lf.A, h->d_ia, h->d_ja, (const float2*)h->d_z, x, cc, y, y_near);
+    }
+  };
+  if (serial) {
+    LW_HIP(hipEventRecord(ev[0], st));
+    aggregate(st);
+    LW_HIP(hipEventRecord(ev[1], st));
+    for (int n = L; n >= 4; --n) up(n, st);
+    LW_HIP(hipEventRecord(ev[2], st));
+    for (int n = 3; n <= L; ++n) launch_translate_parent(h, n, st);
+    LW_HIP(hipEventRecord(ev[3], st));
+    for (int n = 4; n <= L; ++n) down(n, st);
+    LW_HIP(hipEventRecord(ev[4], st));
+    near(st);
+    LW_HIP(hipEventRecord(ev[5], st));
+    receive(st);
+    LW_HIP(hipEventRecord(ev[6], st));
+  } else {
+    // critical path on the main stream: aggregation, upward passes, level
+    // 3's translation, downward passes, receive.  Beside it, once the
+    // aggregation is done (started earlier, the near field would share HBM
+    // with it): the near field (side 0), the leaf translation (side 1) and
+    // the translations of levels L-1..4 (side 2, each once its campoC is
+    // ready).  Measured at 122,880 unknowns: 0.59 ms against 0.67 ms with
+    // the same kernels one after the other; starting the side work after
+    // the upward passes instead gave 0.60-0.61 ms.
+    hipStream_t sn = h->side[0], sa = h->side[1], sb = h->side[2];
+    aggregate(st);
+    LW_HIP(hipEventRecord(h->ev_fork, st));
+    LW_HIP(hipStreamWaitEvent(sn, h->ev_fork, 0));
+    near(sn);
+    LW_HIP(hipEventRecord(h->ev_near, sn));
+    const bool leaf_side = L > 3, mid_side = L > 4;
+    if (leaf_side) {
+      LW_HIP(hipStreamWaitEvent(sa, h->ev_fork, 0));
+      launch_translate_parent(h, L, sa);
+      LW_HIP(hipEventRecord(h->ev_ta, sa));
+    }
+    for (int n = L; n >= 4; --n) {
+      up(n, st);
+      if (n - 1 >= 4) {
+        LW_HIP(hipEventRecord(h->ev_c[n - 1], st));
+        LW_HIP(hipStreamWaitEvent(sb, h->ev_c[n - 1], 0));
+        launch_translate_parent(h, n - 1, sb);
+      }
+    }
+    if (mid_side) LW_HIP(hipEventRecord(h->ev_tb, sb));
+    launch_translate_parent(h, 3, st);
+    if (leaf_side) LW_HIP(hipStreamWaitEvent(st, h->ev_ta, 0));
+    if (mid_side) LW_HIP(hipStreamWaitEvent(st, h->ev_tb, 0));
+    for (int n = 4; n <= L; ++n) down(n, st);
+    LW_HIP(hipStreamWaitEvent(st, h->ev_near, 0));
+    receive(st);
+  }
   LW_HIP(hipGetLastError());
   return LW_OK;
 }
@@ -669,6 +1024,12 @@ static int init_device_state(Handle* h, const Geometry& g) {
   h->nnz = P.nnz;
   LW_HIP(hipGetDevice(&h->device));
   LW_HIP(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+  // the matvec's side streams (default priority) and their events
+  for (auto& s : h->side) LW_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+  for (hipEvent_t* e : {&h->ev_fork, &h->ev_near, &h->ev_ta, &h->ev_tb})
+    LW_HIP(hipEventCreateWithFlags(e, hipEventDisableTiming));
+  h->ev_c.assign(L + 1, nullptr);
+  for (int n = 3; n <= L; ++n) LW_HIP(hipEventCreateWithFlags(&h->ev_c[n], hipEventDisableTiming));
   int rc;
 #define TRY(x)          \
   do {                  \
@@ -780,6 +1141,7 @@ static int init_device_state(Handle* h, const Geometry& g) {
   // work vectors and reduction buffers
   TRY(dmalloc(h, &h->d_x, (size_t)P.ne));
   TRY(dmalloc(h, &h->d_y, (size_t)P.ne));
+  TRY(dmalloc(h, &h->d_ynear, (size_t)P.ne));
   for (auto& w : h->d_w) TRY(dmalloc(h, &w, (size_t)P.ne));
   TRY(dmalloc(h, &h->d_part, 4 * (size_t)kRedBlocks));
   // the RCS current (lw_rcs_init copies into it), the far-zone distance and
@@ -801,6 +1163,33 @@ static int init_device_state(Handle* h, const Geometry& g) {
   }
   LW_HIP(hipFuncSetAttribute((const void*)k_interp_up, hipFuncAttributeMaxDynamicSharedMemorySize,
                              2 * maxk * (int)sizeof(double2)));
+  if (h->recv_lds)
+    LW_HIP(hipFuncSetAttribute((const void*)k_receive_ph, hipFuncAttributeMaxDynamicSharedMemorySize, h->recv_lds));
+  {
+    // the near product by leaf box needs the same near columns in every row
+    // of a box, and the box's x entries in LDS
+    int maxc = 0;
+    bool same = true;
+    h->near_box_cols = 0;
+    for (int b = 0; b < P.lv[L].nb && same; ++b) {
+      const int m0 = P.slot_basis[P.box_slot[b]], n0 = P.ia[m0 + 1] - P.ia[m0];
+      maxc = std::max(maxc, n0);
+      h->near_box_cols += n0;
+      for (int s = P.box_slot[b] + 1; s < P.box_slot[b + 1] && same; ++s) {
+        const int m = P.slot_basis[s];
+        same = P.ia[m + 1] - P.ia[m] == n0 &&
+               std::equal(P.ja.begin() + P.ia[m], P.ja.begin() + P.ia[m + 1], P.ja.begin() + P.ia[m0]);
+      }
+    }
+    const long long nl = (long long)maxc * sizeof(double2);
+    h->near_lds = (same && h->recv_lds && nl <= kRecvLdsMax) ? (int)std::max<long long>(nl, 16) : 0;
+    if (h->near_lds) {
+      LW_HIP(hipFuncSetAttribute((const void*)k_near_box<float2>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 h->near_lds));
+      LW_HIP(hipFuncSetAttribute((const void*)k_near_box<double2>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                 h->near_lds));
+    }
+  }
 #undef TRY
   return LW_OK;
 }
@@ -823,7 +1212,15 @@ static void free_handle(lw_handle* hh) {
   if (!hh) return;
   Handle* h = &hh->h;
   if (h->stream) (void)hipStreamSynchronize(h->stream);
+  for (hipStream_t s : h->side)
+    if (s) (void)hipStreamSynchronize(s);
   for (void* p : h->allocs) (void)hipFree(p);
+  for (hipEvent_t e : {h->ev_fork, h->ev_near, h->ev_ta, h->ev_tb})
+    if (e) (void)hipEventDestroy(e);
+  for (hipEvent_t e : h->ev_c)
+    if (e) (void)hipEventDestroy(e);
+  for (hipStream_t s : h->side)
+    if (s) (void)hipStreamDestroy(s);
   if (h->stream) (void)hipStreamDestroy(h->stream);
   delete hh;
 }
@@ -1321,7 +1718,7 @@ int lw_rcs_sigma(lw_handle* hh, int n, const double* theta, const double* phi, d
 int lw_bench_matvec(lw_handle* hh, int reps, double* ms, double* stage_ms) {
   if (lw::check_handle(hh) || reps < 1 || !ms) return LW_ERR_ARG;
   Handle* h = &hh->h;
-  hipEvent_t ev[6];
+  hipEvent_t ev[lw::kMatvecStages + 1];
   for (auto& e : ev) LW_HIP(hipEventCreate(&e));
   // deterministic input
   std::vector<double2> xin(h->ne);
@@ -1330,23 +1727,29 @@ int lw_bench_matvec(lw_handle* hh, int reps, double* ms, double* stage_ms) {
   int rc = lw::run_matvec(h, h->d_x, h->d_y, nullptr, nullptr);
   if (rc) return rc;
   LW_HIP(hipStreamSynchronize(h->stream));
-  double acc[5] = {0, 0, 0, 0, 0}, tot = 0.0;
+  // the stages one after the other, each bracketed by events
+  double acc[lw::kMatvecStages] = {};
   for (int r = 0; r < reps; ++r) {
     if ((rc = lw::run_matvec(h, h->d_x, h->d_y, nullptr, ev))) return rc;
-    LW_HIP(hipEventSynchronize(ev[5]));
-    for (int s = 0; s < 5; ++s) {
+    LW_HIP(hipEventSynchronize(ev[lw::kMatvecStages]));
+    for (int s = 0; s < lw::kMatvecStages; ++s) {
       float t = 0.f;
       LW_HIP(hipEventElapsedTime(&t, ev[s], ev[s + 1]));
       acc[s] += t;
     }
-    float t = 0.f;
-    LW_HIP(hipEventElapsedTime(&t, ev[0], ev[5]));
-    tot += t;
   }
+  // the matvec as the solver runs it (side streams), back to back
+  LW_HIP(hipEventRecord(ev[0], h->stream));
+  for (int r = 0; r < reps; ++r)
+    if ((rc = lw::run_matvec(h, h->d_x, h->d_y, nullptr, nullptr))) return rc;
+  LW_HIP(hipEventRecord(ev[1], h->stream));
+  LW_HIP(hipEventSynchronize(ev[1]));
+  float t = 0.f;
+  LW_HIP(hipEventElapsedTime(&t, ev[0], ev[1]));
   for (auto& e : ev) (void)hipEventDestroy(e);
-  *ms = tot / reps;
+  *ms = t / reps;
   if (stage_ms)
-    for (int s = 0; s < 5; ++s) stage_ms[s] = acc[s] / reps;
+    for (int s = 0; s < lw::kMatvecStages; ++s) stage_ms[s] = acc[s] / reps;
   return LW_OK;
 }
 
@@ -1403,27 +1806,15 @@ int lw_bench_copy(lw_handle* hh, long long bytes, int reps, double* ms) {
 
 int lw_matvec_bytes(lw_handle* hh, double* total, double* aggregation, double* receive_near) {
   if (lw::check_handle(hh)) return LW_ERR_ARG;
-  Handle* h = &hh->h;
-  const lw::Plan& P = h->plan;
-  const double c16 = 16.0;
-  const double pattern = (double)P.lv[P.L].nk * 2 * P.ne * c16;  // one pass over funcRAD
-  double campo = 0.0, trans = 0.0;
-  for (int n = 3; n <= P.L; ++n) {
-    campo += (double)P.lv[n].nk * 2 * P.lv[n].nb * c16;
-    trans += (double)P.lv[n].nk * P.lv[n].nT * c16;
+  double sb[lw::kMatvecStages];
+  int rc = lw_matvec_stage_bytes(hh, sb);
+  if (rc) return rc;
+  if (total) {
+    *total = 0.0;
+    for (double b : sb) *total += b;
   }
-  const double zb = h->fp64_near ? 16.0 : 8.0;
-  const double near = (zb + 4.0) * (double)P.nnz + 4.0 * (P.ne + 1);
-  const double xy = 2.0 * P.ne * c16;
-  // the aggregation reads the phase rows (one per leaf box and triangle of
-  // its bases) and the bases' RWG vectors instead of the pattern array
-  const double phases = (double)h->npairs * P.lv[P.L].nk * c16 + (double)P.ne * sizeof(lw::RadGeom);
-  if (total) *total = pattern + phases + 6 * campo + trans + near + xy;
-  // aggregation: reads the phase rows, RWG vectors and x once, writes campoC at the leaf
-  if (aggregation) *aggregation = phases + P.ne * c16 + (double)P.lv[P.L].nk * 2 * P.lv[P.L].nb * c16;
-  // receive + near: patterns, leaf campoA, near CSR, x gathered once, y written
-  if (receive_near)
-    *receive_near = pattern + (double)P.lv[P.L].nk * 2 * P.lv[P.L].nb * c16 + near + 2.0 * P.ne * c16;
+  if (aggregation) *aggregation = sb[0];
+  if (receive_near) *receive_near = sb[4] + sb[5];
   return LW_OK;
 }
 
@@ -1433,10 +1824,14 @@ int lw_matvec_stage_bytes(lw_handle* hh, double* sb) {
   const lw::Plan& P = h->plan;
   const int L = P.L;
   auto field = [&](int n) { return (double)P.lv[n].nk * 2 * P.lv[n].nb * 16.0; };
-  const double pattern = (double)P.lv[L].nk * 2 * P.ne * 16.0;
-  const double phases = (double)h->npairs * P.lv[L].nk * 16.0 + (double)P.ne * sizeof(lw::RadGeom);
+  // the phase rows (one per leaf box and triangle of its bases) and the
+  // per-basis RWG vectors, row indices and slot order
+  const double phases = (double)h->npairs * P.lv[L].nk * 16.0 +
+                        (double)P.ne * (sizeof(lw::RadGeom) + sizeof(int2) + sizeof(int));
   const double zb = h->fp64_near ? 16.0 : 8.0;
-  sb[0] = phases + 16.0 * P.ne + field(L) + 16.0 * P.ne;
+  const double csr = (zb + 4.0) * (double)P.nnz + 4.0 * (P.ne + 1);
+  // aggregation: phase rows and RWG vectors, x, leaf campoC written
+  sb[0] = phases + 16.0 * P.ne + field(L);
   sb[1] = 0.0;
   for (int n = L; n >= 4; --n) sb[1] += field(n) + field(n - 1) + 12.0 * 9 * P.lv[n - 1].nk + 16.0 * 8 * P.lv[n - 1].nk;
   sb[2] = 0.0;
@@ -1447,7 +1842,21 @@ int lw_matvec_stage_bytes(lw_handle* hh, double* sb) {
              8.0 * (P.lv[n - 1].nb + 1);
   sb[3] = 0.0;
   for (int n = 4; n <= L; ++n) sb[3] += field(n - 1) + 2 * field(n) + 12.0 * 9 * P.lv[n - 1].nk;
-  sb[4] = pattern + field(L) + (zb + 4.0) * (double)P.nnz + 4.0 * (P.ne + 1) + 2.0 * 16.0 * P.ne + 8.0 * P.ne;
+  if (h->recv_lds && h->near_lds) {
+    // near field by leaf box: the CSR values and row pointers, one column
+    // list per box and its x entries, the near sums written
+    sb[4] = zb * (double)P.nnz + 4.0 * (P.ne + 1) + 20.0 * (double)h->near_box_cols + 16.0 * P.ne + 4.0 * P.ne;
+    sb[5] = phases + field(L) + 48.0 * P.lv[L].nk + 4.0 * (P.lv[L].nb + 1) + 2.0 * 16.0 * P.ne;
+  } else if (h->recv_lds) {
+    // near field: the CSR, x gathered once, the near sums written
+    sb[4] = csr + 16.0 * P.ne + 16.0 * P.ne + 4.0 * P.ne;
+    // receive: phase rows, leaf campoA, sample directions, near sums read, y written
+    sb[5] = phases + field(L) + 48.0 * P.lv[L].nk + 4.0 * (P.lv[L].nb + 1) + 2.0 * 16.0 * P.ne;
+  } else {
+    // one kernel: patterns, leaf campoA, the CSR, x gathered once, y written
+    sb[4] = 0.0;
+    sb[5] = (double)P.lv[L].nk * 2 * P.ne * 16.0 + field(L) + csr + 2.0 * 16.0 * P.ne + 8.0 * P.ne;
+  }
   return LW_OK;
 }
 

@@ -126,6 +126,14 @@ struct Handle {
   bool fp64_near = false;
   int device = 0;
   hipStream_t stream = nullptr;
+  // side streams of the matvec (near field; leaf translation; translations
+  // of levels L-1..4) and the events that order them with the main stream:
+  // fork, near done, leaf translation done, other translations done, and
+  // campoC of level n ready (ev_c[n])
+  hipStream_t side[3] = {nullptr, nullptr, nullptr};
+  hipEvent_t ev_fork = nullptr, ev_near = nullptr, ev_ta = nullptr, ev_tb = nullptr;
+  std::vector<hipEvent_t> ev_c;
+  double2* d_ynear = nullptr;  // near part of the matvec
   std::vector<DevLevel> lv;  // [1..L]
   int L = 0, ne = 0;
   long long nnz = 0;
@@ -143,6 +151,20 @@ struct Handle {
   int2* d_slot_pair = nullptr; // (ne) phase rows of each slot's two triangles
   double *d_ath = nullptr, *d_aph = nullptr;  // (3, nk_L) theta and phi unit vectors of the leaf samples
   int *d_slot_basis = nullptr, *d_basis_slot = nullptr, *d_basis_box = nullptr, *d_box_slot = nullptr;
+  // phase rows of each leaf box: [box_row[b], box_row[b+1]); the receive
+  // from phase rows needs recv_lds bytes of LDS (0: it does not fit, and
+  // the receive streams the tabulated patterns instead)
+  int* d_box_row = nullptr;
+  // the bases each phase row serves: entries 2 slot + side (0 plus, 1 minus
+  // triangle) of row r at [row_ptr[r], row_ptr[r+1])
+  int *d_row_ptr = nullptr, *d_row_ent = nullptr;
+  int recv_lds = 0;
+  // LDS of the factored aggregation (the H vectors of a box's phase rows;
+  // used with the receive from phase rows) and of the near product by leaf
+  // box (the box's x entries; 0: the rows of some box have different near
+  // columns or they do not fit, and the near product runs row by row)
+  int agg_lds = 0, near_lds = 0;
+  long long near_box_cols = 0;  // sum over leaf boxes of their near-column counts
   // near field CSR (natural order)
   int *d_ia = nullptr, *d_ja = nullptr;
   void* d_z = nullptr;  // float2 or double2 values
@@ -203,6 +225,11 @@ int upload(Handle* h, T** p, const std::vector<T>& v) {
   return LW_OK;
 }
 
+// Stages of the single-RHS matvec as lw_bench_matvec times them.
+constexpr int kMatvecStages = 6;
+// Largest LDS request of the receive from phase rows (k_receive_ph).
+constexpr int kRecvLdsMax = 128 * 1024;
+
 // setup kernels (lw_setup_kernels.hip)
 int launch_setup_rad(Handle* h);
 int launch_setup_phases(Handle* h, const int* e_t);
@@ -231,11 +258,16 @@ int rcs_static_setup(Handle* h);
 // lists of every level must regroup into parent-neighbour blocks (checked
 // against the plan's per-box lists); LW_ERR_STATE otherwise.
 int setup_far_parent(Handle* h);
-// campoB of level n by parent blocks on the handle's stream (lw_batch.hip).
-void launch_translate_parent(Handle* h, int n);
+// campoB of level n by parent blocks on stream st (lw_batch.hip).
+void launch_translate_parent(Handle* h, int n, hipStream_t st);
 
-// One single-RHS matvec (MatxVec_MLFMA) on device vectors, on the handle's
-// stream (lw_device.hip).
+// One single-RHS matvec (MatxVec_MLFMA) on device vectors, ordered on the
+// handle's stream (lw_device.hip).  With y_near, y receives the far part and
+// y_near the near part.  With ev (7 events), the stages run one after the
+// other on the handle's stream and ev[s], ev[s+1] bracket stage s
+// (aggregation, upward passes, translations, downward passes, near field,
+// receive); without, the near field and the translations of levels 4..L run
+// on the handle's side streams beside the upward and downward passes.
 int run_matvec(Handle* h, const double2* x, double2* y, double2* y_near, hipEvent_t* ev);
 // The operator a batch was created on (lw_batch.hip); nullptr for nullptr.
 Handle* batch_handle(lw_batch* b);

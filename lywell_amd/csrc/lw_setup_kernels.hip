@@ -8,6 +8,8 @@
 // reference's, rounding for rounding (no fused multiply-adds).
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "lw_internal.h"
 
 namespace lw {
@@ -224,10 +226,11 @@ int launch_setup_rad(Handle* h) {
 int launch_setup_phases(Handle* h, const int* e_t) {
   const Plan& P = h->plan;
   const LevelPlan& lf = P.lv[h->L];
-  std::vector<int> pair_tri, pair_box;
+  std::vector<int> pair_tri, pair_box, box_row(lf.nb + 1, 0);
   std::vector<int2> slot_pair(P.ne);
   for (int b = 0; b < lf.nb; ++b) {
     const int first = (int)pair_tri.size();
+    box_row[b] = first;
     for (int s = P.box_slot[b]; s < P.box_slot[b + 1]; ++s) {
       const int f = P.slot_basis[s];
       int rows[2];
@@ -250,9 +253,34 @@ int launch_setup_phases(Handle* h, const int* e_t) {
     }
   }
   h->npairs = (int)pair_tri.size();
+  box_row[lf.nb] = h->npairs;
+  // LDS of the receive from phase rows: the box's 3-vector field and one
+  // 3-vector per phase row of the box
+  int maxrows = 0;
+  for (int b = 0; b < lf.nb; ++b) maxrows = std::max(maxrows, box_row[b + 1] - box_row[b]);
+  const long long lds = 3LL * (lf.nk + maxrows) * (long long)sizeof(double2);
+  h->recv_lds = lds <= kRecvLdsMax ? (int)lds : 0;
+  h->agg_lds = h->recv_lds ? 3 * maxrows * (int)sizeof(double2) : 0;
+  // the bases each phase row serves: per row, entries 2 s + side (s the
+  // basis slot, side 0 = its plus triangle, 1 = its minus triangle) in slot
+  // order, plus then minus
+  std::vector<int> row_ptr(h->npairs + 1, 0), row_ent(2 * (size_t)P.ne);
+  for (int s = 0; s < P.ne; ++s) {
+    ++row_ptr[slot_pair[s].x + 1];
+    ++row_ptr[slot_pair[s].y + 1];
+  }
+  for (int r = 0; r < h->npairs; ++r) row_ptr[r + 1] += row_ptr[r];
+  {
+    std::vector<int> fill(row_ptr.begin(), row_ptr.end() - 1);
+    for (int s = 0; s < P.ne; ++s) {
+      row_ent[fill[slot_pair[s].x]++] = 2 * s;
+      row_ent[fill[slot_pair[s].y]++] = 2 * s + 1;
+    }
+  }
   int *d_pt = nullptr, *d_pb = nullptr;
   int rc;
-  if ((rc = upload(h, &d_pt, pair_tri)) || (rc = upload(h, &d_pb, pair_box)) ||
+  if ((rc = upload(h, &d_pt, pair_tri)) || (rc = upload(h, &d_pb, pair_box)) || (rc = upload(h, &h->d_box_row, box_row)) ||
+      (rc = upload(h, &h->d_row_ptr, row_ptr)) || (rc = upload(h, &h->d_row_ent, row_ent)) ||
       (rc = upload(h, &h->d_slot_pair, slot_pair)) || (rc = dmalloc(h, &h->d_tph, (size_t)h->npairs * lf.nk)))
     return rc;
   const long long tot = (long long)h->npairs * lf.nk;
