@@ -779,12 +779,16 @@ __global__ void k_b_wp(int nkc, int nwt, const int* __restrict__ wt_row, const d
 // one pattern row (coalesced pieces), so within a block of 4*QB items, MFMA
 // step q pairs lane group g with item QB*g + q; the A_L operand is fetched
 // with the same assignment.
+// With Y given, the epilogue writes Y = (j w mu / 4 pi) Yf + Yn instead of Yf
+// (Yn: the near field of k_b_near_box), the sum k_b_near_box forms from a
+// stored Yf, operand for operand.
 template <typename S, int R>
 __global__ void __launch_bounds__(128) k_b_receive(int nk, int nb, const int* __restrict__ box_slot,
                                                    const int* __restrict__ slot_basis,
                                                    const typename Cx<S>::T* __restrict__ rad,
                                                    const typename Cx<S>::T* __restrict__ AL,
-                                                   typename Cx<S>::T* __restrict__ Yf) {
+                                                   typename Cx<S>::T* __restrict__ Yf, double2 cc,
+                                                   const double2* __restrict__ Yn, double2* __restrict__ Y) {
   typedef typename Cx<S>::T T2;
   typedef typename Cx<S>::Acc Acc;
   constexpr int NT = 4, NCH = R / 64;
@@ -857,15 +861,26 @@ __global__ void __launch_bounds__(128) k_b_receive(int nk, int nb, const int* __
     for (int v = 0; v < 4; ++v) {
       const int so = st * 16 + mma_row<S>(lk, v);
       if (so < nf) {
-        T2* o = Yf + (size_t)slot_basis[s0 + so] * R + c0 + li;
+        const size_t at0 = (size_t)slot_basis[s0 + so] * R + c0 + li;
+        if (Y) {
 #pragma unroll
-        for (int jt = 0; jt < NT; ++jt) o[jt * 16] = mk<S>(dr[jt][v], di[jt][v]);
+          for (int jt = 0; jt < NT; ++jt) {
+            const T2 f = mk<S>(dr[jt][v], di[jt][v]);
+            const double2 fs = dmul(make_double2((double)f.x, (double)f.y), cc);
+            const double2 yn = Yn[at0 + jt * 16];
+            Y[at0 + jt * 16] = make_double2(fs.x + yn.x, fs.y + yn.y);
+          }
+        } else {
+#pragma unroll
+          for (int jt = 0; jt < NT; ++jt) Yf[at0 + jt * 16] = mk<S>(dr[jt][v], di[jt][v]);
+        }
       }
     }
   }
 }
 
-// Near field plus the scaled far field, one workgroup per leaf box.  Every
+// Near field plus the scaled far field (Yf null: the near field alone), one
+// workgroup per leaf box.  Every
 // row of a box has the same near columns (the bases of the box's near boxes,
 // ascending), so the box's block of the near matrix is a dense (nf x ncol)
 // matrix applied to ncol rows of X: a complex GEMM on FP64 MFMA.  A wave owns
@@ -949,11 +964,17 @@ __global__ void __launch_bounds__(128) k_b_near_box(int R, const int* __restrict
       const int so = st * 16 + mma_row<double>(lk, v);
       if (so < nf) {
         const size_t at0 = (size_t)slot_basis[s0 + so] * R + c0 + li;
+        if (Yf) {
 #pragma unroll
-        for (int jt = 0; jt < NT; ++jt) {
-          const F f = Yf[at0 + jt * 16];
-          const double2 fs = dmul(make_double2((double)f.x, (double)f.y), cc);
-          Y[at0 + jt * 16] = make_double2(fs.x + dr[jt][v], fs.y + di[jt][v]);
+          for (int jt = 0; jt < NT; ++jt) {
+            const F f = Yf[at0 + jt * 16];
+            const double2 fs = dmul(make_double2((double)f.x, (double)f.y), cc);
+            Y[at0 + jt * 16] = make_double2(fs.x + dr[jt][v], fs.y + di[jt][v]);
+          }
+        } else {
+          // the near field alone (k_b_receive adds the far field)
+#pragma unroll
+          for (int jt = 0; jt < NT; ++jt) Y[at0 + jt * 16] = make_double2(dr[jt][v], di[jt][v]);
         }
       }
     }
@@ -1412,6 +1433,8 @@ struct Batch {
   int up_tb[16] = {};
   double2 *X = nullptr, *Y = nullptr, *w[12] = {};
   void* Yf = nullptr;
+  double2* Yn = nullptr;  // near field of the matvec in progress (FP64)
+  hipEvent_t ev_fork = nullptr, ev_near = nullptr;  // near field on the handle's side stream
   double* part = nullptr;
   double2* sc = nullptr;  // 8 per-column scalar rows of R
   int *active = nullptr, *first = nullptr;
@@ -1605,6 +1628,11 @@ static void release_batch(Batch* b) {
     if (g) (void)hipGraphExecDestroy(g);
   for (void* p : b->allocs) (void)hipFree(p);
   b->allocs.clear();
+  for (hipEvent_t* e : {&b->ev_fork, &b->ev_near})
+    if (*e) {
+      (void)hipEventDestroy(*e);
+      *e = nullptr;
+    }
   if (b->ref) {
     release_batch(b->ref);
     delete b->ref;
@@ -1620,8 +1648,10 @@ constexpr int agg_chunk() {
   return std::min(64, (int)(49152 / ((R + 8) * 2 * sizeof(S))) / 4 * 4);
 }
 
+// The leaf aggregation (up) or the receive; with Y, the receive combines
+// its far field with the near field Yn into Y.
 template <typename S, int R>
-static void launch_leaf(Batch* bt, bool up, const double2* X) {
+static void launch_leaf(Batch* bt, bool up, const double2* X, double2* Y = nullptr) {
   typedef typename Cx<S>::T T2;
   Handle* h = bt->h;
   BatchLevel& lf = bt->lv[h->L];
@@ -1630,8 +1660,10 @@ static void launch_leaf(Batch* bt, bool up, const double2* X) {
     hipLaunchKernelGGL((k_b_aggregate<S, R, KC>), dim3(lf.nb), dim3(256), 0, h->stream, lf.nk,
                        lf.nb, h->d_box_slot, h->d_slot_basis, (const T2*)bt->rad, X, (T2*)lf.C);
   } else {
+    const double2 cc = make_double2(0.0, h->plan.phys.w * kMu / (4.0 * kPi));
     hipLaunchKernelGGL((k_b_receive<S, R>), dim3(lf.nb), dim3(128), 0, h->stream, lf.nk, lf.nb, h->d_box_slot,
-                       h->d_slot_basis, (const T2*)bt->rad, (const T2*)lf.A, (T2*)bt->Yf);
+                       h->d_slot_basis, (const T2*)bt->rad, (const T2*)lf.A, (T2*)bt->Yf, cc,
+                       (const double2*)bt->Yn, Y);
   }
 }
 
@@ -1718,34 +1750,39 @@ static void launch_far(Batch* bt, int n, int v, int ns, bool split) {
   }
 }
 
+// One batched matvec.  The near field (FP64 MFMA; it reads only X) runs on
+// the handle's first side stream into Yn, beside the far field's VALU- and
+// latency-bound passes on the main stream; the receive then adds it.
 template <typename S>
 static int batch_matvec_t(Batch* bt, const double2* X, double2* Y) {
   typedef typename Cx<S>::T T2;
   Handle* h = bt->h;
   const int R = bt->R, L = h->L;
-  hipStream_t st = h->stream;
+  hipStream_t st = h->stream, sn = h->side[0];
   BatchLevel& lf = bt->lv[L];
+  LW_HIP(hipEventRecord(bt->ev_fork, st));
+  LW_HIP(hipStreamWaitEvent(sn, bt->ev_fork, 0));
+  const double2 cc = make_double2(0.0, h->plan.phys.w * kMu / (4.0 * kPi));
+  if (h->fp64_near)
+    hipLaunchKernelGGL((k_b_near_box<double2, T2>), dim3(lf.nb), dim3(128), 0, sn, R, h->d_box_slot,
+                       h->d_slot_basis, bt->ncol_ptr, bt->ncols, h->d_ia, (const double2*)h->d_z, X,
+                       (const T2*)nullptr, cc, bt->Yn);
+  else
+    hipLaunchKernelGGL((k_b_near_box<float2, T2>), dim3(lf.nb), dim3(128), 0, sn, R, h->d_box_slot,
+                       h->d_slot_basis, bt->ncol_ptr, bt->ncols, h->d_ia, (const float2*)h->d_z, X,
+                       (const T2*)nullptr, cc, bt->Yn);
+  LW_HIP(hipEventRecord(bt->ev_near, sn));
   if (R == 64)
     launch_leaf<S, 64>(bt, true, X);
   else
     launch_leaf<S, 128>(bt, true, X);
-  const int ncol = 2 * R, nchunk = ncol / kWave;
-  (void)nchunk;
   for (int n = L; n >= 4; --n) launch_up<S>(bt, n, bt->up_tb[n]);
   for (int n = 3; n <= L; ++n) launch_far<S>(bt, n, bt->far_var[n], bt->far_split[n], bt->far_sep[n] != 0);
+  LW_HIP(hipStreamWaitEvent(st, bt->ev_near, 0));
   if (R == 64)
-    launch_leaf<S, 64>(bt, false, nullptr);
+    launch_leaf<S, 64>(bt, false, nullptr, Y);
   else
-    launch_leaf<S, 128>(bt, false, nullptr);
-  const double2 cc = make_double2(0.0, h->plan.phys.w * kMu / (4.0 * kPi));
-  if (h->fp64_near)
-    hipLaunchKernelGGL((k_b_near_box<double2, T2>), dim3(lf.nb), dim3(128), 0, st, R, h->d_box_slot,
-                       h->d_slot_basis, bt->ncol_ptr, bt->ncols, h->d_ia, (const double2*)h->d_z, X,
-                       (const T2*)bt->Yf, cc, Y);
-  else
-    hipLaunchKernelGGL((k_b_near_box<float2, T2>), dim3(lf.nb), dim3(128), 0, st, R, h->d_box_slot,
-                       h->d_slot_basis, bt->ncol_ptr, bt->ncols, h->d_ia, (const float2*)h->d_z, X,
-                       (const T2*)bt->Yf, cc, Y);
+    launch_leaf<S, 128>(bt, false, nullptr, Y);
   LW_HIP(hipGetLastError());
   return LW_OK;
 }
@@ -2156,6 +2193,12 @@ static int batch_operator(Batch* bt) {
     TRY(upload_b(bt, &bt->ncols, cl));
   }
   TRY(bmalloc(bt, (char**)&bt->Yf, (size_t)h->ne * R * (bt->fp64 ? sizeof(double2) : sizeof(float2))));
+  TRY(bmalloc(bt, &bt->Yn, (size_t)h->ne * R));
+  if (hipEventCreateWithFlags(&bt->ev_fork, hipEventDisableTiming) != hipSuccess ||
+      hipEventCreateWithFlags(&bt->ev_near, hipEventDisableTiming) != hipSuccess) {
+    set_error("lw_batch_create: hipEventCreate failed");
+    return LW_ERR_HIP;
+  }
 #undef TRY
   return LW_OK;
 }
