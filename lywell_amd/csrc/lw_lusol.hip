@@ -13,13 +13,14 @@
 // position has all its inputs, and the sweep cannot deadlock.
 //
 // Each row is summed exactly as the level-synchronous kernel below sums it
-// (lane l takes the terms l, l+64, ... in order, then the wave's butterfly
-// sum; this unit is compiled without contraction into FMAs), so both kernels
-// give the same bits.  The level-synchronous kernel serves systems whose row
+// (lane l takes the terms l, l+64, ... in order, then the wave's DPP sum;
+// this unit is compiled without contraction into FMAs), so both kernels give
+// the same bits.  The level-synchronous kernel serves systems whose row
 // bitmaps do not fit in LDS.
 //
 // A row's values and column indices do not depend on other rows: they are
-// loaded one row ahead, so that only the gathers of x wait on other rows.
+// loaded one row ahead, as are the row's own y(i) or x(i) and diagonal, so
+// that only the gathers of x wait on other rows.
 // A lane's x loads of one row are issued together after its waits.
 #include <hip/hip_runtime.h>
 
@@ -43,14 +44,30 @@ __device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_doub
 __device__ __forceinline__ double2 to_d2(float2 v) { return make_double2((double)v.x, (double)v.y); }
 __device__ __forceinline__ double2 to_d2(double2 v) { return v; }
 
-__device__ __forceinline__ double2 wave_sum(double2 v) {
-#pragma unroll
-  for (int o = kWave / 2; o > 0; o >>= 1) {
-    v.x += __shfl_xor(v.x, o, kWave);
-    v.y += __shfl_xor(v.y, o, kWave);
-  }
-  return v;
+// Sum over the wavefront, every lane receiving it (all 64 lanes active):
+// each 16-lane DPP row sums its lanes (rotations by 8 and 4, then the quad
+// permutations [2,3,0,1] and [1,0,3,2]), then the four row sums are read
+// into scalars and added as (r0 + r1) + (r2 + r3).  DPP moves are VALU
+// operations, so the reduction stays off the LDS pipe that the row-ready
+// bitmap uses; it is the step of every row on the solve's critical path.
+template <int CTRL>
+__device__ __forceinline__ double dpp_mov(double v) {
+  const int lo = __builtin_amdgcn_update_dpp(0, __double2loint(v), CTRL, 0xF, 0xF, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, __double2hiint(v), CTRL, 0xF, 0xF, false);
+  return __hiloint2double(hi, lo);
 }
+__device__ __forceinline__ double read_lane(double v, int l) {
+  return __hiloint2double(__builtin_amdgcn_readlane(__double2hiint(v), l),
+                          __builtin_amdgcn_readlane(__double2loint(v), l));
+}
+__device__ __forceinline__ double wave_sum(double v) {
+  v += dpp_mov<0x128>(v);  // row_ror:8
+  v += dpp_mov<0x124>(v);  // row_ror:4
+  v += dpp_mov<0x4E>(v);   // quad_perm [2,3,0,1]
+  v += dpp_mov<0xB1>(v);   // quad_perm [1,0,3,2]
+  return (read_lane(v, 0) + read_lane(v, 16)) + (read_lane(v, 32) + read_lane(v, 48));
+}
+__device__ __forceinline__ double2 wave_sum(double2 v) { return make_double2(wave_sum(v.x), wave_sum(v.y)); }
 
 // Waits until row j's bit is set (acquire: its x is then visible to this
 // lane's loads; all waves of the workgroup share the CU's L1).  A wait that
@@ -113,10 +130,21 @@ __device__ __forceinline__ void sweep(int n, const double2* __restrict__ y, doub
       c[u] = in ? jlu[k] - 1 : -1;
     }
   };
+  // the row's own operands, which no other row writes: y(i) (forward) or
+  // x(i) from the forward sweep and the inverted diagonal alu(i) (backward)
+  double2 own = make_double2(0.0, 0.0);
+  V dia{};
+  auto fetch_own = [&](int r) {
+    own = upper ? x[r - 1] : y[r - 1];
+    if (upper) dia = alu[r];
+  };
   meta(0);
   int t = 0;  // this wave's row count
   int i = __shfl(mi, 0, kWave), k0 = __shfl(m0, 0, kWave), k1 = __shfl(m1, 0, kWave);
-  if (i) fetch(k0, k1);
+  if (i) {
+    fetch(k0, k1);
+    fetch_own(i);
+  }
   while (i) {
     // the wave's next row
     const int tn = t + 1;
@@ -130,7 +158,12 @@ __device__ __forceinline__ void sweep(int n, const double2* __restrict__ y, doub
       an[u] = a[u];
       cn[u] = c[u];
     }
-    if (in_) fetch(n0, n1);  // a, c now hold the next row's terms
+    const double2 own_i = own;
+    const V dia_i = dia;
+    if (in_) {
+      fetch(n0, n1);  // a, c now hold the next row's terms
+      fetch_own(in_);
+    }
 #pragma unroll
     for (int u = 0; u < kAhead; ++u)
       if (cn[u] >= 0) wait_row(bits, cn[u], lerr);
@@ -163,13 +196,10 @@ __device__ __forceinline__ void sweep(int n, const double2* __restrict__ y, doub
     }
     s = wave_sum(s);
     if (lane == 0) {
-      if (upper) {
-        const double2 xi = x[i - 1];
-        x[i - 1] = cmul(to_d2(alu[i]), make_double2(xi.x - s.x, xi.y - s.y));
-      } else {
-        const double2 yi = y[i - 1];
-        x[i - 1] = make_double2(yi.x - s.x, yi.y - s.y);
-      }
+      if (upper)
+        x[i - 1] = cmul(to_d2(dia_i), make_double2(own_i.x - s.x, own_i.y - s.y));
+      else
+        x[i - 1] = make_double2(own_i.x - s.x, own_i.y - s.y);
       publish_row(bits, i - 1);
     }
     t = tn;
