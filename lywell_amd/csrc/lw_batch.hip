@@ -1054,10 +1054,20 @@ __global__ void k_b_dot_finish(int R, int nblk, const double* __restrict__ part,
   const int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= R) return;
   double2 s1 = make_double2(0, 0), s2 = make_double2(0, 0);
-  for (int b = 0; b < nblk; ++b) {
-    const double* o = part + ((size_t)b * R + r) * 4;
-    s1 = dadd(s1, make_double2(o[0], o[1]));
-    s2 = dadd(s2, make_double2(o[2], o[3]));
+  // the partials of eight blocks are loaded together, then added in block
+  // order
+  constexpr int kU = 8;
+  for (int b0 = 0; b0 < nblk; b0 += kU) {
+    double4 v[kU];
+#pragma unroll
+    for (int u = 0; u < kU; ++u)
+      if (b0 + u < nblk) v[u] = *reinterpret_cast<const double4*>(part + ((size_t)(b0 + u) * R + r) * 4);
+#pragma unroll
+    for (int u = 0; u < kU; ++u)
+      if (b0 + u < nblk) {
+        s1 = dadd(s1, make_double2(v[u].x, v[u].y));
+        s2 = dadd(s2, make_double2(v[u].z, v[u].w));
+      }
   }
   out1[r] = s1;
   if (out2) out2[r] = s2;

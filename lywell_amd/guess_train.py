@@ -2,6 +2,9 @@
 
     python -m lywell_amd.guess_train [--meshes 12] [--subdiv 3,4] [--angles 64,192,384]
                                      [--steps 3000] [--out lywell_amd/guess_mlp.npz]
+    python -m lywell_amd.guess_train --kinds ellipsoid --subdiv 4,5 --resolve --amin 0.5 --amax 3
+                                     --angles 64 --step-deg 2.8,0.94,0.47,0.25 --meshes 50
+                                     [--init-from previous.npz] [--time-budget 840]
 
 Training meshes are seeded random bodies: ellipsoids with semi-axes uniform
 in [amin, amax] wavelengths and spheres perturbed by real spherical
@@ -22,6 +25,15 @@ at zero weights with the bias of o2 at `init`: 1 makes the untrained network
 the linear extrapolation 2 xt1 - xt2 of the two phase-corrected neighbours,
 0 the reference's warm start xt1.  The weights and a held-out evaluation are
 written to `out`.
+
+The second form trains at the sizes of the north-star configuration:
+ellipsoids only (`--kinds ellipsoid`), semi-axes in [0.5, 3] wavelengths,
+each meshed at lambda/10 from subdivisions 4 and 5 (7,680 and 30,720 RWG,
+`--resolve`), 64 angles per body with the angular step drawn per body from
+`--step-deg` (so the steps of both the coarse and the fine sweeps are seen).
+`--init-from` continues a saved network on new bodies, so training can be
+spread over several runs, each bounded by `--time-budget`; the metadata
+accumulates every run's training meshes.
 """
 import argparse
 import json
@@ -73,9 +85,10 @@ def random_body(rng, subdiv, amin, amax, kind):
                   "coef": coef.tolist()}
 
 
-def solve_sweep(op, n, pol, errmax, maxit, R=64):
-    """All n angles of one sweep from zero guesses, in batches of R."""
-    phi, dirs = sweep_directions(n)
+def solve_sweep(op, n, pol, errmax, maxit, R=64, span=np.pi):
+    """All n angles of one sweep over [0, span) from zero guesses, in batches
+    of R."""
+    phi, dirs = sweep_directions(n, span)
     pols = polarisations(dirs, pol)
     X = np.zeros((op.ne, n), np.complex128)
     its = np.zeros(n, np.int64)
@@ -154,36 +167,73 @@ def main(argv=None):
     ap.add_argument("--init", type=float, default=1.0, help="initial bias of o2 (1: extrapolation, 0: warm start)")
     ap.add_argument("--seed", type=int, default=2024)
     ap.add_argument("--out", default=DEFAULT_OUT)
+    ap.add_argument("--kinds", choices=["mixed", "ellipsoid"], default="mixed",
+                    help="ellipsoids and perturbed spheres in turn, or ellipsoids only")
+    ap.add_argument("--resolve", action="store_true",
+                    help="mesh each body with the smallest subdivision of --subdiv whose mean edge is at most "
+                         "lambda/10 for its largest semi-axis (else subdivisions in turn)")
+    ap.add_argument("--step-deg", default="",
+                    help="angular steps in degrees, cycled over the meshes: a sweep of n angles then spans "
+                         "n * step (default: n angles over [0, pi))")
+    ap.add_argument("--init-from", default="",
+                    help="continue training the network saved at this path (its training meshes are kept in the "
+                         "metadata); the optimiser state starts afresh")
+    ap.add_argument("--time-budget", type=float, default=0.0,
+                    help="seconds of sweep solving after which no further training mesh is drawn (0: none)")
     args = ap.parse_args(argv)
     if lw.device_count() < 1:
         raise SystemExit("guess_train: no HIP device")
     rng = np.random.default_rng(args.seed)
-    net = lw.Mlp(19, 256, 3, 4, seed=args.seed)
-    init_head(net, args.init)
+    prev_bodies = []
+    if args.init_from:
+        net = load(args.init_from)
+        if net is None:
+            raise SystemExit(f"guess_train: no network at {args.init_from}")
+        prev = json.loads(str(np.load(args.init_from, allow_pickle=False)["meta"]))
+        prev_bodies = prev.get("train_meshes", [])
+    else:
+        net = lw.Mlp(19, 256, 3, 4, seed=args.seed)
+        init_head(net, args.init)
     bodies, skipped = [], 0
     subdivs = [int(v) for v in str(args.subdiv).split(",")]
     angles = [int(v) for v in str(args.angles).split(",")]
+    steps_deg = [float(v) for v in str(args.step_deg).split(",")] if args.step_deg else []
     t0 = time.time()
     i = 0
-    while i <= args.meshes:  # the last mesh is held out
-        kind = "ellipsoid" if i % 2 == 0 else "perturbed_sphere"
-        mesh, meta = random_body(rng, subdivs[i % len(subdivs)], args.amin, args.amax, kind)
+    nmesh = args.meshes
+    while i <= nmesh:  # the last mesh is held out
+        if args.time_budget > 0 and i < nmesh and time.time() - t0 > args.time_budget:
+            log(f"time budget reached after {i} meshes")
+            nmesh = i  # this draw becomes the held-out mesh
+        kind = "ellipsoid" if (args.kinds == "ellipsoid" or i % 2 == 0) else "perturbed_sphere"
+        if args.resolve:
+            # draw the body at the densest subdivision to know its size, then mesh it at the resolved one
+            state = rng.bit_generator.state
+            probe, pmeta = random_body(rng, min(subdivs), args.amin, args.amax, kind)
+            size = max(pmeta["axes"]) if kind == "ellipsoid" else pmeta["radius"] * 1.05
+            rng.bit_generator.state = state
+            sub = resolved_subdiv(size, subdivs)
+        else:
+            sub = subdivs[i % len(subdivs)]
+        mesh, meta = random_body(rng, sub, args.amin, args.amax, kind)
         op = lw.MLFMA(mesh, lam=1.0, ladomax=args.ladomax)
         pol = "VV" if i % 2 == 0 else "HH"
         nang = angles[i % len(angles)]
-        sol = solve_sweep(op, nang, pol, args.errmax, args.maxit)
+        span = np.deg2rad(steps_deg[i % len(steps_deg)]) * nang if steps_deg else np.pi
+        meta["span_deg"] = float(np.rad2deg(span))
+        sol = solve_sweep(op, nang, pol, args.errmax, args.maxit, span=span)
         if sol is None:
             # unpreconditioned CGS stalls on some bodies: draw another one
             log(f"mesh {i}: {kind} {mesh.ne} RWG, {pol}: sweep not converged in {args.maxit} iterations, redrawn")
             op.close()
             skipped += 1
-            if skipped > 2 * args.meshes:
+            if skipped > 2 * max(nmesh, 1):
                 raise RuntimeError("too many training sweeps failed to converge")
             continue
         phi, dirs, pols, X, its = sol
         meta.update({"ne": mesh.ne, "levels": op.nlevels, "pol": pol, "angles": nang,
                      "iterations_mean": float(its.mean())})
-        if i < args.meshes:
+        if i < nmesh:
             samples(net, op, dirs, pols, X)
             bodies.append(meta)
             log(f"mesh {i}: {kind} {mesh.ne} RWG, {pol}, {nang} angles solved (mean {its.mean():.0f} iterations), "
@@ -204,8 +254,8 @@ def main(argv=None):
         ev["reverse" if reverse else "forward"] = {k: float(np.median(v)) for k, v in err.items()}
     log("held-out median relative error of the guess:", json.dumps(ev))
     np.savez(args.out, params=net.get_params(), arch=np.array([net.in_dim, net.hidden, net.depth, net.out_dim]),
-             loss=loss, meta=json.dumps({"train_meshes": bodies, "held_out": meta, "held_out_error": ev,
-                                         "redrawn": skipped, "args": vars(args)}))
+             loss=loss, meta=json.dumps({"train_meshes": prev_bodies + bodies, "held_out": meta,
+                                         "held_out_error": ev, "redrawn": skipped, "args": vars(args)}))
     log(f"wrote {args.out}")
     return ev
 
