@@ -1422,6 +1422,7 @@ int upload_structure(Handle* h) {
   lw::tri_levels(n, h->h_jlu, h->h_ju, true, uptr, urows);
   h->nlev_l = (int)lptr.size() - 1;
   h->nlev_u = (int)uptr.size() - 1;
+  h->h_lptr = lptr;
   if (int rc = lw::lusol_flow_setup(h)) return rc;
   if (int rc = lw::upload(h, &h->d_lptr, lptr)) return rc;
   if (int rc = lw::upload(h, &h->d_lrows, lrows)) return rc;

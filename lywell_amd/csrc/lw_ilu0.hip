@@ -8,8 +8,8 @@
 // Row ii is eliminated against the rows jrow of its L part in ascending
 // order, exactly as the reference does, so it can start once those rows are
 // final: the rows of one level of the lower-triangular dependency graph are
-// independent.  One workgroup walks the levels; each wave factors one row at
-// a time with the row staged in LDS: per elimination step the wave's lanes
+// independent.  One launch per level spreads its rows over workgroups of four
+// waves; each wave factors one row at a time with the row staged in LDS: per elimination step the wave's lanes
 // take consecutive runs of jrow's U part, find their columns in row ii by
 // interleaved binary searches over the row's sorted columns, and update them.  Every entry of row
 // ii is updated in the same order and with the same arithmetic as the
@@ -29,7 +29,7 @@ namespace lw {
 namespace {
 
 constexpr int kWaveSize = 64;
-constexpr int kIluMaxWaves = 16;
+constexpr int kIluWaves = 4;  // waves (rows in flight) per workgroup
 constexpr int kIluChunk = 4;  // consecutive U-part entries per lane and block
 
 // Orders the wave's LDS accesses: every lane's writes are complete before any
@@ -83,8 +83,11 @@ struct IluCx<double> {
   typedef double2 T;
 };
 
+// Levels l0 .. l1-1; the rows of a level are dealt to the waves of all
+// workgroups (a launch per level, so the rows of one level run on as many
+// CUs as it has rows; the kernel boundary orders the levels).
 template <typename S>
-__global__ void __launch_bounds__(1024) k_ilu0_factor(int nlev, const int* __restrict__ lptr,
+__global__ void __launch_bounds__(1024) k_ilu0_factor(int l0, int l1, const int* __restrict__ lptr,
                                                       const int* __restrict__ lrows, const int* __restrict__ jlu,
                                                       const int* __restrict__ ju, typename IluCx<S>::T* alu,
                                                       int maxrow, int* __restrict__ piv) {
@@ -97,8 +100,8 @@ __global__ void __launch_bounds__(1024) k_ilu0_factor(int nlev, const int* __res
   T2* vals = reinterpret_cast<T2*>(smem) + nw + (size_t)wave * maxrow;  // its other entries
   int* cols = reinterpret_cast<int*>(reinterpret_cast<T2*>(smem) + nw + (size_t)nw * maxrow) +
               (size_t)wave * maxrow;
-  for (int l = 0; l < nlev; ++l) {
-    for (int r = lptr[l] + wave; r < lptr[l + 1]; r += nw) {
+  for (int l = l0; l < l1; ++l) {
+    for (int r = lptr[l] + (int)blockIdx.x * nw + wave; r < lptr[l + 1]; r += (int)gridDim.x * nw) {
       const int ii = lrows[r] + 1;
       const int p0 = jlu[ii], len = jlu[ii + 1] - p0, pu = ju[ii];
       for (int e = lane; e < len; e += kWaveSize) {
@@ -109,13 +112,13 @@ __global__ void __launch_bounds__(1024) k_ilu0_factor(int nlev, const int* __res
       wave_lds_sync();
       // Per elimination step the lanes need jrow's U-part bounds, its
       // inverted diagonal and their run of its U part: all final, so they are
-      // fetched ahead (bounds two steps, runs one step), with clamped
+      // fetched ahead (bounds three steps, runs two steps), with clamped
       // addresses so that the loads are unconditional and stay in flight
       // while the current step is applied.
-      int qa0 = 0, qa1 = 0, qb0 = 0, qb1 = 0;
-      T2 ba = {}, bb = {};
-      int cca[kIluChunk], ccb[kIluChunk];
-      T2 uua[kIluChunk], uub[kIluChunk];
+      int qa0 = 0, qa1 = 0, qb0 = 0, qb1 = 0, qc0 = 0, qc1 = 0;
+      T2 ba = {}, bb = {}, bc = {};
+      int cca[kIluChunk], ccb[kIluChunk], ccc[kIluChunk];
+      T2 uua[kIluChunk], uub[kIluChunk], uuc[kIluChunk];
       auto meta = [&](int j, int& q0, int& q1, T2& b) {
         const int jr = cols[min(j, pu - 1) - p0];
         q0 = ju[jr];
@@ -136,12 +139,14 @@ __global__ void __launch_bounds__(1024) k_ilu0_factor(int nlev, const int* __res
         meta(p0, qa0, qa1, ba);
         load(qa0, qa1, cca, uua);
         meta(p0 + 1, qb0, qb1, bb);
+        load(qb0, qb1, ccb, uub);
+        meta(p0 + 2, qc0, qc1, bc);
       }
       for (int j = p0; j < pu; ++j) {
-        int qc0, qc1;
-        T2 bc;
-        load(qb0, qb1, ccb, uub);
-        meta(j + 2, qc0, qc1, bc);
+        int qd0, qd1;
+        T2 bd;
+        load(qc0, qc1, ccc, uuc);
+        meta(j + 3, qd0, qd1, bd);
         // tl = alu(j) * alu(jrow), in the storage precision
         const T2 a = vals[j - p0];
         T2 t;
@@ -196,10 +201,15 @@ __global__ void __launch_bounds__(1024) k_ilu0_factor(int nlev, const int* __res
         qb0 = qc0;
         qb1 = qc1;
         bb = bc;
+        qc0 = qd0;
+        qc1 = qd1;
+        bc = bd;
 #pragma unroll
         for (int u = 0; u < kIluChunk; ++u) {
           cca[u] = ccb[u];
           uua[u] = uub[u];
+          ccb[u] = ccc[u];
+          uub[u] = uuc[u];
         }
       }
       if (lane == 0) {
@@ -284,7 +294,7 @@ int ilu0_device(Handle* h, int* piv) {
   if (hipDeviceGetAttribute(&lds_optin, hipDeviceAttributeSharedMemPerBlockOptin, dev) == hipSuccess)
     lds = std::max(lds, lds_optin);
   const size_t per_wave = (size_t)std::max(maxrow, 1) * (esz + sizeof(int)) + esz;
-  const int nw = (int)std::min<size_t>(kIluMaxWaves, lds / per_wave);
+  const int nw = (int)std::min<size_t>(kIluWaves, lds / per_wave);
   if (nw < 1) {
     set_error("ilu0: a near-field row of " + std::to_string(maxrow) + " entries does not fit in LDS");
     return LW_ERR_ARG;
@@ -315,14 +325,20 @@ int ilu0_device(Handle* h, int* piv) {
       hipLaunchKernelGGL(k_ilu0_scatter<double2>, dim3((n + rows_per_block - 1) / rows_per_block),
                          dim3(rows_per_block * kWaveSize), 0, st, n, h->d_ia, h->d_ja, (const double2*)h->d_z,
                          h->d_jlu, (double2*)h->d_alu);
-      hipLaunchKernelGGL(k_ilu0_factor<double>, dim3(1), dim3(nw * kWaveSize), shm, st, h->nlev_l, h->d_lptr,
-                         h->d_lrows, h->d_jlu, h->d_ju, (double2*)h->d_alu, maxrow, d_piv);
+      for (int l = 0; l < h->nlev_l; ++l) {
+        const int rows = h->h_lptr[l + 1] - h->h_lptr[l];
+        hipLaunchKernelGGL(k_ilu0_factor<double>, dim3((rows + nw - 1) / nw), dim3(nw * kWaveSize), shm, st, l,
+                           l + 1, h->d_lptr, h->d_lrows, h->d_jlu, h->d_ju, (double2*)h->d_alu, maxrow, d_piv);
+      }
     } else {
       hipLaunchKernelGGL(k_ilu0_scatter<float2>, dim3((n + rows_per_block - 1) / rows_per_block),
                          dim3(rows_per_block * kWaveSize), 0, st, n, h->d_ia, h->d_ja, (const float2*)h->d_z,
                          h->d_jlu, (float2*)h->d_alu);
-      hipLaunchKernelGGL(k_ilu0_factor<float>, dim3(1), dim3(nw * kWaveSize), shm, st, h->nlev_l, h->d_lptr,
-                         h->d_lrows, h->d_jlu, h->d_ju, (float2*)h->d_alu, maxrow, d_piv);
+      for (int l = 0; l < h->nlev_l; ++l) {
+        const int rows = h->h_lptr[l + 1] - h->h_lptr[l];
+        hipLaunchKernelGGL(k_ilu0_factor<float>, dim3((rows + nw - 1) / nw), dim3(nw * kWaveSize), shm, st, l,
+                           l + 1, h->d_lptr, h->d_lrows, h->d_jlu, h->d_ju, (float2*)h->d_alu, maxrow, d_piv);
+      }
     }
     e = hipGetLastError();
   }

@@ -173,6 +173,7 @@ struct Handle {
   void* d_alu = nullptr;  // float2 / double2, alu(k) at index k
   int *d_jlu = nullptr, *d_ju = nullptr;
   int *d_lptr = nullptr, *d_lrows = nullptr, *d_uptr = nullptr, *d_urows = nullptr;
+  std::vector<int> h_lptr;  // host copy of the forward sweep's level pointers
   int nlev_l = 0, nlev_u = 0;
   int* d_lusol_err = nullptr;  // set when a triangular solve's wait reached its limit
   bool lusol_flow = false;      // the dataflow triangular solve fits (lw_lusol.hip)
