@@ -33,6 +33,12 @@ extern "C" {
 #define LW_NEAR_FP64 1 /* store near field and ILU0 factor in complex(8)
                           (the reference's dp_near = dp_prec = 8 variant);
                           default keeps complex(4) storage as shipped */
+#define LW_PATTERN_ORDER 2 /* aggregate and receive pattern by pattern, in the
+                              reference's order over the bases (the tabulated
+                              radiation patterns, receive and near field in one
+                              kernel) instead of factored through the per-box
+                              triangle phase rows; the default falls back to
+                              this where a box's rows do not fit in LDS */
 
 typedef struct lw_handle lw_handle;
 

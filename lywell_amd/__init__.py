@@ -23,6 +23,7 @@ LW_ERR_STATE = -5
 LW_ERR_PIVOT = -6
 LW_ERR_NODEV = -7
 LW_NEAR_FP64 = 1
+LW_PATTERN_ORDER = 2
 
 _lib = None
 
@@ -215,12 +216,16 @@ def det_vector(n):
 class MLFMA:
     """One MLFMA operator on the GPU (inicializar_MLFMA ... destruir_MLFMA)."""
 
-    def __init__(self, mesh, lam=1.0, ladomax=0.5, fp64_near=False):
+    def __init__(self, mesh, lam=1.0, ladomax=0.5, fp64_near=False, pattern_order=False):
+        """pattern_order: aggregation and receive pattern by pattern, in the
+        reference's order over the bases (LW_PATTERN_ORDER), instead of
+        factored through the per-box triangle phase rows."""
         L = lib()
         self.mesh = mesh
         h = _C.c_void_p()
         a = mesh.init_args()
-        rc = L.lw_mlfma_init(_C.c_double(lam), _C.c_double(ladomax), LW_NEAR_FP64 if fp64_near else 0, a[0],
+        flags = (LW_NEAR_FP64 if fp64_near else 0) | (LW_PATTERN_ORDER if pattern_order else 0)
+        rc = L.lw_mlfma_init(_C.c_double(lam), _C.c_double(ladomax), flags, a[0],
                              _P(a[1]), _P(a[2]), _P(a[3]), _P(a[4]), a[5], _P(a[6]), _P(a[7]), _P(a[8]), a[9],
                              _P(a[10]), _P(a[11]), _P(a[12]), _P(a[13]), _C.byref(h))
         _check(rc)

@@ -1163,8 +1163,12 @@ static int init_device_state(Handle* h, const Geometry& g) {
   }
   LW_HIP(hipFuncSetAttribute((const void*)k_interp_up, hipFuncAttributeMaxDynamicSharedMemorySize,
                              2 * maxk * (int)sizeof(double2)));
-  if (h->recv_lds)
+  if (h->flags & LW_PATTERN_ORDER) h->recv_lds = h->agg_lds = 0;  // the pattern-by-pattern kernels
+  if (h->recv_lds) {
     LW_HIP(hipFuncSetAttribute((const void*)k_receive_ph, hipFuncAttributeMaxDynamicSharedMemorySize, h->recv_lds));
+    LW_HIP(hipFuncSetAttribute((const void*)k_aggregate_fac, hipFuncAttributeMaxDynamicSharedMemorySize,
+                               std::max(h->agg_lds, 16)));
+  }
   {
     // the near product by leaf box needs the same near columns in every row
     // of a box, and the box's x entries in LDS
@@ -1232,6 +1236,10 @@ int lw_mlfma_init(double lambda, double ladomax, int flags, int num_e, const dou
   if (!out || !e_centro || !e_t || !e_po || !e_long || !t_baric || !t_baric_sub || !p_coord || num_e < 2 ||
       num_t < 2 || num_p < 3) {
     lw::set_error("lw_mlfma_init: missing geometry array or empty mesh");
+    return LW_ERR_ARG;
+  }
+  if (flags & ~(LW_NEAR_FP64 | LW_PATTERN_ORDER)) {
+    lw::set_error("lw_mlfma_init: unknown flags");
     return LW_ERR_ARG;
   }
   *out = nullptr;

@@ -108,3 +108,16 @@ def test_ilu0_csr_checks_its_arguments_on_the_host():
     with pytest.raises(lw.LywellError) as e:
         lw.ilu0_csr(ia, ja, np.ones(2, complex))
     assert e.value.code == lw.LW_ERR_ARG
+
+
+def test_unknown_init_flags_are_refused_before_any_device_use():
+    import ctypes as C
+    mesh = lw.Mesh.icosphere(2, 0.5)
+    a = mesh.init_args()
+    P = _oracle._P
+    h = C.c_void_p()
+    rc = lw.lib().lw_mlfma_init(C.c_double(1.0), C.c_double(0.2), 4, a[0], P(a[1]), P(a[2]), P(a[3]), P(a[4]),
+                                a[5], P(a[6]), P(a[7]), P(a[8]), a[9], P(a[10]), P(a[11]), P(a[12]), P(a[13]),
+                                C.byref(h))
+    assert rc == lw.LW_ERR_ARG
+    assert "unknown flags" in lw.lib().lw_last_error().decode()

@@ -91,6 +91,31 @@ def test_matvec(name, vec):
 
 
 @pytest.mark.parametrize("name", list(CASES))
+def test_matvec_pattern_order(name):
+    # LW_PATTERN_ORDER: aggregation and receive pattern by pattern in the
+    # reference's order over the bases (the kernels the factored form falls
+    # back to), against the oracle and against the factored default; the
+    # near sums are formed in the same order by both, so they agree bit for bit
+    mesh, gpu, ora = setup_case(name)
+    sub, ax, lad = CASES[name]
+    pat = lw.MLFMA(mesh, lam=1.0, ladomax=lad, pattern_order=True)
+    try:
+        rng = np.random.default_rng(3)
+        x = rng.standard_normal(mesh.ne) + 1j * rng.standard_normal(mesh.ne)
+        y_o = ora.matvec(x)
+        y_p = pat.matvec(x)
+        assert rel(y_p, y_o) < 1e-10
+        assert rel(y_p, gpu.matvec(x)) < 1e-13
+        yf_p, yn_p = pat.matvec_parts(x)
+        yf, yn = gpu.matvec_parts(x)
+        yf_o, _ = ora.matvec_parts(x)
+        assert rel(yf_p, yf_o) < 1e-12
+        assert np.array_equal(yn_p, yn)
+    finally:
+        pat.close()
+
+
+@pytest.mark.parametrize("name", list(CASES))
 def test_matvec_linearity(name):
     mesh, gpu, _ = setup_case(name)
     rng = np.random.default_rng(1)
