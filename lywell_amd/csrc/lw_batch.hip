@@ -619,13 +619,13 @@ __global__ void __launch_bounds__(256) k_b_anterp(int nk, int nP, int nchunk, in
 // Single right-hand side translation by parent blocks (the sum of
 // modMLFMA.f90:1837-1853 regrouped as in k_b_far): one lane per sample k; a
 // workgroup takes 64 consecutive samples of one parent P, its wave w the
-// children 2w and 2w+1 of P.  P's flat source list is walked in groups of
+// children w and w+4 of P.  P's flat source list is walked in groups of
 // kSrcU = 4 rows: wave u fetches row u of the next group (both
 // polarisations) into registers while the current group is applied from LDS,
 // then stores it to the other LDS buffer before the group barrier, so every
 // source row is read from memory once per workgroup.  Each row is applied to
 // the wave's children with the pair's transfer value Tc[code][k] (zero for
-// near offsets and pads), loaded one source ahead.  Each child's far boxes
+// near offsets and pads), the group's values loaded together.  Each child's far boxes
 // come in ascending box order, the order of the plan's per-box far lists,
 // and each term is added as the reference adds it (campoB += campoC * T, then
 // campoB * pesos), so the sums are those of the per-box form: the near and
@@ -639,26 +639,21 @@ __device__ __forceinline__ void translate_group(const double2 (*sv)[2][kWave], i
                                                 const int2 (&e)[kSrcU], const int (&coff)[2],
                                                 const double2* __restrict__ Tc, double2 (&a0)[2],
                                                 double2 (&a1)[2]) {
-  double2 ta[NW], tb[NW];
-  auto tload = [&](int u, double2(&t)[NW]) {
+  // the group's transfer values (every source, every child) are loaded
+  // together, then applied source by source
+  double2 t[kSrcU][NW];
 #pragma unroll
-    for (int i = 0; i < NW; ++i) t[i] = Tc[(size_t)(e[u].y + coff[i]) * nk + kl];
-  };
-  auto apply = [&](int u, const double2(&t)[NW]) {
+  for (int u = 0; u < kSrcU; ++u)
+#pragma unroll
+    for (int i = 0; i < NW; ++i) t[u][i] = Tc[(size_t)(e[u].y + coff[i]) * nk + kl];
+#pragma unroll
+  for (int u = 0; u < kSrcU; ++u) {
     const double2 v0 = sv[u][0][lane], v1 = sv[u][1][lane];
 #pragma unroll
     for (int i = 0; i < NW; ++i) {
-      a0[i] = dadd(a0[i], dmul(v0, t[i]));
-      a1[i] = dadd(a1[i], dmul(v1, t[i]));
+      a0[i] = dadd(a0[i], dmul(v0, t[u][i]));
+      a1[i] = dadd(a1[i], dmul(v1, t[u][i]));
     }
-  };
-  tload(0, ta);
-#pragma unroll
-  for (int u = 0; u < kSrcU; u += 2) {
-    tload(u + 1, tb);
-    apply(u, ta);
-    if (u + 2 < kSrcU) tload(u + 2, ta);
-    apply(u + 1, tb);
   }
 }
 
@@ -678,11 +673,15 @@ __global__ void __launch_bounds__(256) k_translate_parent(int nk, int nchk, cons
   const int lane = threadIdx.x % kWave;
   const int k = (wb % nchk) * kWave + lane, kl = min(k, nk - 1);
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave);
-  const int c0 = pc_ptr[P] + 2 * w, nw = max(0, min(2, pc_ptr[P + 1] - c0));
+  // wave w takes the children w and w + 4 of P, so that the four waves share
+  // the children of a parent with four or fewer (most of them at the leaf
+  // level)
+  const int nc = pc_ptr[P + 1] - pc_ptr[P];
+  const int c0 = pc_ptr[P] + w, nw = (w < nc ? 1 : 0) + (w + kTrWaves < nc ? 1 : 0);
   const int s0 = ps_ptr[P], s1 = ps_ptr[P + 1];
   int coff[2];
 #pragma unroll
-  for (int i = 0; i < 2; ++i) coff[i] = kMaxCq - bcd[min(c0 + i, pc_ptr[P + 1] - 1)];
+  for (int i = 0; i < 2; ++i) coff[i] = kMaxCq - bcd[min(c0 + kTrWaves * i, pc_ptr[P + 1] - 1)];
   double2 a0[2], a1[2];
 #pragma unroll
   for (int i = 0; i < 2; ++i) {
@@ -698,13 +697,18 @@ __global__ void __launch_bounds__(256) k_translate_parent(int nk, int nchk, cons
     }
     __syncthreads();
     int cur = 0;
-    for (int g = s0; g < s1; g += kSrcU) {
-      int2 e[kSrcU];
+    // the group's entries (source box, code part), loaded a group ahead
+    int2 e[kSrcU];
 #pragma unroll
-      for (int u = 0; u < kSrcU; ++u) e[u] = ps_ent[g + u];
-      // row w of the next group (the last group again at the end, so that
-      // the load stays in bounds)
-      const double2* cr = C + (size_t)ps_ent[min(g + kSrcU, s1 - kSrcU) + w].x * 2 * nk + kl;
+    for (int u = 0; u < kSrcU; ++u) e[u] = ps_ent[s0 + u];
+    for (int g = s0; g < s1; g += kSrcU) {
+      // the next group (the last group again at the end, so that the loads
+      // stay in bounds): its entries and its row w
+      const int gn = min(g + kSrcU, s1 - kSrcU);
+      int2 en[kSrcU];
+#pragma unroll
+      for (int u = 0; u < kSrcU; ++u) en[u] = ps_ent[gn + u];
+      const double2* cr = C + (size_t)ps_ent[gn + w].x * 2 * nk + kl;
       const double2 n0 = cr[0], n1 = cr[nk];
       if (nw == 2)
         translate_group<2>(sv[cur], lane, nk, kl, e, coff, Tc, a0, a1);
@@ -714,6 +718,8 @@ __global__ void __launch_bounds__(256) k_translate_parent(int nk, int nchk, cons
       sv[cur ^ 1][w][1][lane] = n1;
       __syncthreads();
       cur ^= 1;
+#pragma unroll
+      for (int u = 0; u < kSrcU; ++u) e[u] = en[u];
     }
   }
   if (k < nk) {
@@ -721,7 +727,7 @@ __global__ void __launch_bounds__(256) k_translate_parent(int nk, int nchk, cons
 #pragma unroll
     for (int i = 0; i < 2; ++i)
       if (i < nw) {
-        double2* o = B + (size_t)(c0 + i) * 2 * nk + k;
+        double2* o = B + (size_t)(c0 + kTrWaves * i) * 2 * nk + k;
         o[0] = make_double2(a0[i].x * p, a0[i].y * p);
         o[nk] = make_double2(a1[i].x * p, a1[i].y * p);
       }
