@@ -699,15 +699,29 @@ __global__ void k_cgs_q(int n, const double2* __restrict__ rho1p, const double2*
   t[i] = cadd(u[i], qi);
 }
 
-// x += alfa U_; r -= alfa Q_   (:434-437)
+// x += alfa U_; r -= alfa Q_   (:434-437); nothing when *stop is set (the
+// iteration was enqueued before the host saw the previous one end the solve)
 __global__ void k_cgs_xr(int n, const double2* __restrict__ alfap, const double2* __restrict__ U,
-                         const double2* __restrict__ Q, double2* __restrict__ x, double2* __restrict__ r) {
+                         const double2* __restrict__ Q, double2* __restrict__ x, double2* __restrict__ r,
+                         const int* __restrict__ stop) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+  if (i >= n || *stop) return;
   const double2 a = *alfap;
   x[i] = cadd(x[i], cmul(a, U[i]));
   const double2 aq = cmul(a, Q[i]);
   r[i] = make_double2(r[i].x - aq.x, r[i].y - aq.y);
+}
+
+// The host's two exits after an iteration, decided again on the device from
+// the same scalars: the stopping test |r|^2 < thr (thr chosen so that this is
+// sqrt(|r|^2) < errmax ||b||, modIterativo.f90:472) and a zero next rho (the
+// breakdown exit at the top of the next iteration, :392); then rho2 = rho1,
+// rho1 = next rho (:487). s[4] = |r|^2, s[5] = next rho.
+__global__ void k_cgs_check(double thr, double2* __restrict__ s, int* __restrict__ stop) {
+  const double2 nx = s[5];
+  *stop = (s[4].x < thr || (nx.x == 0.0 && nx.y == 0.0)) ? 1 : 0;
+  s[1] = s[0];
+  s[0] = nx;
 }
 
 __global__ void k_sub(int n, const double2* __restrict__ a, const double2* __restrict__ b, double2* __restrict__ o) {
@@ -963,6 +977,22 @@ int run_dot2(Handle* h, const double2* a, const double2* b, int ca, const double
 // LW_ERR_STATE if a dataflow triangular solve on this handle gave up a wait
 // (it then finished with wrong values; it cannot happen with a factor whose
 // level sets were built here); the flag is cleared.
+// The least s with sqrt(s) >= t. A correctly rounded square root is
+// monotone, so for s >= 0 (and NaN) s < sq_threshold(t) exactly when
+// sqrt(s) < t: the CGS stopping test without a square root, the same on the
+// host and on the device.
+double sq_threshold(double t) {
+  if (!(t > 0.0)) return 0.0;  // sqrt(s) < t is false for every s >= 0
+  if (std::isinf(t)) return t;
+  double c = t * t;
+  if (std::sqrt(c) >= t) {
+    while (c > 0.0 && std::sqrt(std::nextafter(c, 0.0)) >= t) c = std::nextafter(c, 0.0);
+  } else {
+    while (std::sqrt(c) < t) c = std::nextafter(c, HUGE_VAL);
+  }
+  return c;
+}
+
 int lusol_status(Handle* h) {
   int e = 0;
   LW_HIP(hipMemcpyAsync(&e, h->d_lusol_err, sizeof(int), hipMemcpyDeviceToHost, h->stream));
@@ -1153,6 +1183,8 @@ static int init_device_state(Handle* h, const Geometry& g) {
   LW_HIP(hipMemsetAsync(h->d_lusol_err, 0, sizeof(int), h->stream));
   h->scal_cap = 64;
   TRY(dmalloc(h, &h->d_scal, (size_t)h->scal_cap));
+  LW_HIP(hipHostMalloc((void**)&h->h_cgs, 4 * sizeof(double2), hipHostMallocDefault));
+  for (hipEvent_t& e : h->ev_cgs) LW_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   LW_HIP(hipStreamSynchronize(h->stream));
   // large LDS requests for the downward kernel at the top level
   int maxk = 0;
@@ -1223,6 +1255,9 @@ static void free_handle(lw_handle* hh) {
     if (e) (void)hipEventDestroy(e);
   for (hipEvent_t e : h->ev_c)
     if (e) (void)hipEventDestroy(e);
+  for (hipEvent_t e : h->ev_cgs)
+    if (e) (void)hipEventDestroy(e);
+  if (h->h_cgs) (void)hipHostFree(h->h_cgs);
   for (hipStream_t s : h->side)
     if (s) (void)hipStreamDestroy(s);
   if (h->stream) (void)hipStreamDestroy(h->stream);
@@ -1543,10 +1578,15 @@ int lw_cgs(lw_handle* hh, const double* b, const double* guess, double errmax, i
   const int n = h->ne;
   const size_t bytes = sizeof(double2) * n;
   hipStream_t st = h->stream;
-  double2 *d_b = h->d_w[0], *r = h->d_w[1], *R_ = h->d_w[2], *u = h->d_w[3], *p = h->d_w[4], *P_ = h->d_w[5],
-          *V = h->d_w[6], *q = h->d_w[7], *U = h->d_w[8], *Q = h->d_w[9], *t = h->d_w[10], *xv = h->d_x;
-  // device scalars: [0] rho1 [1] rho2 [2] den [3] alfa [4] |r|^2 or |b|^2 [5] next rho
+  // without the preconditioner P_ = p and U_ = u + q = t (modIterativo.f90:408, 424): the
+  // matvecs read p and t themselves
+  double2 *d_b = h->d_w[0], *r = h->d_w[1], *R_ = h->d_w[2], *u = h->d_w[3], *p = h->d_w[4], *V = h->d_w[6],
+          *q = h->d_w[7], *Q = h->d_w[9], *t = h->d_w[10], *xv = h->d_x;
+  double2 *P_ = use_prec ? h->d_w[5] : p, *U = use_prec ? h->d_w[8] : t;
+  // device scalars: [0] rho1 [1] rho2 [2] den [3] alfa [4] |r|^2 or |b|^2 [5] next rho,
+  // and the stop flag of k_cgs_check
   double2* s = h->d_scal;
+  int* stop = reinterpret_cast<int*>(s + 8);
   LW_HIP(hipMemcpyAsync(d_b, b, bytes, hipMemcpyHostToDevice, st));
   if (guess)
     LW_HIP(hipMemcpyAsync(xv, guess, bytes, hipMemcpyHostToDevice, st));
@@ -1566,55 +1606,64 @@ int lw_cgs(lw_handle* hh, const double* b, const double* guess, double errmax, i
   LW_HIP(hipMemcpyAsync(hs, s + 4, sizeof(hs), hipMemcpyDeviceToHost, st));
   LW_HIP(hipStreamSynchronize(st));
   const double nb = std::sqrt(hs[0].x);
-  double2 rho1 = hs[1];
   // ||r|| / ||b||; a zero right-hand side counts a zero residual as 0
   auto relative = [nb](double nr) { return nb > 0.0 ? nr / nb : (nr == 0.0 ? 0.0 : HUGE_VAL); };
-  double nr = std::sqrt(hs[2].x);
+  double r2 = hs[2].x, nr = std::sqrt(r2);
   double rr = relative(nr);
+  const double thr = lw::sq_threshold(errmax * nb);
   LW_HIP(hipMemcpyAsync(s + 0, s + 5, sizeof(double2), hipMemcpyDeviceToDevice, st));
-  int it = 0;
-  bool conv = false;
-  while (it < maxit) {
-    ++it;
-    if (rho1.x == 0.0 && rho1.y == 0.0) {
-      // breakdown exit (modIterativo.f90:392-395): x keeps the last iterate,
-      // which has converged if its residual passes the stopping test (an
-      // exact guess, or a zero right-hand side from a zero guess)
-      conv = nr < errmax * nb || nr == 0.0;
-      break;
-    }
-    hipLaunchKernelGGL(lw::k_cgs_up, dim3(vb), dim3(256), 0, st, n, it == 1 ? 1 : 0, s + 0, s + 1, r, q, u, p);
-    if (use_prec) {
-      if ((rc = lw::run_lusol(h, p, P_))) return rc;
-    } else {
-      LW_HIP(hipMemcpyAsync(P_, p, bytes, hipMemcpyDeviceToDevice, st));
-    }
-    if ((rc = lw::run_matvec(h, P_, V, nullptr, nullptr))) return rc;
-    if ((rc = lw::run_dot2(h, R_, V, 0, nullptr, nullptr, 0, s + 2))) return rc;
+  LW_HIP(hipMemsetAsync(stop, 0, sizeof(int), st));
+  // one iteration (modIterativo.f90:396-437), its |r|^2 and next rho copied
+  // to pinned slot i % 2, then the event of that slot
+  auto iteration = [&](int i) -> int {
+    hipLaunchKernelGGL(lw::k_cgs_up, dim3(vb), dim3(256), 0, st, n, i == 1 ? 1 : 0, s + 0, s + 1, r, q, u, p);
+    int e;
+    if (use_prec && (e = lw::run_lusol(h, p, P_))) return e;
+    if ((e = lw::run_matvec(h, P_, V, nullptr, nullptr))) return e;
+    if ((e = lw::run_dot2(h, R_, V, 0, nullptr, nullptr, 0, s + 2))) return e;
     hipLaunchKernelGGL(lw::k_cgs_q, dim3(vb), dim3(256), 0, st, n, s + 0, s + 2, s + 3, u, V, q, t);
-    if (use_prec) {
-      if ((rc = lw::run_lusol(h, t, U))) return rc;
-    } else {
-      LW_HIP(hipMemcpyAsync(U, t, bytes, hipMemcpyDeviceToDevice, st));
-    }
-    if ((rc = lw::run_matvec(h, U, Q, nullptr, nullptr))) return rc;
-    hipLaunchKernelGGL(lw::k_cgs_xr, dim3(vb), dim3(256), 0, st, n, s + 3, U, Q, xv, r);
+    if (use_prec && (e = lw::run_lusol(h, t, U))) return e;
+    if ((e = lw::run_matvec(h, U, Q, nullptr, nullptr))) return e;
+    hipLaunchKernelGGL(lw::k_cgs_xr, dim3(vb), dim3(256), 0, st, n, s + 3, U, Q, xv, r, stop);
     // |r|^2 and the next rho = sum(r*R_)
-    if ((rc = lw::run_dot2(h, r, r, 1, r, R_, 0, s + 4))) return rc;
-    LW_HIP(hipMemcpyAsync(hs, s + 4, sizeof(hs), hipMemcpyDeviceToHost, st));
-    LW_HIP(hipStreamSynchronize(st));
-    nr = std::sqrt(hs[0].x);
-    rr = relative(nr);
-    if (nr < errmax * nb) {
-      conv = true;
-      break;
+    if ((e = lw::run_dot2(h, r, r, 1, r, R_, 0, s + 4))) return e;
+    hipLaunchKernelGGL(lw::k_cgs_check, dim3(1), dim3(1), 0, st, thr, s, stop);
+    LW_HIP(hipMemcpyAsync(h->h_cgs + 2 * (i & 1), s + 4, 2 * sizeof(double2), hipMemcpyDeviceToHost, st));
+    LW_HIP(hipEventRecord(h->ev_cgs[i & 1], st));
+    return LW_OK;
+  };
+  int it = 1;
+  bool conv = false;
+  // the breakdown exit (modIterativo.f90:392-395) keeps the last iterate, which
+  // has converged if its residual passes the stopping test (an exact guess, or
+  // a zero right-hand side from a zero guess)
+  if (hs[1].x == 0.0 && hs[1].y == 0.0) {
+    conv = r2 < thr || nr == 0.0;
+  } else {
+    if ((rc = iteration(1))) return rc;
+    for (;;) {
+      // iteration it + 1 is queued before iteration it is read back, so the
+      // GPU does not wait for the host; when the host stops after it,
+      // k_cgs_check has set the flag that makes it leave x alone
+      if (it < maxit && (rc = iteration(it + 1))) return rc;
+      LW_HIP(hipEventSynchronize(h->ev_cgs[it & 1]));
+      r2 = h->h_cgs[2 * (it & 1)].x;
+      const double2 next = h->h_cgs[2 * (it & 1) + 1];
+      nr = std::sqrt(r2);
+      rr = relative(nr);
+      if (r2 < thr) {
+        conv = true;
+        break;
+      }
+      if (lw::progress_every() > 0 && it % lw::progress_every() == 0)
+        std::fprintf(stderr, "[lw_cgs] iteration %d: relres %.3e\n", it, rr);
+      if (it == maxit) break;
+      ++it;
+      if (next.x == 0.0 && next.y == 0.0) {
+        conv = nr == 0.0;
+        break;
+      }
     }
-    if (lw::progress_every() > 0 && it % lw::progress_every() == 0)
-      std::fprintf(stderr, "[lw_cgs] iteration %d: relres %.3e\n", it, rr);
-    // rho2 = rho1; rho1 = next (modIterativo.f90:486)
-    LW_HIP(hipMemcpyAsync(s + 1, s + 0, sizeof(double2), hipMemcpyDeviceToDevice, st));
-    LW_HIP(hipMemcpyAsync(s + 0, s + 5, sizeof(double2), hipMemcpyDeviceToDevice, st));
-    rho1 = hs[1];
   }
   LW_HIP(hipMemcpyAsync(x, xv, bytes, hipMemcpyDeviceToHost, st));
   LW_HIP(hipStreamSynchronize(st));

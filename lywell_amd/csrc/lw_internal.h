@@ -185,6 +185,9 @@ struct Handle {
   double* d_part = nullptr;   // reduction partials
   double2* d_scal = nullptr;  // CGS scalars
   int scal_cap = 0;
+  // pinned host slots of two CGS iterations (|r|^2, next rho) and their events
+  double2* h_cgs = nullptr;
+  hipEvent_t ev_cgs[2] = {nullptr, nullptr};
   // RCS state
   double2* d_cur = nullptr;
   double2 pol[3] = {};
@@ -247,6 +250,7 @@ int launch_lusol(Handle* h, const double2* y, double2* x);
 int lusol_flow_setup(Handle* h);
 // LW_ERR_STATE (and clears the flag) if a dataflow solve gave up a wait.
 int lusol_status(Handle* h);
+double sq_threshold(double t);
 // ILU(0) of the near field on the device (lw_ilu0.hip); *piv = 0 or the first
 // row with a zero pivot.
 int ilu0_device(Handle* h, int* piv);

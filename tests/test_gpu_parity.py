@@ -199,6 +199,28 @@ def test_cgs_first_iterations(name, use_prec):
     assert abs(rr - rr_o) <= 1e-8 * rr_o
 
 
+@pytest.mark.parametrize("use_prec", [False, True])
+def test_cgs_returns_the_iterate_that_stopped_it(use_prec):
+    # lw_cgs queues iteration i + 1 before it reads iteration i back; the
+    # solve that converges at iteration k must return exactly the iterate of
+    # a solve capped at k iterations with a test that never passes
+    mesh, gpu, _ = setup_case("sphere_1920")
+    b = gpu.plane_wave([0.2, 0.1, -1.0], [1, 0, 0.2])
+    # ILU0-CGS stalls near 1e-2 on this sphere (the oracle too): it reaches
+    # 0.05 after about 1,300 iterations
+    errmax = 0.05 if use_prec else 1e-3
+    if use_prec:
+        assert gpu.ilu0() == 0
+    x, it, rr, conv = gpu.cgs(b, errmax=errmax, use_prec=use_prec, maxit=3000)
+    assert conv and rr < errmax
+    x_k, it_k, rr_k, conv_k = gpu.cgs(b, errmax=0.0, use_prec=use_prec, maxit=it)
+    assert it_k == it and not conv_k
+    assert np.array_equal(x, x_k) and rr == rr_k
+    # one iteration short: not converged at the stopping test
+    _, _, rr_s, conv_s = gpu.cgs(b, errmax=errmax, use_prec=use_prec, maxit=it - 1)
+    assert not conv_s and rr_s >= errmax
+
+
 def test_cgs_converges_and_rcs():
     mesh, gpu, ora = setup_case("sphere_1920")
     b = gpu.plane_wave([0, 0, -1.0], [1, 0, 0])
