@@ -608,18 +608,15 @@ __global__ void __launch_bounds__(256) k_copy(long long n, const double2* __rest
 
 constexpr int kRedBlocks = 512;
 
-// partial[b] = sum over the block's stripe of (a*b) (conj_a: conj(a)*b);
-// two products in one pass when a2/b2 are given.
-__global__ void __launch_bounds__(256) k_dot2(int n, const double2* __restrict__ a, const double2* __restrict__ b,
-                                              int conj_a, const double2* __restrict__ a2,
-                                              const double2* __restrict__ b2, int conj_a2,
-                                              double* __restrict__ part) {
+// Two grid-wide sums in fixed order: each block writes its partials (waves
+// in order), and the block that arrives last at *counter sums all partials
+// (lane b of wave 0 takes blocks b, b + 64, ..., then a butterfly), so a sum
+// has the same bits from run to run. True in wave 0 of that block, with the
+// sums in t1, t2; *counter is back at zero for the next launch. 256 threads.
+__device__ bool grid_sum2(double2 s1, double2 s2, double* __restrict__ part, int* __restrict__ counter,
+                          double2& t1, double2& t2) {
   __shared__ double2 sh[2][256 / kWave];
-  double2 s1 = make_double2(0.0, 0.0), s2 = make_double2(0.0, 0.0);
-  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-    s1 = cadd(s1, conj_a ? cmulc(a[i], b[i]) : cmul(a[i], b[i]));
-    if (a2) s2 = cadd(s2, conj_a2 ? cmulc(a2[i], b2[i]) : cmul(a2[i], b2[i]));
-  }
+  __shared__ int last;
   s1 = wave_sum(s1);
   s2 = wave_sum(s2);
   const int wv = threadIdx.x / kWave, ln = threadIdx.x % kWave;
@@ -629,31 +626,49 @@ __global__ void __launch_bounds__(256) k_dot2(int n, const double2* __restrict__
   }
   __syncthreads();
   if (threadIdx.x == 0) {
-    double2 t1 = make_double2(0.0, 0.0), t2 = make_double2(0.0, 0.0);
+    double2 b1 = make_double2(0.0, 0.0), b2 = make_double2(0.0, 0.0);
     for (int w = 0; w < (int)(blockDim.x / kWave); ++w) {
-      t1 = cadd(t1, sh[0][w]);
-      t2 = cadd(t2, sh[1][w]);
+      b1 = cadd(b1, sh[0][w]);
+      b2 = cadd(b2, sh[1][w]);
     }
-    part[4 * blockIdx.x + 0] = t1.x;
-    part[4 * blockIdx.x + 1] = t1.y;
-    part[4 * blockIdx.x + 2] = t2.x;
-    part[4 * blockIdx.x + 3] = t2.y;
+    part[4 * blockIdx.x + 0] = b1.x;
+    part[4 * blockIdx.x + 1] = b1.y;
+    part[4 * blockIdx.x + 2] = b2.x;
+    part[4 * blockIdx.x + 3] = b2.y;
+    // release: this block's partials before its count; acquire: the others'
+    last = __hip_atomic_fetch_add(counter, 1, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT) == (int)gridDim.x - 1;
   }
+  __syncthreads();
+  if (!last || threadIdx.x >= kWave) return false;
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+  t1 = make_double2(0.0, 0.0);
+  t2 = make_double2(0.0, 0.0);
+  for (int b = threadIdx.x; b < (int)gridDim.x; b += kWave) {
+    t1 = cadd(t1, make_double2(part[4 * b], part[4 * b + 1]));
+    t2 = cadd(t2, make_double2(part[4 * b + 2], part[4 * b + 3]));
+  }
+  t1 = wave_sum(t1);
+  t2 = wave_sum(t2);
+  if (threadIdx.x == 0) __hip_atomic_store(counter, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  return true;
 }
 
-// Fixed-order final sum of the partials: out[0] = first product, out[1] = second.
-__global__ void __launch_bounds__(kWave) k_dot_finish(int nblk, const double* __restrict__ part,
-                                                      double2* __restrict__ out) {
+// out[0] = sum(a*b) (conj_a: conj(a)*b), out[1] = sum(a2*b2) (0 without a2),
+// in one pass over the vectors.
+__global__ void __launch_bounds__(256) k_dot2(int n, const double2* __restrict__ a, const double2* __restrict__ b,
+                                              int conj_a, const double2* __restrict__ a2,
+                                              const double2* __restrict__ b2, int conj_a2,
+                                              double* __restrict__ part, int* __restrict__ counter,
+                                              double2* __restrict__ out) {
   double2 s1 = make_double2(0.0, 0.0), s2 = make_double2(0.0, 0.0);
-  for (int b = threadIdx.x; b < nblk; b += kWave) {
-    s1 = cadd(s1, make_double2(part[4 * b], part[4 * b + 1]));
-    s2 = cadd(s2, make_double2(part[4 * b + 2], part[4 * b + 3]));
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    s1 = cadd(s1, conj_a ? cmulc(a[i], b[i]) : cmul(a[i], b[i]));
+    if (a2) s2 = cadd(s2, conj_a2 ? cmulc(a2[i], b2[i]) : cmul(a2[i], b2[i]));
   }
-  s1 = wave_sum(s1);
-  s2 = wave_sum(s2);
-  if (threadIdx.x == 0) {
-    out[0] = s1;
-    out[1] = s2;
+  double2 t1, t2;
+  if (grid_sum2(s1, s2, part, counter, t1, t2) && threadIdx.x == 0) {
+    out[0] = t1;
+    out[1] = t2;
   }
 }
 
@@ -699,29 +714,45 @@ __global__ void k_cgs_q(int n, const double2* __restrict__ rho1p, const double2*
   t[i] = cadd(u[i], qi);
 }
 
-// x += alfa U_; r -= alfa Q_   (:434-437); nothing when *stop is set (the
-// iteration was enqueued before the host saw the previous one end the solve)
-__global__ void k_cgs_xr(int n, const double2* __restrict__ alfap, const double2* __restrict__ U,
-                         const double2* __restrict__ Q, double2* __restrict__ x, double2* __restrict__ r,
-                         const int* __restrict__ stop) {
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n || *stop) return;
-  const double2 a = *alfap;
-  x[i] = cadd(x[i], cmul(a, U[i]));
-  const double2 aq = cmul(a, Q[i]);
-  r[i] = make_double2(r[i].x - aq.x, r[i].y - aq.y);
-}
-
-// The host's two exits after an iteration, decided again on the device from
-// the same scalars: the stopping test |r|^2 < thr (thr chosen so that this is
-// sqrt(|r|^2) < errmax ||b||, modIterativo.f90:472) and a zero next rho (the
-// breakdown exit at the top of the next iteration, :392); then rho2 = rho1,
-// rho1 = next rho (:487). s[4] = |r|^2, s[5] = next rho.
-__global__ void k_cgs_check(double thr, double2* __restrict__ s, int* __restrict__ stop) {
-  const double2 nx = s[5];
-  *stop = (s[4].x < thr || (nx.x == 0.0 && nx.y == 0.0)) ? 1 : 0;
+// The end of an iteration. x += alfa U_; r -= alfa Q_ (:434-437), nothing
+// when *stop is set (the iteration was queued before the host saw the
+// previous one end the solve); |r|^2 and the next rho = sum(r*R_) summed as
+// k_dot2 sums them; then, in the last block, the host's two exits decided
+// again from the same sums: the stopping test |r|^2 < thr (thr chosen so that
+// this is sqrt(|r|^2) < errmax ||b||, :472) and a zero next rho (the breakdown
+// exit at the top of the next iteration, :392). rho2 = rho1, rho1 = next rho
+// (:487); s[4] = |r|^2 and s[5] = next rho, also written to the host slot.
+__global__ void __launch_bounds__(256) k_cgs_xr_dot(int n, const double2* __restrict__ U,
+                                                    const double2* __restrict__ Q, double2* __restrict__ x,
+                                                    double2* __restrict__ r, const double2* __restrict__ R_,
+                                                    double* __restrict__ part, int* __restrict__ counter,
+                                                    double thr, double2* __restrict__ s, int* __restrict__ stop,
+                                                    double2* __restrict__ host) {
+  // every block reads the flag before its count, the last block writes it after
+  const bool skip = *stop != 0;
+  const double2 a = s[3];
+  double2 s1 = make_double2(0.0, 0.0), s2 = make_double2(0.0, 0.0);
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+    double2 ri = r[i];
+    if (!skip) {
+      x[i] = cadd(x[i], cmul(a, U[i]));
+      const double2 aq = cmul(a, Q[i]);
+      ri = make_double2(ri.x - aq.x, ri.y - aq.y);
+      r[i] = ri;
+    }
+    s1 = cadd(s1, cmulc(ri, ri));
+    s2 = cadd(s2, cmul(ri, R_[i]));
+  }
+  double2 t1, t2;
+  if (!grid_sum2(s1, s2, part, counter, t1, t2) || threadIdx.x != 0) return;
+  s[4] = t1;
+  s[5] = t2;
+  *stop = (t1.x < thr || (t2.x == 0.0 && t2.y == 0.0)) ? 1 : 0;
   s[1] = s[0];
-  s[0] = nx;
+  s[0] = t2;
+  host[0] = t1;
+  host[1] = t2;
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "");
 }
 
 __global__ void k_sub(int n, const double2* __restrict__ a, const double2* __restrict__ b, double2* __restrict__ o) {
@@ -963,11 +994,14 @@ int run_lusol(Handle* h, const double2* y, double2* x) {
   return launch_lusol(h, y, x);
 }
 
+// the grid counter of k_dot2 and k_cgs_xr_dot, in the CGS scalar block
+int* red_counter(Handle* h) { return reinterpret_cast<int*>(h->d_scal + 10); }
+
 // dot products: out[0] = sum(a*b) (conj a if ca), out[1] = sum(a2*b2)
 int run_dot2(Handle* h, const double2* a, const double2* b, int ca, const double2* a2, const double2* b2, int ca2,
              double2* out) {
-  hipLaunchKernelGGL(k_dot2, dim3(kRedBlocks), dim3(256), 0, h->stream, h->ne, a, b, ca, a2, b2, ca2, h->d_part);
-  hipLaunchKernelGGL(k_dot_finish, dim3(1), dim3(kWave), 0, h->stream, kRedBlocks, h->d_part, out);
+  hipLaunchKernelGGL(k_dot2, dim3(kRedBlocks), dim3(256), 0, h->stream, h->ne, a, b, ca, a2, b2, ca2, h->d_part,
+                     red_counter(h), out);
   LW_HIP(hipGetLastError());
   return LW_OK;
 }
@@ -1183,7 +1217,10 @@ static int init_device_state(Handle* h, const Geometry& g) {
   LW_HIP(hipMemsetAsync(h->d_lusol_err, 0, sizeof(int), h->stream));
   h->scal_cap = 64;
   TRY(dmalloc(h, &h->d_scal, (size_t)h->scal_cap));
+  // CGS scalars, stop flag and grid counter start at zero
+  LW_HIP(hipMemsetAsync(h->d_scal, 0, sizeof(double2) * h->scal_cap, h->stream));
   LW_HIP(hipHostMalloc((void**)&h->h_cgs, 4 * sizeof(double2), hipHostMallocDefault));
+  LW_HIP(hipHostGetDevicePointer((void**)&h->d_cgs_host, h->h_cgs, 0));
   for (hipEvent_t& e : h->ev_cgs) LW_HIP(hipEventCreateWithFlags(&e, hipEventDisableTiming));
   LW_HIP(hipStreamSynchronize(h->stream));
   // large LDS requests for the downward kernel at the top level
@@ -1584,7 +1621,7 @@ int lw_cgs(lw_handle* hh, const double* b, const double* guess, double errmax, i
           *q = h->d_w[7], *Q = h->d_w[9], *t = h->d_w[10], *xv = h->d_x;
   double2 *P_ = use_prec ? h->d_w[5] : p, *U = use_prec ? h->d_w[8] : t;
   // device scalars: [0] rho1 [1] rho2 [2] den [3] alfa [4] |r|^2 or |b|^2 [5] next rho,
-  // and the stop flag of k_cgs_check
+  // and the stop flag of k_cgs_xr_dot
   double2* s = h->d_scal;
   int* stop = reinterpret_cast<int*>(s + 8);
   LW_HIP(hipMemcpyAsync(d_b, b, bytes, hipMemcpyHostToDevice, st));
@@ -1613,7 +1650,7 @@ int lw_cgs(lw_handle* hh, const double* b, const double* guess, double errmax, i
   const double thr = lw::sq_threshold(errmax * nb);
   LW_HIP(hipMemcpyAsync(s + 0, s + 5, sizeof(double2), hipMemcpyDeviceToDevice, st));
   LW_HIP(hipMemsetAsync(stop, 0, sizeof(int), st));
-  // one iteration (modIterativo.f90:396-437), its |r|^2 and next rho copied
+  // one iteration (modIterativo.f90:396-437), its |r|^2 and next rho written
   // to pinned slot i % 2, then the event of that slot
   auto iteration = [&](int i) -> int {
     hipLaunchKernelGGL(lw::k_cgs_up, dim3(vb), dim3(256), 0, st, n, i == 1 ? 1 : 0, s + 0, s + 1, r, q, u, p);
@@ -1624,11 +1661,9 @@ int lw_cgs(lw_handle* hh, const double* b, const double* guess, double errmax, i
     hipLaunchKernelGGL(lw::k_cgs_q, dim3(vb), dim3(256), 0, st, n, s + 0, s + 2, s + 3, u, V, q, t);
     if (use_prec && (e = lw::run_lusol(h, t, U))) return e;
     if ((e = lw::run_matvec(h, U, Q, nullptr, nullptr))) return e;
-    hipLaunchKernelGGL(lw::k_cgs_xr, dim3(vb), dim3(256), 0, st, n, s + 3, U, Q, xv, r, stop);
-    // |r|^2 and the next rho = sum(r*R_)
-    if ((e = lw::run_dot2(h, r, r, 1, r, R_, 0, s + 4))) return e;
-    hipLaunchKernelGGL(lw::k_cgs_check, dim3(1), dim3(1), 0, st, thr, s, stop);
-    LW_HIP(hipMemcpyAsync(h->h_cgs + 2 * (i & 1), s + 4, 2 * sizeof(double2), hipMemcpyDeviceToHost, st));
+    hipLaunchKernelGGL(lw::k_cgs_xr_dot, dim3(lw::kRedBlocks), dim3(256), 0, st, n, U, Q, xv, r, R_, h->d_part,
+                       lw::red_counter(h), thr, s, stop, h->d_cgs_host + 2 * (i & 1));
+    LW_HIP(hipGetLastError());
     LW_HIP(hipEventRecord(h->ev_cgs[i & 1], st));
     return LW_OK;
   };
@@ -1644,7 +1679,7 @@ int lw_cgs(lw_handle* hh, const double* b, const double* guess, double errmax, i
     for (;;) {
       // iteration it + 1 is queued before iteration it is read back, so the
       // GPU does not wait for the host; when the host stops after it,
-      // k_cgs_check has set the flag that makes it leave x alone
+      // k_cgs_xr_dot has set the flag that makes it leave x alone
       if (it < maxit && (rc = iteration(it + 1))) return rc;
       LW_HIP(hipEventSynchronize(h->ev_cgs[it & 1]));
       r2 = h->h_cgs[2 * (it & 1)].x;

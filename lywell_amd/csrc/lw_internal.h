@@ -187,6 +187,7 @@ struct Handle {
   int scal_cap = 0;
   // pinned host slots of two CGS iterations (|r|^2, next rho) and their events
   double2* h_cgs = nullptr;
+  double2* d_cgs_host = nullptr;  // the same slots' device address
   hipEvent_t ev_cgs[2] = {nullptr, nullptr};
   // RCS state
   double2* d_cur = nullptr;
