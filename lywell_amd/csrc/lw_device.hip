@@ -156,10 +156,12 @@ __global__ void __launch_bounds__(256) k_aggregate_ph(int nk, const int* __restr
 //         and of l/2 x(f) rho- over those whose minus triangle is T,
 // so each phase row of the box is read once and the pattern is never formed
 // (the terms are summed in another order than the reference's; same value
-// to rounding).  One workgroup per leaf box: H of its rows in LDS (a thread
-// per row over the bases the row serves, in slot order), then one thread
-// per sample.
-__global__ void __launch_bounds__(256) k_aggregate_fac(int nk, const int* __restrict__ box_row,
+// to rounding).  The rows hold the samples m < nk/2; the antipodal sample
+// m' = nk/2 + sample_flip(m) has the conjugate phase, so one load serves
+// both: a(m) = sum_T e_T(m) H_T, a(m') = sum_T conj(e_T(m)) H_T.  One
+// workgroup per leaf box: H of its rows in LDS (a thread per row over the
+// bases the row serves, in slot order), then one thread per sample pair.
+__global__ void __launch_bounds__(256) k_aggregate_fac(int nk, int nth, const int* __restrict__ box_row,
                                                        const int* __restrict__ row_ptr,
                                                        const int* __restrict__ row_ent,
                                                        const int* __restrict__ slot_basis,
@@ -192,35 +194,44 @@ __global__ void __launch_bounds__(256) k_aggregate_fac(int nk, const int* __rest
     for (int d = 0; d < 3; ++d) hs[3 * t + d] = h[d];
   }
   __syncthreads();
-  for (int k = threadIdx.x; k < nk; k += blockDim.x) {
-    double2 a[3];
+  const int nh = nk / 2;
+  for (int m = threadIdx.x; m < nh; m += blockDim.x) {
+    double2 a[3], am[3];
 #pragma unroll
-    for (int d = 0; d < 3; ++d) a[d] = make_double2(0.0, 0.0);
+    for (int d = 0; d < 3; ++d) a[d] = am[d] = make_double2(0.0, 0.0);
     constexpr int kU = 8;  // phase loads issued together
     for (int t0 = 0; t0 < nr; t0 += kU) {
       double2 e[kU];
 #pragma unroll
       for (int u = 0; u < kU; ++u)
-        if (t0 + u < nr) e[u] = tph[(size_t)(r0 + t0 + u) * nk + k];
+        if (t0 + u < nr) e[u] = tph[(size_t)(r0 + t0 + u) * nh + m];
 #pragma unroll
       for (int u = 0; u < kU; ++u)
         if (t0 + u < nr) {
           const double2* h = hs + 3 * (t0 + u);
 #pragma unroll
-          for (int d = 0; d < 3; ++d) a[d] = cadd(a[d], cmul(e[u], h[d]));
+          for (int d = 0; d < 3; ++d) {
+            a[d] = cadd(a[d], cmul(e[u], h[d]));
+            am[d] = cadd(am[d], cmulc(e[u], h[d]));
+          }
         }
     }
-    double2 c0 = make_double2(0.0, 0.0), c1 = c0;
+    // campoC at sample k from the 3-vector v: theta-hat . v, phi-hat . v
+    auto store = [&](int k, const double2 (&v)[3]) {
+      double2 c0 = make_double2(0.0, 0.0), c1 = c0;
 #pragma unroll
-    for (int d = 0; d < 3; ++d) {
-      const double t = ath[3 * k + d], p = aph[3 * k + d];
-      c0.x += t * a[d].x;
-      c0.y += t * a[d].y;
-      c1.x += p * a[d].x;
-      c1.y += p * a[d].y;
-    }
-    C[((size_t)b * 2 + 0) * nk + k] = c0;
-    C[((size_t)b * 2 + 1) * nk + k] = c1;
+      for (int d = 0; d < 3; ++d) {
+        const double t = ath[3 * k + d], p = aph[3 * k + d];
+        c0.x += t * v[d].x;
+        c0.y += t * v[d].y;
+        c1.x += p * v[d].x;
+        c1.y += p * v[d].y;
+      }
+      C[((size_t)b * 2 + 0) * nk + k] = c0;
+      C[((size_t)b * 2 + 1) * nk + k] = c1;
+    };
+    store(m, a);
+    store(nh + sample_flip(m, nth), am);
   }
 }
 
@@ -512,13 +523,16 @@ __global__ void __launch_bounds__(256) k_near_box(const int* __restrict__ box_sl
 // over samples and polarisations factors as
 //   sum_k sum_p conj(funcRAD(k,p,f)) campoA(k,p) = l/2 (rho+ . G+ + rho- . G-),
 //   G_T = sum_k conj(e_T(k)) F(k),  F(k) = sum_p u_p(k) campoA(k,p),
-// so each phase row is read once (0.44 of the pattern bytes at 122,880
-// unknowns) and contracted with the box's 3-vector field F in LDS.  The
-// terms are summed in another order than the reference's (same value to
-// rounding).  y(f) = (j w mu / 4 pi) (...) + y_near(f) when y_near is given
-// (the near sum of k_near_part, added as k_receive_near adds it), else the
-// far part alone.
-__global__ void __launch_bounds__(256) k_receive_ph(int nk, const int* __restrict__ box_slot,
+// so each phase row is read once and contracted with the box's 3-vector
+// field F in LDS.  The rows hold the samples m < nk/2, and the antipodal
+// sample m' = nk/2 + sample_flip(m) has the conjugate phase, so
+//   G_T = sum_{m < nk/2} conj(e_T(m)) F(m) + e_T(m) F(m'),
+// 0.22 of the pattern bytes at 122,880 unknowns.  The terms are summed in
+// another order than the reference's (same value to rounding).
+// y(f) = (j w mu / 4 pi) (...) + y_near(f) when y_near is given (the near
+// sum of k_near_part, added as k_receive_near adds it), else the far part
+// alone.
+__global__ void __launch_bounds__(256) k_receive_ph(int nk, int nth, const int* __restrict__ box_slot,
                                                     const int* __restrict__ box_row,
                                                     const int* __restrict__ slot_basis,
                                                     const int2* __restrict__ slot_pair,
@@ -529,16 +543,19 @@ __global__ void __launch_bounds__(256) k_receive_ph(int nk, const int* __restric
                                                     const double2* __restrict__ AL, double2 cc,
                                                     const double2* __restrict__ y_near, double2* __restrict__ y) {
   extern __shared__ double2 sm[];
-  double2* F = sm;           // [3][nk]
-  double2* G = sm + 3 * nk;  // [row of the box][3]
+  const int nh = nk / 2;
+  double2* F = sm;            // [3][nh]: F(m), m < nh
+  double2* Fm = sm + 3 * nh;  // [3][nh]: F of m's antipode
+  double2* G = sm + 3 * nk;   // [row of the box][3]
   const int b = xcd_block(blockIdx.x, gridDim.x);
   const double2* a = AL + (size_t)b * 2 * nk;
   for (int k = threadIdx.x; k < nk; k += blockDim.x) {
     const double2 a0 = a[k], a1 = a[nk + k];
+    double2* f = k < nh ? F + k : Fm + sample_flip(k - nh, nth);
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
       const double t = ath[3 * k + d], p = aph[3 * k + d];
-      F[d * nk + k] = make_double2(t * a0.x + p * a1.x, t * a0.y + p * a1.y);
+      f[d * nh] = make_double2(t * a0.x + p * a1.x, t * a0.y + p * a1.y);
     }
   }
   __syncthreads();
@@ -547,23 +564,25 @@ __global__ void __launch_bounds__(256) k_receive_ph(int nk, const int* __restric
   // samples j, j + 16, ...; the group's partial sums are combined by DPP
   const int j = threadIdx.x % 16, ng = blockDim.x / 16;
   for (int r = r0 + (int)(threadIdx.x / 16); r < r1; r += ng) {
-    const double2* ph = tph + (size_t)r * nk;
+    const double2* ph = tph + (size_t)r * nh;
     double g[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
     constexpr int kU = 8;  // row loads issued together
-    for (int kb = j; kb < nk; kb += kU * 16) {
+    for (int kb = j; kb < nh; kb += kU * 16) {
       double2 p[kU];
 #pragma unroll
       for (int u = 0; u < kU; ++u)
-        if (kb + 16 * u < nk) p[u] = ph[kb + 16 * u];
+        if (kb + 16 * u < nh) p[u] = ph[kb + 16 * u];
 #pragma unroll
       for (int u = 0; u < kU; ++u) {
-        const int k = kb + 16 * u;
-        if (k < nk) {
+        const int m = kb + 16 * u;
+        if (m < nh) {
 #pragma unroll
           for (int d = 0; d < 3; ++d) {
-            const double2 v = cmulc(p[u], F[d * nk + k]);
+            const double2 v = cmulc(p[u], F[d * nh + m]), w = cmul(p[u], Fm[d * nh + m]);
             g[2 * d] += v.x;
             g[2 * d + 1] += v.y;
+            g[2 * d] += w.x;
+            g[2 * d + 1] += w.y;
           }
         }
       }
@@ -870,7 +889,8 @@ int run_matvec(Handle* h, const double2* x, double2* y, double2* y_near, hipEven
   double2* yn = y_near ? y_near : h->d_ynear;
   auto aggregate = [&](hipStream_t s) {
     if (h->agg_lds)
-      hipLaunchKernelGGL(k_aggregate_fac, dim3(lf.nb), dim3(256), h->agg_lds, s, lf.nk, h->d_box_row, h->d_row_ptr,
+      hipLaunchKernelGGL(k_aggregate_fac, dim3(lf.nb), dim3(h->ph_len <= 128 ? 128 : 256), h->agg_lds, s, lf.nk,
+                         h->plan.lv[L].nth, h->d_box_row, h->d_row_ptr,
                          h->d_row_ent, h->d_slot_basis, h->d_rgeo, h->d_ath, h->d_aph, h->d_tph, x, lf.C);
     else
       hipLaunchKernelGGL(k_aggregate_ph, dim3(lf.nb), dim3(256), 0, s, lf.nk, h->d_box_slot, h->d_slot_basis,
@@ -918,7 +938,8 @@ int run_matvec(Handle* h, const double2* x, double2* y, double2* y_near, hipEven
   };
   auto receive = [&](hipStream_t s) {
     if (h->recv_lds) {
-      hipLaunchKernelGGL(k_receive_ph, dim3(lf.nb), dim3(256), h->recv_lds, s, lf.nk, h->d_box_slot, h->d_box_row,
+      hipLaunchKernelGGL(k_receive_ph, dim3(lf.nb), dim3(256), h->recv_lds, s, lf.nk, h->plan.lv[L].nth,
+                         h->d_box_slot, h->d_box_row,
                          h->d_slot_basis, h->d_slot_pair, h->d_rgeo, h->d_ath, h->d_aph, h->d_tph, lf.A, cc,
                          y_near ? nullptr : yn, y);
     } else if (h->fp64_near) {
@@ -1919,7 +1940,7 @@ int lw_matvec_stage_bytes(lw_handle* hh, double* sb) {
   auto field = [&](int n) { return (double)P.lv[n].nk * 2 * P.lv[n].nb * 16.0; };
   // the phase rows (one per leaf box and triangle of its bases) and the
   // per-basis RWG vectors, row indices and slot order
-  const double phases = (double)h->npairs * P.lv[L].nk * 16.0 +
+  const double phases = (double)h->npairs * h->ph_len * 16.0 +
                         (double)P.ne * (sizeof(lw::RadGeom) + sizeof(int2) + sizeof(int));
   const double zb = h->fp64_near ? 16.0 : 8.0;
   const double csr = (zb + 4.0) * (double)P.nnz + 4.0 * (P.ne + 1);

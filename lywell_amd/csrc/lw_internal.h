@@ -56,6 +56,16 @@ __device__ inline void rad_eval(const RadGeom& g, const double* __restrict__ vk,
   rad_from_phase(g, rad_phase(vk, g.dp, ka), rad_phase(vk, g.dm, ka), at, ap, r0, r1);
 }
 
+// Samples s = iphi * nth + itheta (theta = (itheta + 1/2) pi / nth, phi =
+// (iphi + 1/2) 2 pi / nph, nph = 2 nth).  Sample s < nk/2 and sample
+// nk/2 + sample_flip(s, nth) (phi + pi, pi - theta) have opposite directions
+// khat, so exp(j ka khat . d) of the second is the conjugate of the first's.
+// sample_flip(u) reverses theta within u's phi column (an involution).
+__host__ __device__ __forceinline__ int sample_flip(int u, int nth) {
+  const int q = u / nth;
+  return q * nth + (nth - 1 - (u - q * nth));
+}
+
 // Workgroup id w of n -> logical block: the hardware deals workgroups to the
 // 8 XCDs round-robin (XCD = w mod 8); this bijection gives each XCD one
 // contiguous range of logical blocks, so neighbouring blocks (which share
@@ -147,7 +157,10 @@ struct Handle {
   // C)) per (leaf box, triangle of one of its bases) and leaf sample, and per
   // basis slot the phase rows of its plus and minus triangles
   int npairs = 0;
-  double2* d_tph = nullptr;   // (npairs, nk_L) phase rows
+  double2* d_tph = nullptr;   // (npairs, ph_len) phase rows
+  // phase-row length: nk_L / 2 for the factored aggregation and receive
+  // (the samples with iphi < nph/2; the others are their antipodes), else nk_L
+  int ph_len = 0;
   int2* d_slot_pair = nullptr; // (ne) phase rows of each slot's two triangles
   double *d_ath = nullptr, *d_aph = nullptr;  // (3, nk_L) theta and phi unit vectors of the leaf samples
   int *d_slot_basis = nullptr, *d_basis_slot = nullptr, *d_basis_box = nullptr, *d_box_slot = nullptr;

@@ -58,14 +58,15 @@ __global__ void k_rad_geom(int ne, const int* __restrict__ slot_basis, const int
 }
 
 // The phase rows of the matvec's pattern form: row p = exp(j ka khat_k .
-// (c_T - C_b)) for the triangle T and leaf box b of pair p, at every leaf
-// sample k (the factors calcRad multiplies, with its operation order).
-__global__ void k_tri_phase(int npairs, int nk, const int* __restrict__ pair_tri, const int* __restrict__ pair_box,
+// (c_T - C_b)) for the triangle T and leaf box b of pair p, at the leaf
+// samples k < len (all of them, or the first half, whose antipodes give the
+// rest), the factors calcRad multiplies, with its operation order.
+__global__ void k_tri_phase(int npairs, int len, const int* __restrict__ pair_tri, const int* __restrict__ pair_box,
                             const double* __restrict__ box_c, const double* __restrict__ tb,
                             const double* __restrict__ kap, double ka, double2* __restrict__ tph) {
   const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (idx >= (long long)npairs * nk) return;
-  const int p = (int)(idx / nk), k = (int)(idx % nk);
+  if (idx >= (long long)npairs * len) return;
+  const int p = (int)(idx / len), k = (int)(idx % len);
   const double* t = tb + 3 * pair_tri[p];
   const double* c = box_c + 3 * pair_box[p];
   double d[3];
@@ -261,6 +262,14 @@ int launch_setup_phases(Handle* h, const int* e_t) {
   const long long lds = 3LL * (lf.nk + maxrows) * (long long)sizeof(double2);
   h->recv_lds = lds <= kRecvLdsMax ? (int)lds : 0;
   h->agg_lds = h->recv_lds ? 3 * maxrows * (int)sizeof(double2) : 0;
+  // the factored aggregation and receive read half rows (the antipodal
+  // samples from the conjugates); the pattern-by-pattern kernels full rows
+  const bool half = h->recv_lds && !(h->flags & LW_PATTERN_ORDER) && lf.nph == 2 * lf.nth && lf.nk == lf.nth * lf.nph;
+  if (h->recv_lds && !(h->flags & LW_PATTERN_ORDER) && !half) {
+    set_error("lw_mlfma_init: leaf sample grid without antipodal pairs");
+    return LW_ERR_ARG;
+  }
+  h->ph_len = half ? lf.nk / 2 : lf.nk;
   // the bases each phase row serves: per row, entries 2 s + side (s the
   // basis slot, side 0 = its plus triangle, 1 = its minus triangle) in slot
   // order, plus then minus
@@ -281,10 +290,10 @@ int launch_setup_phases(Handle* h, const int* e_t) {
   int rc;
   if ((rc = upload(h, &d_pt, pair_tri)) || (rc = upload(h, &d_pb, pair_box)) || (rc = upload(h, &h->d_box_row, box_row)) ||
       (rc = upload(h, &h->d_row_ptr, row_ptr)) || (rc = upload(h, &h->d_row_ent, row_ent)) ||
-      (rc = upload(h, &h->d_slot_pair, slot_pair)) || (rc = dmalloc(h, &h->d_tph, (size_t)h->npairs * lf.nk)))
+      (rc = upload(h, &h->d_slot_pair, slot_pair)) || (rc = dmalloc(h, &h->d_tph, (size_t)h->npairs * h->ph_len)))
     return rc;
-  const long long tot = (long long)h->npairs * lf.nk;
-  hipLaunchKernelGGL(k_tri_phase, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, h->stream, h->npairs, lf.nk,
+  const long long tot = (long long)h->npairs * h->ph_len;
+  hipLaunchKernelGGL(k_tri_phase, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, h->stream, h->npairs, h->ph_len,
                      d_pt, d_pb, h->lv[h->L].centre, h->d_tb, h->lv[h->L].kap, (2.0 * kPi) / h->plan.phys.lambda,
                      h->d_tph);
   LW_HIP(hipGetLastError());
