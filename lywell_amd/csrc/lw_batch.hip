@@ -13,6 +13,10 @@
 //                      sample-major so one sample's slice of a level is
 //                      contiguous for the translation kernel)
 //   patterns           rad[(s*2 + p)*nk + k]       (leaf slot order)
+//   half patterns      rad_h[(s*2 + p)*nh + m]     (the samples m < nh =
+//                      nk/2; sample nh + sample_flip(m) has the pattern
+//                      conj(rad) (p = theta) or -conj(rad) (p = phi):
+//                      opposite khat, same theta-hat, opposite phi-hat)
 // The per-RHS CGS follows CGSold (modIterativo.f90:331-493) column by column:
 // a column that meets its tolerance (or breaks down) is frozen, so every
 // column's iterate is the one a single-RHS run would produce with this
@@ -89,6 +93,17 @@ __device__ __forceinline__ f64x4v mma(double a, double b, f64x4v c) {
 template <typename S>
 __device__ __forceinline__ int mma_row(int lk, int v) {
   return sizeof(S) == sizeof(float) ? 4 * lk + v : lk + 4 * v;
+}
+
+// out[(s*2 + p)*nh + k] = rad[(s*2 + p)*nk + k] for k < nh = nk/2, in the
+// far-field precision
+template <typename S>
+__global__ void k_rad_half(long long n, int nk, const double2* __restrict__ rad, typename Cx<S>::T* __restrict__ out) {
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const int nh = nk / 2;
+  const double2 v = rad[(i / nh) * nk + i % nh];
+  out[i] = mk<S>((S)v.x, (S)v.y);
 }
 
 template <typename S>
@@ -780,16 +795,22 @@ __global__ void k_b_wp(int nkc, int nwt, const int* __restrict__ wt_row, const d
 
 // Receive on MFMA, one workgroup per leaf box:
 //   Yf[basis(s)][r] = sum_m conj(rad(s; m)) A_L[m][b][r],  m = p*nk + k,
-// a complex GEMM with the 2nk items as the contraction.  A wave owns 16 of
+// a complex GEMM with the items as the contraction.  The half pattern rows
+// hold the items (p, k < nh); with a = rad(s; p, k), the antipode k' =
+// nh + sample_flip(k) and sg = +1 (p = theta) or -1 (p = phi), the item and
+// its antipode together add
+//   conj(a) A(k) + sg a A(k') = a.x U1 + a.y U2 + j (a.x U3 - a.y U4),
+//   U1 = A.x + sg A'.x, U2 = A.y - sg A'.y, U3 = A.y + sg A'.y, U4 = A.x - sg A'.x,
+// so the contraction runs over nk items instead of 2 nk.  A wave owns 16 of
 // the box's bases and 64 columns.  Each lane loads QB consecutive items of
 // one pattern row (coalesced pieces), so within a block of 4*QB items, MFMA
-// step q pairs lane group g with item QB*g + q; the A_L operand is fetched
+// step q pairs lane group g with item QB*g + q; the A_L operands are fetched
 // with the same assignment.
 // With Y given, the epilogue writes Y = (j w mu / 4 pi) Yf + Yn instead of Yf
 // (Yn: the near field of k_b_near_box), the sum k_b_near_box forms from a
 // stored Yf, operand for operand.
 template <typename S, int R>
-__global__ void __launch_bounds__(128) k_b_receive(int nk, int nb, const int* __restrict__ box_slot,
+__global__ void __launch_bounds__(128) k_b_receive(int nk, int nth, int nb, const int* __restrict__ box_slot,
                                                    const int* __restrict__ slot_basis,
                                                    const typename Cx<S>::T* __restrict__ rad,
                                                    const typename Cx<S>::T* __restrict__ AL,
@@ -800,14 +821,14 @@ __global__ void __launch_bounds__(128) k_b_receive(int nk, int nb, const int* __
   constexpr int NT = 4, NCH = R / 64;
   const int b = blockIdx.x;
   const int s0 = box_slot[b], nf = box_slot[b + 1] - s0;
-  const int nk2 = 2 * nk;
+  const int nh = nk / 2;  // items (p, k < nh): nk of them
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kWave), l = threadIdx.x % kWave;
   const int li = l % 16, lk = l / 16;
   const int nst = (nf + 15) / 16;
   for (int task = wave; task < nst * NCH; task += blockDim.x / kWave) {
     const int st = task / NCH, c0 = (task % NCH) * 64;
     const int sl = st * 16 + li;  // basis row this lane loads
-    const T2* rrow = rad + (size_t)(s0 + min(sl, nf - 1)) * nk2;
+    const T2* rrow = rad + (size_t)(s0 + min(sl, nf - 1)) * nk;
     const bool row_ok = sl < nf;
     Acc dr[NT], di[NT];
 #pragma unroll
@@ -819,49 +840,64 @@ __global__ void __launch_bounds__(128) k_b_receive(int nk, int nb, const int* __
     // the pattern pieces and the A_L operands, with clamped unconditional
     // loads (out-of-range items weigh 0), fetched a block ahead of the MFMAs
     // that use them
-    constexpr int QB = 2;
-    T2 aa[QB], ab[QB], xa[QB][NT], xb[QB][NT];
-    auto load = [&](int mb, T2(&a4)[QB], T2(&x)[QB][NT]) {
+    constexpr int QB = 1;
+    T2 aa[QB], ab[QB], xa[QB][NT], xb[QB][NT], ya[QB][NT], yb[QB][NT];
+    S ga[QB], gb[QB];
+    // item m: its pattern piece, A at the item and A at its antipode, and sg
+    auto load = [&](int mb, T2(&a4)[QB], T2(&x)[QB][NT], T2(&y)[QB][NT], S(&sg)[QB]) {
       const int m0 = mb + QB * lk;
 #pragma unroll
       for (int q = 0; q < QB; ++q) {
-        const int m = min(m0 + q, nk2 - 1);
+        const int m = min(m0 + q, nk - 1);
         const T2 av = rrow[m];
-        a4[q] = (row_ok && m0 + q < nk2) ? av : mk<S>(0, 0);
-        const int p = m / nk, k = m % nk;
+        a4[q] = (row_ok && m0 + q < nk) ? av : mk<S>(0, 0);
+        const int p = m / nh, k = m % nh, ka = nh + sample_flip(k, nth);
+        sg[q] = p == 0 ? S(1) : S(-1);
         const T2* ap = AL + (((size_t)k * nb + b) * 2 + p) * R + c0 + li;
+        const T2* aq = AL + (((size_t)ka * nb + b) * 2 + p) * R + c0 + li;
 #pragma unroll
-        for (int jt = 0; jt < NT; ++jt) x[q][jt] = ap[jt * 16];
+        for (int jt = 0; jt < NT; ++jt) {
+          x[q][jt] = ap[jt * 16];
+          y[q][jt] = aq[jt * 16];
+        }
       }
     };
-    auto apply = [&](const T2(&a4)[QB], const T2(&x)[QB][NT]) {
+    auto apply = [&](const T2(&a4)[QB], const T2(&x)[QB][NT], const T2(&y)[QB][NT], const S(&sg)[QB]) {
       // the 2*NT accumulators in turn, so that no MFMA waits on the one
       // before it (each accumulator sees its terms in the same order)
 #pragma unroll
       for (int q = 0; q < QB; ++q) {
+        S u1[NT], u2[NT], u3[NT], u4[NT];
 #pragma unroll
-        for (int jt = 0; jt < NT; ++jt) dr[jt] = mma(a4[q].x, x[q][jt].x, dr[jt]);
+        for (int jt = 0; jt < NT; ++jt) {
+          u1[jt] = x[q][jt].x + sg[q] * y[q][jt].x;
+          u2[jt] = x[q][jt].y - sg[q] * y[q][jt].y;
+          u3[jt] = x[q][jt].y + sg[q] * y[q][jt].y;
+          u4[jt] = x[q][jt].x - sg[q] * y[q][jt].x;
+        }
 #pragma unroll
-        for (int jt = 0; jt < NT; ++jt) di[jt] = mma(a4[q].x, x[q][jt].y, di[jt]);
+        for (int jt = 0; jt < NT; ++jt) dr[jt] = mma(a4[q].x, u1[jt], dr[jt]);
 #pragma unroll
-        for (int jt = 0; jt < NT; ++jt) dr[jt] = mma(a4[q].y, x[q][jt].y, dr[jt]);
+        for (int jt = 0; jt < NT; ++jt) di[jt] = mma(a4[q].x, u3[jt], di[jt]);
 #pragma unroll
-        for (int jt = 0; jt < NT; ++jt) di[jt] = mma(-a4[q].y, x[q][jt].x, di[jt]);
+        for (int jt = 0; jt < NT; ++jt) dr[jt] = mma(a4[q].y, u2[jt], dr[jt]);
+#pragma unroll
+        for (int jt = 0; jt < NT; ++jt) di[jt] = mma(-a4[q].y, u4[jt], di[jt]);
       }
     };
     constexpr int MB = 4 * QB;
-    load(0, aa, xa);
+    load(0, aa, xa, ya, ga);
     for (int mb = 0;;) {
-      load(min(mb + MB, nk2 - 1), ab, xb);
+      load(min(mb + MB, nk - 1), ab, xb, yb, gb);
       __builtin_amdgcn_sched_barrier(0);
-      apply(aa, xa);
+      apply(aa, xa, ya, ga);
       mb += MB;
-      if (mb >= nk2) break;
-      load(min(mb + MB, nk2 - 1), aa, xa);
+      if (mb >= nk) break;
+      load(min(mb + MB, nk - 1), aa, xa, ya, ga);
       __builtin_amdgcn_sched_barrier(0);
-      apply(ab, xb);
+      apply(ab, xb, yb, gb);
       mb += MB;
-      if (mb >= nk2) break;
+      if (mb >= nk) break;
     }
 #pragma unroll
     for (int v = 0; v < 4; ++v) {
@@ -1439,6 +1475,7 @@ struct Batch {
   int maxnf = 0;      // largest number of bases in a leaf box
   int *ncol_ptr = nullptr, *ncols = nullptr;  // near columns of each leaf box (shared by its rows)
   void* rad = nullptr;                        // leaf patterns in the far-field precision
+  void* rad_h = nullptr;                      // their half rows (the receive's)
   std::vector<BatchLevel> lv;
   // tile shapes per level, set at creation and by lw_batch_tune: the parent
   // ranges of the far kernel (1, 2, 4 or 8; see k_b_far), its variant
@@ -1677,8 +1714,9 @@ static void launch_leaf(Batch* bt, bool up, const double2* X, double2* Y = nullp
                        lf.nb, h->d_box_slot, h->d_slot_basis, (const T2*)bt->rad, X, (T2*)lf.C);
   } else {
     const double2 cc = make_double2(0.0, h->plan.phys.w * kMu / (4.0 * kPi));
-    hipLaunchKernelGGL((k_b_receive<S, R>), dim3(lf.nb), dim3(128), 0, h->stream, lf.nk, lf.nb, h->d_box_slot,
-                       h->d_slot_basis, (const T2*)bt->rad, (const T2*)lf.A, (T2*)bt->Yf, cc,
+    hipLaunchKernelGGL((k_b_receive<S, R>), dim3(lf.nb), dim3(128), 0, h->stream, lf.nk, h->plan.lv[h->L].nth,
+                       lf.nb, h->d_box_slot,
+                       h->d_slot_basis, (const T2*)bt->rad_h, (const T2*)lf.A, (T2*)bt->Yf, cc,
                        (const double2*)bt->Yn, Y);
   }
 }
@@ -2038,6 +2076,20 @@ static int setup_far(Batch* bt) {
     if ((rc = bmalloc(bt, &r, (size_t)nrad))) return rc;
     hipLaunchKernelGGL(k_to_prec<S>, dim3(nblk(nrad, 256)), dim3(256), 0, st, nrad, h->d_rad, r);
     bt->rad = r;
+  }
+  // the receive's half pattern rows (samples k < nk/2) in the far-field
+  // precision; the antipodal samples are their conjugates
+  const LevelPlan& ll = P.lv[L];
+  if (ll.nph != 2 * ll.nth || ll.nk != ll.nth * ll.nph) {
+    set_error("lw_batch_create: leaf sample grid without antipodal pairs");
+    return LW_ERR_ARG;
+  }
+  {
+    const long long nh = (long long)(ll.nk / 2) * 2 * h->ne;
+    T2* r = nullptr;
+    if ((rc = bmalloc(bt, &r, (size_t)nh))) return rc;
+    hipLaunchKernelGGL(k_rad_half<S>, dim3(nblk(nh, 256)), dim3(256), 0, st, nh, ll.nk, h->d_rad, r);
+    bt->rad_h = r;
   }
   for (int n = 3; n <= L; ++n) {
     const LevelPlan& lp = P.lv[n];
