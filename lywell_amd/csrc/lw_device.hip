@@ -968,17 +968,15 @@ int run_matvec(Handle* h, const double2* x, double2* y, double2* y_near, hipEven
     // critical path on the main stream: aggregation, upward passes, level
     // 3's translation, downward passes, receive.  Beside it, once the
     // aggregation is done (started earlier, the near field would share HBM
-    // with it): the near field (side 0), the leaf translation (side 1) and
-    // the translations of levels L-1..4 (side 2, each once its campoC is
-    // ready).  Measured at 122,880 unknowns: 0.59 ms against 0.67 ms with
-    // the same kernels one after the other; starting the side work after
-    // the upward passes instead gave 0.60-0.61 ms.
+    // with it): the near field and then level 4's translation (side 0), the
+    // leaf translation (side 1) and the translations of levels L-1..5 (side
+    // 2), each translation once its campoC is ready.  Each downward pass
+    // waits only for the translation of its own level.
     hipStream_t sn = h->side[0], sa = h->side[1], sb = h->side[2];
     aggregate(st);
     LW_HIP(hipEventRecord(h->ev_fork, st));
     LW_HIP(hipStreamWaitEvent(sn, h->ev_fork, 0));
     near(sn);
-    LW_HIP(hipEventRecord(h->ev_near, sn));
     const bool leaf_side = L > 3, mid_side = L > 4;
     if (leaf_side) {
       LW_HIP(hipStreamWaitEvent(sa, h->ev_fork, 0));
@@ -988,16 +986,24 @@ int run_matvec(Handle* h, const double2* x, double2* y, double2* y_near, hipEven
     for (int n = L; n >= 4; --n) {
       up(n, st);
       if (n - 1 >= 4) {
+        // level 4 behind the near field rather than behind level 5's
+        // translation, whose campoC is ready much earlier
+        hipStream_t s = n - 1 == 4 ? sn : sb;
         LW_HIP(hipEventRecord(h->ev_c[n - 1], st));
-        LW_HIP(hipStreamWaitEvent(sb, h->ev_c[n - 1], 0));
-        launch_translate_parent(h, n - 1, sb);
+        LW_HIP(hipStreamWaitEvent(s, h->ev_c[n - 1], 0));
+        launch_translate_parent(h, n - 1, s);
       }
     }
+    // near field and level 4's translation; levels L-1..5
+    LW_HIP(hipEventRecord(h->ev_near, sn));
     if (mid_side) LW_HIP(hipEventRecord(h->ev_tb, sb));
     launch_translate_parent(h, 3, st);
-    if (leaf_side) LW_HIP(hipStreamWaitEvent(st, h->ev_ta, 0));
-    if (mid_side) LW_HIP(hipStreamWaitEvent(st, h->ev_tb, 0));
-    for (int n = 4; n <= L; ++n) down(n, st);
+    for (int n = 4; n <= L; ++n) {
+      if (n == 4 && mid_side) LW_HIP(hipStreamWaitEvent(st, h->ev_near, 0));
+      if (n == 5 && L > 5) LW_HIP(hipStreamWaitEvent(st, h->ev_tb, 0));
+      if (n == L && leaf_side) LW_HIP(hipStreamWaitEvent(st, h->ev_ta, 0));
+      down(n, st);
+    }
     LW_HIP(hipStreamWaitEvent(st, h->ev_near, 0));
     receive(st);
   }
