@@ -909,9 +909,11 @@ int run_matvec(Handle* h, const double2* x, double2* y, double2* y_near, hipEven
     DevLevel& c = h->lv[n];
     DevLevel& p = h->lv[n - 1];
     const double2* src = (n == 4) ? p.B : p.A;
+    // LDS form: a thread per fine sample, 256 threads when the child has no
+    // more samples, so that more boxes share a CU
     if (2 * p.nk * (int)sizeof(double2) <= kDownLdsBytes)
-      hipLaunchKernelGGL(k_interp_down_lds, dim3(c.nb), dim3(512), 2 * p.nk * sizeof(double2), s, p.nk, c.nk,
-                         c.parent, c.oct, p.ph, p.wt_ptr, p.wt, src, c.B, c.A);
+      hipLaunchKernelGGL(k_interp_down_lds, dim3(c.nb), dim3(c.nk <= 256 ? 256 : 512), 2 * p.nk * sizeof(double2), s,
+                         p.nk, c.nk, c.parent, c.oct, p.ph, p.wt_ptr, p.wt, src, c.B, c.A);
     else
       hipLaunchKernelGGL(k_interp_down, dim3(c.nb * blocks_for(c.nk, 256)), dim3(256), 0, s, p.nk, c.nk,
                          c.parent, c.oct, p.ph, p.wt_ptr, p.wt, src, c.B, c.A);
