@@ -387,6 +387,68 @@ __global__ void __launch_bounds__(512) k_interp_down_lds(int nkc, int nkf, const
   }
 }
 
+// Downward pass for levels whose parent field does not fit LDS whole: a
+// workgroup takes one child box and 256 of its fine samples, whose stencils
+// read a cyclic range of the parent's phi columns (first column win[2 ch],
+// count win[2 ch + 1]).  That window of the phase-shifted parent field is
+// staged in LDS, and the stencil (rows relative to the window) gathers from
+// it with k_interp_down_lds's operations.
+__global__ void __launch_bounds__(256) k_interp_down_win(int nkc, int nkf, int nthc, int nphc,
+                                                         const int* __restrict__ parent, const int* __restrict__ oct,
+                                                         const double2* __restrict__ ph, const int* __restrict__ win,
+                                                         const int* __restrict__ wt_ptr,
+                                                         const WtEntry* __restrict__ wtw,
+                                                         const double2* __restrict__ Ap,
+                                                         const double2* __restrict__ Bf, double2* __restrict__ Af) {
+  extern __shared__ double2 lds[];  // 2 * (window columns) * nthc
+  const int nch = (nkf + 255) / 256;
+  const int wb = xcd_block(blockIdx.x, gridDim.x);
+  const int c = wb / nch, ch = wb % nch;
+  const int P = parent[c];
+  const double2* src = Ap + (size_t)P * 2 * nkc;
+  const double2* e = ph + (size_t)oct[c] * nkc;
+  const int ip0 = win[2 * ch], nw = win[2 * ch + 1] * nthc;
+  for (int i = threadIdx.x; i < nw; i += blockDim.x) {
+    const int q = i / nthc;
+    int ip = ip0 + q;
+    if (ip >= nphc) ip -= nphc;
+    const int r = ip * nthc + (i - q * nthc);
+    const double2 pr = e[r];
+    lds[i] = cmulc(pr, src[r]);
+    lds[nw + i] = cmulc(pr, src[nkc + r]);
+  }
+  __syncthreads();
+  const int s = ch * 256 + threadIdx.x;
+  if (s >= nkf) return;
+  double2 a0 = make_double2(0.0, 0.0), a1 = make_double2(0.0, 0.0);
+  const int q0 = wt_ptr[s / kWave], cnt = (wt_ptr[s / kWave + 1] - q0) / kWave;
+  const WtEntry* tq = wtw + q0 + s % kWave;
+  int q = 0;
+  for (; q + 4 <= cnt; q += 4) {
+    WtEntry t[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) t[u] = tq[(q + u) * kWave];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+      const double2 v0 = lds[t[u].r], v1 = lds[nw + t[u].r];
+      a0.x += t[u].w * v0.x;
+      a0.y += t[u].w * v0.y;
+      a1.x += t[u].w * v1.x;
+      a1.y += t[u].w * v1.y;
+    }
+  }
+  for (; q < cnt; ++q) {
+    const WtEntry t = tq[q * kWave];
+    a0.x += t.w * lds[t.r].x;
+    a0.y += t.w * lds[t.r].y;
+    a1.x += t.w * lds[nw + t.r].x;
+    a1.y += t.w * lds[nw + t.r].y;
+  }
+  const size_t o0 = ((size_t)c * 2 + 0) * nkf + s, o1 = ((size_t)c * 2 + 1) * nkf + s;
+  Af[o0] = cadd(Bf[o0], a0);
+  Af[o1] = cadd(Bf[o1], a1);
+}
+
 // Receive (modMLFMA.f90:1915-1933, funcREC = conjg(funcRAD) under EFIE) and
 // near-field CSR product (:1935-1942), one wavefront per row.
 template <typename V>
@@ -914,6 +976,10 @@ int run_matvec(Handle* h, const double2* x, double2* y, double2* y_near, hipEven
     if (2 * p.nk * (int)sizeof(double2) <= kDownLdsBytes)
       hipLaunchKernelGGL(k_interp_down_lds, dim3(c.nb), dim3(c.nk <= 256 ? 256 : 512), 2 * p.nk * sizeof(double2), s,
                          p.nk, c.nk, c.parent, c.oct, p.ph, p.wt_ptr, p.wt, src, c.B, c.A);
+    else if (p.win_lds)
+      hipLaunchKernelGGL(k_interp_down_win, dim3(c.nb * blocks_for(c.nk, 256)), dim3(256), p.win_lds, s, p.nk, c.nk,
+                         h->plan.lv[n - 1].nth, h->plan.lv[n - 1].nph, c.parent, c.oct, p.ph, p.win, p.wt_ptr, p.wtw,
+                         src, c.B, c.A);
     else
       hipLaunchKernelGGL(k_interp_down, dim3(c.nb * blocks_for(c.nk, 256)), dim3(256), 0, s, p.nk, c.nk,
                          c.parent, c.oct, p.ph, p.wt_ptr, p.wt, src, c.B, c.A);
@@ -1183,6 +1249,49 @@ static int init_device_state(Handle* h, const Geometry& g) {
             wt[off[q / kWave] + (e - lp.wt_ptr[q]) * kWave + q % kWave] = WtEntry{lp.wt_w[e], lp.wt_row[e], 0};
         TRY(upload(h, &d.wt_ptr, off));
         TRY(upload(h, &d.wt, wt));
+        if (2 * lp.nk * (int)sizeof(double2) > kDownLdsBytes) {
+          // LDS windows for the downward pass: per chunk of 256 fine
+          // samples, the shortest cyclic range of phi columns holding every
+          // row its stencils read (the complement of the longest cyclic run
+          // of unread columns), and the rows renumbered within it (padding
+          // entries, weight 0, read window row 0)
+          const int nch = (nkf + 255) / 256, nth = lp.nth, nph = lp.nph;
+          std::vector<int> win(2 * (size_t)nch);
+          std::vector<WtEntry> wtw(wt);
+          int maxc = 0;
+          for (int ch = 0; ch < nch; ++ch) {
+            const int q1 = std::min(nkf, (ch + 1) * 256);
+            std::vector<char> used(nph, 0);
+            for (int q = ch * 256; q < q1; ++q)
+              for (int e = lp.wt_ptr[q]; e < lp.wt_ptr[q + 1]; ++e) used[lp.wt_row[e] / nth] = 1;
+            int best = 0, bstart = 0, run = 0;
+            for (int j = 0; j < 2 * nph; ++j) {
+              run = used[j % nph] ? 0 : run + 1;
+              if (run > best && run <= nph) {
+                best = run;
+                bstart = j - run + 1;
+              }
+            }
+            const int start = best >= nph ? 0 : (bstart + best) % nph, cnt = nph - std::min(best, nph);
+            win[2 * ch] = start;
+            win[2 * ch + 1] = cnt;
+            maxc = std::max(maxc, cnt);
+            for (int q = ch * 256; q < q1; ++q) {
+              const int base = off[q / kWave] + q % kWave, ne = lp.wt_ptr[q + 1] - lp.wt_ptr[q];
+              const int m = (off[q / kWave + 1] - off[q / kWave]) / kWave;
+              for (int u = 0; u < m; ++u) {
+                WtEntry& t = wtw[base + (size_t)u * kWave];
+                t.r = u < ne ? ((t.r / nth - start + nph) % nph) * nth + t.r % nth : 0;
+              }
+            }
+          }
+          const long long lds = 2LL * maxc * nth * (long long)sizeof(double2);
+          if (lds <= kDownLdsBytes) {
+            d.win_lds = (int)lds;
+            TRY(upload(h, &d.win, win));
+            TRY(upload(h, &d.wtw, wtw));
+          }
+        }
       }
       TRY(dmalloc(h, &d.ph, 8 * (size_t)lp.nk));
       hipLaunchKernelGGL(k_phase_table64, dim3(blocks_for(lp.nk, 256)), dim3(256), 0, h->stream, lp.nk, d.kap,

@@ -112,6 +112,13 @@ struct DevLevel {
   // wt[wt_ptr[s / 64] + 64 u + s % 64]), each block padded with zero weights
   int* wt_ptr = nullptr;      // (ceil(nk_fine / 64) + 1) block offsets
   WtEntry* wt = nullptr;
+  // the downward pass through LDS windows (levels whose field does not fit
+  // LDS whole): per chunk of 256 fine samples, the cyclic range of this
+  // level's phi columns its stencils read (first column, count), and the
+  // stencil with rows relative to the window; win_lds = 0 when not in use
+  int* win = nullptr;         // (2 * chunks)
+  WtEntry* wtw = nullptr;
+  int win_lds = 0;
   // exp(+j k khat_i . r_o) for the 8 child offsets r_o = (+-q,+-q,+-q),
   // q = lado/4, octant bits (x,y,z) = (4,2,1), set = positive side (n < L)
   double2* ph = nullptr;      // (8, nk)
