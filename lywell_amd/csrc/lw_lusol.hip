@@ -96,11 +96,10 @@ __device__ __forceinline__ void publish_row(unsigned* bits, int i) {
 // [jlu(i), ju(i)), U [ju(i), jlu(i+1)).
 //
 // Per row, a lane first waits for all of its terms' rows, then issues all
-// its x loads together, then sums them in term order: the waits, not the
-// loads, are serial.  The row numbers and term ranges of a wave's next 64 rows
-// are loaded at once (lane l holds those of the wave's l-th next row) and
-// broadcast when their turn comes; the next row's values and columns are
-// loaded while the current row waits.
+// its x loads together, then the next row's values and columns, then sums
+// its terms in order: the waits, not the loads, are serial.  The row numbers
+// and term ranges of a wave's next 64 rows are loaded at once (lane l holds
+// those of the wave's l-th next row) and broadcast when their turn comes.
 template <typename V, bool upper>
 __device__ __forceinline__ void sweep(int n, const double2* __restrict__ y, double2* __restrict__ x,
                                       const V* __restrict__ alu, const int* __restrict__ jlu,
@@ -118,6 +117,10 @@ __device__ __forceinline__ void sweep(int n, const double2* __restrict__ y, doub
     } else {
       m0 = m1 = 0;
     }
+    // use the values here: a register whose load may still be in flight
+    // after the branch that reloads it would make every row wait for all of
+    // its loads (the join cannot tell whether the reload ran)
+    asm volatile("" ::"v"(mi), "v"(m0), "v"(m1));
   };
   V a[kAhead];
   int c[kAhead];
@@ -126,17 +129,24 @@ __device__ __forceinline__ void sweep(int n, const double2* __restrict__ y, doub
     for (int u = 0; u < kAhead; ++u) {
       const int k = k0 + lane + u * kWave;
       const bool in = k < k1;
+      // unconditional loads (jlu(0) is the pad): a load inside a branch whose
+      // result the branch uses would be waited for on the spot
       a[u] = alu[in ? k : 0];
-      c[u] = in ? jlu[k] - 1 : -1;
+      const int j = jlu[in ? k : 0];
+      c[u] = in ? j - 1 : -1;
     }
   };
   // the row's own operands, which no other row writes: y(i) (forward) or
   // x(i) from the forward sweep and the inverted diagonal alu(i) (backward)
   double2 own = make_double2(0.0, 0.0);
   V dia{};
+  // r = 0 (past the wave's last row) loads row 1's: the next row's loads
+  // are unconditional, so that no wait for one of them stands for a branch
+  // that skipped them (such a wait would drain every load in flight)
   auto fetch_own = [&](int r) {
-    own = upper ? x[r - 1] : y[r - 1];
-    if (upper) dia = alu[r];
+    const int o = max(r, 1);
+    own = upper ? x[o - 1] : y[o - 1];
+    if (upper) dia = alu[o];
   };
   meta(0);
   int t = 0;  // this wave's row count
@@ -160,16 +170,17 @@ __device__ __forceinline__ void sweep(int n, const double2* __restrict__ y, doub
     }
     const double2 own_i = own;
     const V dia_i = dia;
-    if (in_) {
-      fetch(n0, n1);  // a, c now hold the next row's terms
-      fetch_own(in_);
-    }
 #pragma unroll
     for (int u = 0; u < kAhead; ++u)
       if (cn[u] >= 0) wait_row(bits, cn[u], lerr);
     double2 xv[kAhead];
 #pragma unroll
     for (int u = 0; u < kAhead; ++u) xv[u] = x[cn[u] >= 0 ? cn[u] : 0];
+    // the next row's loads go after the gathers: loads complete in order, so
+    // the sum below then waits for the gathers only, not for an HBM fetch
+    __builtin_amdgcn_sched_barrier(0);
+    fetch(n0, n1);  // a, c now hold the next row's terms (none past the last row)
+    fetch_own(in_);
     double2 s = make_double2(0.0, 0.0);
 #pragma unroll
     for (int u = 0; u < kAhead; ++u)
@@ -183,7 +194,8 @@ __device__ __forceinline__ void sweep(int n, const double2* __restrict__ y, doub
         const int k = kb + lane + u * kWave;
         const bool in = k < k1;
         ab[u] = alu[in ? k : 0];
-        cb[u] = in ? jlu[k] - 1 : -1;
+        const int j = jlu[in ? k : 0];
+        cb[u] = in ? j - 1 : -1;
       }
 #pragma unroll
       for (int u = 0; u < kAhead; ++u)
