@@ -43,6 +43,13 @@ __device__ __forceinline__ double2 cmul(double2 a, double2 b) {
 __device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ double2 to_d2(float2 v) { return make_double2((double)v.x, (double)v.y); }
 __device__ __forceinline__ double2 to_d2(double2 v) { return v; }
+// s + a x where the term exists, else s, without a branch: a branch would
+// let the compiler move the load of x into it, behind the next row's loads
+__device__ __forceinline__ double2 add_term(double2 s, double2 a, double2 x, bool in) {
+  const double2 p = cmul(a, x);
+  const double2 t = cadd(s, p);
+  return make_double2(in ? t.x : s.x, in ? t.y : s.y);
+}
 
 // Sum over the wavefront, every lane receiving it (all 64 lanes active):
 // each 16-lane DPP row sums its lanes (rotations by 8 and 4, then the quad
@@ -91,6 +98,15 @@ __device__ __forceinline__ void publish_row(unsigned* bits, int i) {
   __hip_atomic_fetch_or(&bits[i >> 5], 1u << (i & 31), __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_WORKGROUP);
 }
 
+// A row's prefetched operands.
+template <typename V>
+struct RowOps {
+  V a[kAhead];
+  int c[kAhead];
+  double2 own;
+  V dia;
+};
+
 // One sweep over the rows of `order`.  upper = false: the L sweep; true: the
 // U sweep with the inverted diagonal.  Term range of row i (1-based): L
 // [jlu(i), ju(i)), U [ju(i), jlu(i+1)).
@@ -122,69 +138,50 @@ __device__ __forceinline__ void sweep(int n, const double2* __restrict__ y, doub
     // its loads (the join cannot tell whether the reload ran)
     asm volatile("" ::"v"(mi), "v"(m0), "v"(m1));
   };
-  V a[kAhead];
-  int c[kAhead];
-  auto fetch = [&](int k0, int k1) {
+  // the row's operands: its values and columns (lane l: terms l, l+64, ...),
+  // its own y(i) (forward) or x(i) from the forward sweep and inverted
+  // diagonal alu(i) (backward), which no other row writes.  The loads are
+  // unconditional (jlu(0) is the pad; r = 0, past the wave's last row,
+  // loads row 1's): a wait for a load in a branch, or for one that a
+  // branch may have skipped, would drain every load in flight
+  auto fetch = [&](RowOps<V>& ro, int r, int k0, int k1) {
 #pragma unroll
     for (int u = 0; u < kAhead; ++u) {
       const int k = k0 + lane + u * kWave;
       const bool in = k < k1;
-      // unconditional loads (jlu(0) is the pad): a load inside a branch whose
-      // result the branch uses would be waited for on the spot
-      a[u] = alu[in ? k : 0];
+      ro.a[u] = alu[in ? k : 0];
       const int j = jlu[in ? k : 0];
-      c[u] = in ? j - 1 : -1;
+      ro.c[u] = in ? j - 1 : -1;
     }
-  };
-  // the row's own operands, which no other row writes: y(i) (forward) or
-  // x(i) from the forward sweep and the inverted diagonal alu(i) (backward)
-  double2 own = make_double2(0.0, 0.0);
-  V dia{};
-  // r = 0 (past the wave's last row) loads row 1's: the next row's loads
-  // are unconditional, so that no wait for one of them stands for a branch
-  // that skipped them (such a wait would drain every load in flight)
-  auto fetch_own = [&](int r) {
     const int o = max(r, 1);
-    own = upper ? x[o - 1] : y[o - 1];
-    if (upper) dia = alu[o];
+    ro.own = upper ? x[o - 1] : y[o - 1];
+    if (upper) ro.dia = alu[o];
   };
   meta(0);
   int t = 0;  // this wave's row count
   int i = __shfl(mi, 0, kWave), k0 = __shfl(m0, 0, kWave), k1 = __shfl(m1, 0, kWave);
-  if (i) {
-    fetch(k0, k1);
-    fetch_own(i);
-  }
-  while (i) {
-    // the wave's next row
+  // one row, whose operands are in cur, while the next row's load into nxt;
+  // false after the wave's last row.  The loop below alternates two operand
+  // sets, so that no register copy waits for a load in flight
+  auto step = [&](RowOps<V>& cur, RowOps<V>& nxt) __attribute__((always_inline)) -> bool {
     const int tn = t + 1;
     if (tn % kWave == 0) meta(tn);
     const int in_ = __shfl(mi, tn % kWave, kWave), n0 = __shfl(m0, tn % kWave, kWave),
               n1 = __shfl(m1, tn % kWave, kWave);
-    V an[kAhead];
-    int cn[kAhead];
-#pragma unroll
-    for (int u = 0; u < kAhead; ++u) {
-      an[u] = a[u];
-      cn[u] = c[u];
-    }
-    const double2 own_i = own;
-    const V dia_i = dia;
 #pragma unroll
     for (int u = 0; u < kAhead; ++u)
-      if (cn[u] >= 0) wait_row(bits, cn[u], lerr);
+      if (cur.c[u] >= 0) wait_row(bits, cur.c[u], lerr);
     double2 xv[kAhead];
 #pragma unroll
-    for (int u = 0; u < kAhead; ++u) xv[u] = x[cn[u] >= 0 ? cn[u] : 0];
+    for (int u = 0; u < kAhead; ++u) xv[u] = x[cur.c[u] >= 0 ? cur.c[u] : 0];
     // the next row's loads go after the gathers: loads complete in order, so
     // the sum below then waits for the gathers only, not for an HBM fetch
     __builtin_amdgcn_sched_barrier(0);
-    fetch(n0, n1);  // a, c now hold the next row's terms (none past the last row)
-    fetch_own(in_);
+    fetch(nxt, in_, n0, n1);
+    __builtin_amdgcn_sched_barrier(0);  // and all of them before the first wait
     double2 s = make_double2(0.0, 0.0);
 #pragma unroll
-    for (int u = 0; u < kAhead; ++u)
-      if (cn[u] >= 0) s = cadd(s, cmul(to_d2(an[u]), xv[u]));
+    for (int u = 0; u < kAhead; ++u) s = add_term(s, to_d2(cur.a[u]), xv[u], cur.c[u] >= 0);
     // rows longer than the prefetched terms, kAhead terms per lane at a time
     for (int kb = k0 + kAhead * kWave; kb < k1; kb += kAhead * kWave) {
       V ab[kAhead];
@@ -203,21 +200,25 @@ __device__ __forceinline__ void sweep(int n, const double2* __restrict__ y, doub
 #pragma unroll
       for (int u = 0; u < kAhead; ++u) xv[u] = x[cb[u] >= 0 ? cb[u] : 0];
 #pragma unroll
-      for (int u = 0; u < kAhead; ++u)
-        if (cb[u] >= 0) s = cadd(s, cmul(to_d2(ab[u]), xv[u]));
+      for (int u = 0; u < kAhead; ++u) s = add_term(s, to_d2(ab[u]), xv[u], cb[u] >= 0);
     }
     s = wave_sum(s);
     if (lane == 0) {
-      if (upper)
-        x[i - 1] = cmul(to_d2(dia_i), make_double2(own_i.x - s.x, own_i.y - s.y));
-      else
-        x[i - 1] = make_double2(own_i.x - s.x, own_i.y - s.y);
+      const double2 v = make_double2(cur.own.x - s.x, cur.own.y - s.y);
+      x[i - 1] = upper ? cmul(to_d2(cur.dia), v) : v;
       publish_row(bits, i - 1);
     }
     t = tn;
     i = in_;
     k0 = n0;
     k1 = n1;
+    return i != 0;
+  };
+  RowOps<V> ra, rb;
+  if (i) {
+    fetch(ra, i, k0, k1);
+    while (step(ra, rb) && step(rb, ra)) {
+    }
   }
 }
 
