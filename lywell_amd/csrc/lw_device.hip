@@ -259,7 +259,15 @@ __global__ void __launch_bounds__(256) k_interp_up(int nkc, int nkf, const int* 
   for (int c = child_ptr[P]; c < child_ptr[P + 1]; ++c) {
     __syncthreads();
     const double2* src = Cf + (size_t)c * 2 * nkf;
-    for (int e = threadIdx.x; e < 2 * nkf; e += blockDim.x) lds[e] = src[e];
+    // four loads per thread in flight before its stores (indices clamped:
+    // the excess threads rewrite the last element with its own value)
+    for (int e0 = threadIdx.x; e0 < 2 * nkf; e0 += 4 * blockDim.x) {
+      double2 v[4];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) v[u] = src[min(e0 + u * (int)blockDim.x, 2 * nkf - 1)];
+#pragma unroll
+      for (int u = 0; u < 4; ++u) lds[min(e0 + u * (int)blockDim.x, 2 * nkf - 1)] = v[u];
+    }
     __syncthreads();
     if (i < nkc) {
       double2 a0 = make_double2(0.0, 0.0), a1 = make_double2(0.0, 0.0);
@@ -350,10 +358,23 @@ __global__ void __launch_bounds__(512) k_interp_down_lds(int nkc, int nkf, const
   const int P = parent[c];
   const double2* src = Ap + (size_t)P * 2 * nkc;
   const double2* e = ph + (size_t)oct[c] * nkc;
-  for (int i = threadIdx.x; i < nkc; i += blockDim.x) {
-    const double2 pr = e[i];
-    lds[i] = cmulc(pr, src[i]);
-    lds[nkc + i] = cmulc(pr, src[nkc + i]);
+  // two samples per thread in flight before its stores (indices clamped:
+  // the excess threads rewrite the last sample with its own value)
+  for (int i0 = threadIdx.x; i0 < nkc; i0 += 2 * blockDim.x) {
+    double2 pr[2], v0[2], v1[2];
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      const int i = min(i0 + u * (int)blockDim.x, nkc - 1);
+      pr[u] = e[i];
+      v0[u] = src[i];
+      v1[u] = src[nkc + i];
+    }
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      const int i = min(i0 + u * (int)blockDim.x, nkc - 1);
+      lds[i] = cmulc(pr[u], v0[u]);
+      lds[nkc + i] = cmulc(pr[u], v1[u]);
+    }
   }
   __syncthreads();
   for (int s = threadIdx.x; s < nkf; s += blockDim.x) {
@@ -408,14 +429,27 @@ __global__ void __launch_bounds__(256) k_interp_down_win(int nkc, int nkf, int n
   const double2* src = Ap + (size_t)P * 2 * nkc;
   const double2* e = ph + (size_t)oct[c] * nkc;
   const int ip0 = win[2 * ch], nw = win[2 * ch + 1] * nthc;
-  for (int i = threadIdx.x; i < nw; i += blockDim.x) {
-    const int q = i / nthc;
-    int ip = ip0 + q;
-    if (ip >= nphc) ip -= nphc;
-    const int r = ip * nthc + (i - q * nthc);
-    const double2 pr = e[r];
-    lds[i] = cmulc(pr, src[r]);
-    lds[nw + i] = cmulc(pr, src[nkc + r]);
+  // two window samples per thread in flight before its stores (indices
+  // clamped: the excess threads rewrite the last sample with its own value)
+  for (int i0 = threadIdx.x; i0 < nw; i0 += 2 * blockDim.x) {
+    double2 pr[2], v0[2], v1[2];
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      const int i = min(i0 + u * (int)blockDim.x, nw - 1);
+      const int q = i / nthc;
+      int ip = ip0 + q;
+      if (ip >= nphc) ip -= nphc;
+      const int r = ip * nthc + (i - q * nthc);
+      pr[u] = e[r];
+      v0[u] = src[r];
+      v1[u] = src[nkc + r];
+    }
+#pragma unroll
+    for (int u = 0; u < 2; ++u) {
+      const int i = min(i0 + u * (int)blockDim.x, nw - 1);
+      lds[i] = cmulc(pr[u], v0[u]);
+      lds[nw + i] = cmulc(pr[u], v1[u]);
+    }
   }
   __syncthreads();
   const int s = ch * 256 + threadIdx.x;
@@ -540,7 +574,18 @@ __global__ void __launch_bounds__(256) k_near_box(const int* __restrict__ box_sl
   const int b = xcd_block(blockIdx.x, gridDim.x);
   const int s0 = box_slot[b], s1 = box_slot[b + 1];
   const int c0 = ia[slot_basis[s0]], nc = ia[slot_basis[s0] + 1] - c0;
-  for (int e = threadIdx.x; e < nc; e += blockDim.x) xs[e] = x[ja[c0 + e]];
+  // four columns per thread in flight (indices, then values) before its
+  // stores (clamped: the excess threads rewrite the last column's value)
+  for (int e0 = threadIdx.x; e0 < nc; e0 += 4 * blockDim.x) {
+    int col[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) col[u] = ja[c0 + min(e0 + u * (int)blockDim.x, nc - 1)];
+    double2 v[4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) v[u] = x[col[u]];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) xs[min(e0 + u * (int)blockDim.x, nc - 1)] = v[u];
+  }
   __syncthreads();
   const int lane = threadIdx.x % kWave;
   const int nw = blockDim.x / kWave;
