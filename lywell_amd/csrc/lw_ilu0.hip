@@ -87,7 +87,7 @@ struct IluCx<double> {
 // workgroups (a launch per level, so the rows of one level run on as many
 // CUs as it has rows; the kernel boundary orders the levels).
 template <typename S>
-__global__ void __launch_bounds__(1024) k_ilu0_factor(int l0, int l1, const int* __restrict__ lptr,
+__global__ void __launch_bounds__(kIluWaves * kWaveSize) k_ilu0_factor(int l0, int l1, const int* __restrict__ lptr,
                                                       const int* __restrict__ lrows, const int* __restrict__ jlu,
                                                       const int* __restrict__ ju, typename IluCx<S>::T* alu,
                                                       int maxrow, int* __restrict__ piv) {
@@ -114,16 +114,20 @@ __global__ void __launch_bounds__(1024) k_ilu0_factor(int l0, int l1, const int*
       // inverted diagonal and their run of its U part: all final, so they are
       // fetched ahead (bounds three steps, runs two steps), with clamped
       // addresses so that the loads are unconditional and stay in flight
-      // while the current step is applied.
-      int qa0 = 0, qa1 = 0, qb0 = 0, qb1 = 0, qc0 = 0, qc1 = 0;
-      T2 ba = {}, bb = {}, bc = {};
-      int cca[kIluChunk], ccb[kIluChunk], ccc[kIluChunk];
-      T2 uua[kIluChunk], uub[kIluChunk], uuc[kIluChunk];
-      auto meta = [&](int j, int& q0, int& q1, T2& b) {
+      // while the current step is applied.  Four stages rotate through a
+      // step unrolled four times: copying a stage whose loads are in flight
+      // would wait for them.
+      struct Stage {
+        int q0, q1;
+        T2 b;
+        int cc[kIluChunk];
+        T2 uu[kIluChunk];
+      };
+      auto meta = [&](int j, Stage& st) {
         const int jr = cols[min(j, pu - 1) - p0];
-        q0 = ju[jr];
-        q1 = jlu[jr + 1];
-        b = alu[jr];
+        st.q0 = ju[jr];
+        st.q1 = jlu[jr + 1];
+        st.b = alu[jr];
       };
       auto load = [&](int b0, int q1, int(&cc)[kIluChunk], T2(&uu)[kIluChunk]) {
 #pragma unroll
@@ -135,82 +139,83 @@ __global__ void __launch_bounds__(1024) k_ilu0_factor(int l0, int l1, const int*
           cc[u] = jj < q1 ? c : INT_MAX;
         }
       };
-      if (p0 < pu) {
-        meta(p0, qa0, qa1, ba);
-        load(qa0, qa1, cca, uua);
-        meta(p0 + 1, qb0, qb1, bb);
-        load(qb0, qb1, ccb, uub);
-        meta(p0 + 2, qc0, qc1, bc);
-      }
-      for (int j = p0; j < pu; ++j) {
-        int qd0, qd1;
-        T2 bd;
-        load(qc0, qc1, ccc, uuc);
-        meta(j + 3, qd0, qd1, bd);
+      // row ii's entries at the lanes' columns of one run of jrow's U part,
+      // less tl times those U entries
+      auto eliminate = [&](int lo0, double tr, double ti, const int(&cc)[kIluChunk], const T2(&uu)[kIluChunk]) {
+        if (cc[0] == INT_MAX) return;
+        // position of each of the lane's columns in row ii: interleaved
+        // branch-free lower-bound searches over the row's columns above
+        // jrow (a uniform number of halvings, so no lane waits on
+        // another's longer walk)
+        const int n0 = len - lo0;
+        int base[kIluChunk];
+#pragma unroll
+        for (int u = 0; u < kIluChunk; ++u) base[u] = lo0;
+        for (int nn = n0; nn > 1;) {
+          const int half = nn >> 1;
+#pragma unroll
+          for (int u = 0; u < kIluChunk; ++u) base[u] = cols[base[u] + half] < cc[u] ? base[u] + half : base[u];
+          nn -= half;
+        }
+#pragma unroll
+        for (int u = 0; u < kIluChunk; ++u) {
+          const int c = cc[u];
+          T2* w = nullptr;
+          if (c == ii) {
+            w = dslot;
+          } else if (c != INT_MAX && n0 > 0) {
+            const int pos = base[u] + (cols[base[u]] < c ? 1 : 0);
+            if (pos < len && cols[pos] == c) w = vals + pos;
+          }
+          if (w) {
+            const T2 wv = *w;
+            T2 o;
+            o.x = (S)((double)wv.x - (tr * (double)uu[u].x - ti * (double)uu[u].y));
+            o.y = (S)((double)wv.y - (tr * (double)uu[u].y + ti * (double)uu[u].x));
+            *w = o;
+          }
+        }
+      };
+      // step j: A holds its stage; C's runs and D's bounds are fetched
+      auto step = [&](int j, const Stage& A, Stage& C, Stage& D) __attribute__((always_inline)) {
+        load(C.q0, C.q1, C.cc, C.uu);
+        meta(j + 3, D);
         // tl = alu(j) * alu(jrow), in the storage precision
         const T2 a = vals[j - p0];
         T2 t;
-        t.x = a.x * ba.x - a.y * ba.y;
-        t.y = a.x * ba.y + a.y * ba.x;
+        t.x = a.x * A.b.x - a.y * A.b.y;
+        t.y = a.x * A.b.y + a.y * A.b.x;
         wave_lds_order();
         if (lane == 0) vals[j - p0] = t;
         const double tr = (double)t.x, ti = (double)t.y;
         // row ii's columns above jrow start after position j - p0
         const int lo0 = j - p0 + 1;
-        for (int b0 = qa0; b0 < qa1; b0 += kWaveSize * kIluChunk) {
-          if (b0 > qa0) load(b0, qa1, cca, uua);  // rows with more than 64*kIluChunk U entries
-          if (cca[0] != INT_MAX) {
-            // position of each of the lane's columns in row ii: interleaved
-            // branch-free lower-bound searches over the row's columns above
-            // jrow (a uniform number of halvings, so no lane waits on
-            // another's longer walk)
-            const int n0 = len - lo0;
-            int base[kIluChunk];
-#pragma unroll
-            for (int u = 0; u < kIluChunk; ++u) base[u] = lo0;
-            for (int nn = n0; nn > 1;) {
-              const int half = nn >> 1;
-#pragma unroll
-              for (int u = 0; u < kIluChunk; ++u) base[u] = cols[base[u] + half] < cca[u] ? base[u] + half : base[u];
-              nn -= half;
-            }
-#pragma unroll
-            for (int u = 0; u < kIluChunk; ++u) {
-              const int c = cca[u];
-              T2* w = nullptr;
-              if (c == ii) {
-                w = dslot;
-              } else if (c != INT_MAX && n0 > 0) {
-                const int pos = base[u] + (cols[base[u]] < c ? 1 : 0);
-                if (pos < len && cols[pos] == c) w = vals + pos;
-              }
-              if (w) {
-                const T2 wv = *w;
-                T2 o;
-                o.x = (S)((double)wv.x - (tr * (double)uua[u].x - ti * (double)uua[u].y));
-                o.y = (S)((double)wv.y - (tr * (double)uua[u].y + ti * (double)uua[u].x));
-                *w = o;
-              }
-            }
-          }
+        if (A.q0 < A.q1) eliminate(lo0, tr, ti, A.cc, A.uu);
+        // rows with more than 64*kIluChunk U entries
+        for (int b0 = A.q0 + kWaveSize * kIluChunk; b0 < A.q1; b0 += kWaveSize * kIluChunk) {
+          int cl[kIluChunk];
+          T2 ul[kIluChunk];
+          load(b0, A.q1, cl, ul);
+          eliminate(lo0, tr, ti, cl, ul);
         }
         wave_lds_order();
-        qa0 = qb0;
-        qa1 = qb1;
-        ba = bb;
-        qb0 = qc0;
-        qb1 = qc1;
-        bb = bc;
-        qc0 = qd0;
-        qc1 = qd1;
-        bc = bd;
-#pragma unroll
-        for (int u = 0; u < kIluChunk; ++u) {
-          cca[u] = ccb[u];
-          uua[u] = uub[u];
-          ccb[u] = ccc[u];
-          uub[u] = uuc[u];
-        }
+      };
+      Stage s0, s1, s2, s3;
+      if (p0 < pu) {
+        meta(p0, s0);
+        load(s0.q0, s0.q1, s0.cc, s0.uu);
+        meta(p0 + 1, s1);
+        load(s1.q0, s1.q1, s1.cc, s1.uu);
+        meta(p0 + 2, s2);
+      }
+      for (int j = p0; j < pu; j += 4) {
+        step(j, s0, s2, s3);
+        if (j + 1 >= pu) break;
+        step(j + 1, s1, s3, s0);
+        if (j + 2 >= pu) break;
+        step(j + 2, s2, s0, s1);
+        if (j + 3 >= pu) break;
+        step(j + 3, s3, s1, s2);
       }
       if (lane == 0) {
         // 1/d with Smith's scaling, in double
